@@ -1,0 +1,327 @@
+"""ctypes binding of the C ABI declared in include/coldplate_hip.h.
+
+The structures below mirror the C structs field for field; the library is
+``coldplate-solver_amd/lib/libcoldplate_hip.so`` built by ``build.py``.
+Loading fails loudly when the library is missing: there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "lib" / "libcoldplate_hip.so"
+
+CP_OK = 0
+CP_PREC_FP64 = 0
+CP_PREC_MIXED = 1
+
+F_U, F_V, F_P, F_USTAR, F_VSTAR, F_PSTAR, F_DE, F_DN, F_GAMMA, F_ALPHA = range(10)
+
+SLOT_FREE, SLOT_RUNNING, SLOT_CONVERGED, SLOT_MAXITER, SLOT_FAILED = range(5)
+
+
+class Params(C.Structure):
+    """cp_params (include/coldplate_hip.h)."""
+
+    _fields_ = [
+        ("M", C.c_int32), ("N", C.c_int32),
+        ("hx", C.c_double), ("hy", C.c_double),
+        ("rho", C.c_double), ("mu", C.c_double),
+        ("U_in", C.c_double),
+        ("enableInletRamp", C.c_int32), ("rampSteps", C.c_int32),
+        ("uvAlpha", C.c_double), ("pAlpha", C.c_double),
+        ("convectionScheme", C.c_int32),
+        ("enableTwoPointFiveD", C.c_int32), ("enableConvectionScaling", C.c_int32),
+        ("Ht", C.c_double), ("sinkMultiplier", C.c_double),
+        ("alphaMax", C.c_double), ("KMin", C.c_double), ("brinkmanQ", C.c_double),
+        ("enablePseudoTime", C.c_int32), ("useLocalPseudoTime", C.c_int32),
+        ("timeStepMultiplier", C.c_double), ("minPseudoSpeed", C.c_double),
+        ("enableCflRamp", C.c_int32),
+        ("pseudoCFLInitial", C.c_double), ("pseudoCFLMax", C.c_double),
+        ("cflRampStartRes", C.c_double), ("cflRampExponent", C.c_double),
+        ("cflRampSmooth", C.c_double),
+        ("enableSER", C.c_int32), ("serMinIter", C.c_int32), ("serUseMaxResid", C.c_int32),
+        ("serCFLMin", C.c_double), ("serCFLMax", C.c_double), ("serSmooth", C.c_double),
+        ("serMaxIncrease", C.c_double), ("serMaxDecrease", C.c_double),
+        ("enableLineSearch", C.c_int32), ("lsMaxTries", C.c_int32),
+        ("lsAlphaMin", C.c_double), ("lsAlphaReduce", C.c_double),
+        ("lsResidIncreaseTol", C.c_double),
+        ("momentumTol", C.c_double), ("maxMomentumIter", C.c_int32),
+        ("pTol", C.c_double), ("maxPressureIter", C.c_int32),
+        ("epsilon", C.c_double), ("maxIterations", C.c_int32),
+        ("precision", C.c_int32),
+        ("refineCycles", C.c_int32),
+        ("fixedMomentumSweeps", C.c_int32),
+        ("massResidualScope", C.c_int32),
+    ]
+
+    def copy(self) -> "Params":
+        q = Params()
+        C.memmove(C.byref(q), C.byref(self), C.sizeof(Params))
+        return q
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class StepStats(C.Structure):
+    """cp_step_stats."""
+
+    _fields_ = [
+        ("error", C.c_double), ("residU", C.c_double), ("residV", C.c_double),
+        ("residMass", C.c_double), ("pseudoCFL", C.c_double),
+        ("itU", C.c_int32), ("itV", C.c_int32), ("itP", C.c_int32),
+        ("iter", C.c_int32), ("converged", C.c_int32), ("health", C.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class PlaneMetrics(C.Structure):
+    """cp_plane_metrics (PlaneMetrics, SIMPLE.h:679-686)."""
+
+    _fields_ = [
+        ("flowArea", C.c_double), ("massFlux", C.c_double), ("avgStatic", C.c_double),
+        ("avgDynamic", C.c_double), ("avgTotal", C.c_double), ("valid", C.c_int32),
+    ]
+
+
+class SlotInfo(C.Structure):
+    """cp_slot_info."""
+
+    _fields_ = [
+        ("status", C.c_int32), ("iter", C.c_int32),
+        ("error", C.c_double), ("residU", C.c_double), ("residV", C.c_double),
+        ("residMass", C.c_double), ("health", C.c_int32), ("M", C.c_int32), ("N", C.c_int32),
+    ]
+
+
+def field_shape(which: int, M: int, N: int) -> tuple[int, int]:
+    """Reference shape of a CP_F_* field (SIMPLE.cpp:184-201)."""
+    if which in (F_U, F_USTAR, F_DE):
+        return (M + 1, N)
+    if which in (F_V, F_VSTAR, F_DN):
+        return (M, N + 1)
+    if which in (F_P, F_PSTAR):
+        return (M + 1, N + 1)
+    return (M, N)
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_vp = C.c_void_p
+
+
+def dptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+# (name, restype, argtypes) of every exported symbol of include/coldplate_hip.h.
+EXPORTS = [
+    ("cp_params_default", C.c_int, [C.POINTER(Params)]),
+    ("cp_last_error", C.c_char_p, [_vp]),
+    ("cp_device_count", C.c_int, []),
+    ("cp_create", C.c_int, [C.POINTER(Params), _dp, C.POINTER(_vp)]),
+    ("cp_destroy", C.c_int, [_vp]),
+    ("cp_set_fields", C.c_int, [_vp, _dp, _dp, _dp]),
+    ("cp_get_fields", C.c_int, [_vp, _dp, _dp, _dp]),
+    ("cp_get_field", C.c_int, [_vp, C.c_int, _dp]),
+    ("cp_momentum", C.c_int, [_vp, C.POINTER(StepStats)]),
+    ("cp_pressure", C.c_int, [_vp, C.POINTER(StepStats)]),
+    ("cp_correct", C.c_int, [_vp]),
+    ("cp_step", C.c_int, [_vp, C.POINTER(StepStats)]),
+    ("cp_run", C.c_int, [_vp, C.c_int, C.POINTER(StepStats)]),
+    ("cp_plane", C.c_int, [_vp, C.c_double, C.c_double, C.POINTER(PlaneMetrics)]),
+    ("cp_sync", C.c_int, [_vp]),
+    ("cp_batch_create", C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(_vp)]),
+    ("cp_batch_destroy", C.c_int, [_vp]),
+    ("cp_batch_footprint", C.c_size_t, [_vp, C.c_int, C.c_int]),
+    ("cp_batch_admit", C.c_int, [_vp, C.POINTER(Params), _dp, C.POINTER(C.c_int)]),
+    ("cp_batch_run", C.c_int, [_vp, C.c_int]),
+    ("cp_batch_poll", C.c_int, [_vp, C.POINTER(SlotInfo)]),
+    ("cp_batch_get_fields", C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
+    ("cp_batch_plane", C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.POINTER(PlaneMetrics)]),
+    ("cp_batch_evict", C.c_int, [_vp, C.c_int]),
+    ("cp_geom_straight", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
+    ("cp_geom_network", C.c_int, [C.c_int, C.c_int, C.c_uint64, _dp]),
+    ("cp_geom_gamma", C.c_int, [C.c_int, C.c_int, C.c_uint64, C.c_double, C.c_double, C.c_double, _dp]),
+]
+
+_lib = None
+
+
+def load(path: str | os.PathLike | None = None) -> C.CDLL:
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"{p} is missing: build the HIP library first (python __graft_entry__.py build)")
+    lib = C.CDLL(str(p))
+    for name, res, args in EXPORTS:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+class CpError(RuntimeError):
+    pass
+
+
+def check(rc: int, ctx=None) -> None:
+    if rc != CP_OK:
+        msg = load().cp_last_error(ctx)
+        raise CpError(f"cp call failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def default_params() -> Params:
+    p = Params()
+    load().cp_params_default(C.byref(p))
+    return p
+
+
+class Solver:
+    """Single-geometry device solve (cp_ctx)."""
+
+    def __init__(self, prm: Params, cell_type: np.ndarray):
+        lib = load()
+        ct = np.ascontiguousarray(cell_type, dtype=np.float64)
+        assert ct.shape == (prm.M, prm.N)
+        self.prm = prm.copy()
+        self.M, self.N = prm.M, prm.N
+        h = _vp()
+        check(lib.cp_create(C.byref(self.prm), dptr(ct), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().cp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _stats(self, fn) -> dict:
+        st = StepStats()
+        check(fn(self.h, C.byref(st)), self.h)
+        return st.as_dict()
+
+    def momentum(self) -> dict:
+        return self._stats(load().cp_momentum)
+
+    def pressure(self) -> dict:
+        return self._stats(load().cp_pressure)
+
+    def correct(self) -> None:
+        check(load().cp_correct(self.h), self.h)
+
+    def step(self) -> dict:
+        return self._stats(load().cp_step)
+
+    def run(self, check_every: int = 0) -> dict:
+        st = StepStats()
+        check(load().cp_run(self.h, check_every, C.byref(st)), self.h)
+        return st.as_dict()
+
+    def sync(self) -> None:
+        check(load().cp_sync(self.h), self.h)
+
+    def field(self, which: int) -> np.ndarray:
+        out = np.empty(field_shape(which, self.M, self.N), dtype=np.float64)
+        check(load().cp_get_field(self.h, which, dptr(out)), self.h)
+        return out
+
+    def set_fields(self, u=None, v=None, p=None) -> None:
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, p)]
+        check(load().cp_set_fields(self.h, *[dptr(a) for a in arrs]), self.h)
+
+    def plane(self, x: float, gamma_min: float = 1.0 - 1e-6) -> dict:
+        m = PlaneMetrics()
+        check(load().cp_plane(self.h, x, gamma_min, C.byref(m)), self.h)
+        return {k: getattr(m, k) for k, _ in m._fields_}
+
+    def full_dp(self, gamma_min: float = 1.0 - 1e-6) -> float:
+        """Full-system total-pressure drop (SIMPLE.cpp:343-349)."""
+        a = self.plane(0.0, gamma_min)
+        b = self.plane(self.N * self.prm.hx, gamma_min)
+        return a["avgTotal"] - b["avgTotal"]
+
+
+class Batch:
+    """Packed multi-geometry device solve (cp_batch)."""
+
+    def __init__(self, precision: int, hbm_budget: int, max_slots: int):
+        lib = load()
+        h = _vp()
+        check(lib.cp_batch_create(precision, hbm_budget, max_slots, C.byref(h)))
+        self.h = h
+        self.max_slots = max_slots
+        self.shapes: dict[int, tuple[int, int, float]] = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().cp_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def footprint(self, M: int, N: int) -> int:
+        return int(load().cp_batch_footprint(self.h, M, N))
+
+    def admit(self, prm: Params, cell_type: np.ndarray) -> int | None:
+        """Returns the slot, or None when the batch is full."""
+        ct = np.ascontiguousarray(cell_type, dtype=np.float64)
+        s = C.c_int(-1)
+        rc = load().cp_batch_admit(self.h, C.byref(prm), dptr(ct), C.byref(s))
+        if rc == -5:
+            return None
+        check(rc, self.h)
+        self.shapes[s.value] = (prm.M, prm.N, prm.hx)
+        return s.value
+
+    def run(self, steps: int) -> None:
+        check(load().cp_batch_run(self.h, steps), self.h)
+
+    def poll(self) -> list[dict]:
+        info = (SlotInfo * self.max_slots)()
+        check(load().cp_batch_poll(self.h, info), self.h)
+        return [{k: getattr(x, k) for k, _ in x._fields_} for x in info]
+
+    def fields(self, slot: int):
+        M, N, _ = self.shapes[slot]
+        u = np.empty((M + 1, N)); v = np.empty((M, N + 1)); p = np.empty((M + 1, N + 1))
+        check(load().cp_batch_get_fields(self.h, slot, dptr(u), dptr(v), dptr(p)), self.h)
+        return u, v, p
+
+    def plane(self, slot: int, x: float, gamma_min: float = 1.0 - 1e-6) -> dict:
+        m = PlaneMetrics()
+        check(load().cp_batch_plane(self.h, slot, x, gamma_min, C.byref(m)), self.h)
+        return {k: getattr(m, k) for k, _ in m._fields_}
+
+    def full_dp(self, slot: int, gamma_min: float = 1.0 - 1e-6) -> float:
+        M, N, hx = self.shapes[slot]
+        return self.plane(slot, 0.0, gamma_min)["avgTotal"] - self.plane(slot, N * hx, gamma_min)["avgTotal"]
+
+    def evict(self, slot: int) -> None:
+        check(load().cp_batch_evict(self.h, slot), self.h)
+        self.shapes.pop(slot, None)

@@ -1,0 +1,115 @@
+// cp_device.h — device-side data layout shared by the kernels and the host
+// engine of the gfx950 SIMPLE backend.
+//
+// Every geometry ("slot") lives in one arena region of the batch. All of its
+// fields are stored on one node grid of (M+1) rows x pitch columns,
+// row-major, pitch >= N+1 and a multiple of 32 elements. A node (i, j) holds
+//   u(i,j)  [valid for i<=M,   j<=N-1]     v(i,j)  [i<=M-1, j<=N]
+//   p(i,j), p'(i,j) [i<=M, j<=N]
+//   gam(i,j) = gamma(i-1, j-1)  (cell-centred fields moved onto the p index)
+// so the staggered stencils of the reference (SIMPLE.h:17-31) become fixed
+// offsets on one grid. Kernels cover the grid with 64 x 4 node tiles; one
+// workgroup of 256 threads (4 wave64) processes one tile per visit, each wave
+// one 64-node row segment.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cpdev {
+
+constexpr int kTileW = 64;
+constexpr int kTileH = 4;
+constexpr int kBlock = kTileW * kTileH;
+constexpr int kNumPartials = 4;  // per-tile reduction partials (doubles)
+
+// slot status
+constexpr int ST_RUNNING = 1;
+constexpr int ST_CONVERGED = 2;
+constexpr int ST_MAXITER = 3;
+constexpr int ST_FAILED = 4;
+
+// Solver-typed arrays (T = double in FP64 mode, float in mixed mode).
+enum MomArr { M_AP = 0, M_AE, M_AW, M_AN, M_AS, M_R0, M_D0, M_D1, M_COUNT };
+enum PresArr { P_CE = 0, P_CN, P_AP, P_R, P_PK0, P_PK1, P_Q, P_X, P_COUNT };
+
+// Read-only description of one slot; uploaded when a slot is admitted.
+struct alignas(64) SlotDesc {
+    int M, N, pitch;
+    int tile0, ntiles, tilesX;
+    int nU, nV;                  // momentum DOF counts (Jacobi stop test)
+    int convScheme, pseudo, localPseudo;
+    int maxMomIter, maxPresIter, refineCycles, massScope;
+    int maxIterations, inletRamp, rampSteps;
+    int enableSER, serMinIter, serUseMax, enableLS, lsMaxTries, enableRamp;
+    int mixed;
+    double hx, hy, rho, vol, De, Dn, convScale, sinkDiag, uvAlpha, pAlpha, hChar;
+    double timeStep, minSpeed, momTol, pTol, epsilon, targetVel;
+    double serCFLMin, serCFLMax, serSmooth, serMaxInc, serMaxDec;
+    double lsAlphaMin, lsAlphaReduce, lsTol;
+    double cflInit, cflMax, rampStart, rampExp, rampSmooth;
+    // state arrays (double)
+    double *u, *v, *p, *us, *vs, *pc, *dE, *dN, *aU, *aV, *gam, *bm;
+    // solver arrays (T)
+    void* mu[M_COUNT];
+    void* mv[M_COUNT];
+    void* pr[P_COUNT];
+};
+
+// Mutable per-slot scalars; written only by the finalising tile of a slot
+// (or by the host between launches).
+struct alignas(256) SlotState {
+    int status;
+    int health;
+    int iter;
+    int jk[2], jdone[2], jiters[2];
+    int pk, pdone, piters, pcycle, pitTotal;
+    int itU, itV, itP;
+    int stepPhase;
+    double rz, alpha, beta, rr;
+    double residU, residV, residMass, error;
+    double inletVelocity, pseudoCFL, serResidPrev, lsAlpha;
+    int lsBacktrack;
+    int pad;
+};
+
+// Plane-sample request/result (samplePlaneAtX).
+struct PlaneReq {
+    int slot;
+    double x;
+    double gammaMin;
+};
+struct PlaneOut {
+    double flowArea, massFlux, avgStatic, avgDynamic, avgTotal;
+    int valid;
+};
+
+// Kernel arguments shared by every tile kernel.
+struct KArgs {
+    const SlotDesc* desc;
+    SlotState* state;
+    const int* tileSlot;
+    const int* nTiles;
+    double* partials;
+    unsigned* counters;
+};
+
+// Kernels of one SIMPLE step, in launch order.
+enum Kern {
+    K_MOM_ASM = 0,  // momentum assembly (U and V)
+    K_JAC,          // one Jacobi sweep (U and V)
+    K_MOM_FIN,      // u*, v* scatter + velocity BCs
+    K_PRES_ASM,     // mass residual, p' coefficients, PCG start
+    K_PCG_Q,        // PCG: p update + q = A p
+    K_PCG_U,        // PCG: x, r update + stop test
+    K_REF_ACC,      // mixed: p' += x
+    K_REF_RES,      // mixed: FP64 residual of p'
+    K_CORR,         // p update, velocity correction, BCs
+    K_CORR_OUTER,   // K_CORR + outer-loop scalar logic
+};
+
+// Host-side launchers (defined next to the kernels).
+hipError_t launchKernel(Kern k, bool f32, int grid, hipStream_t s, const KArgs& a);
+hipError_t launchPlane(int nreq, hipStream_t s, const SlotDesc* desc, const PlaneReq* req, PlaneOut* out);
+
+}  // namespace cpdev

@@ -1,0 +1,876 @@
+// cp_kernels.hip — gfx950 kernels of the SIMPLE backend.
+//
+// Every kernel is a persistent grid-stride loop over the batch's tile list
+// (tile -> slot table), so one launch covers every running geometry of a
+// batch, and the launch shape never depends on which slots are admitted
+// (the sequence can be captured once in a hipGraph). Per-slot reductions are
+// deterministic: each tile writes its block-reduced partials, and the tile
+// that arrives last for its slot sums that slot's partials in tile order and
+// then runs the slot's scalar logic (stop tests, CFL control). The reduction
+// order depends only on the slot's own shape, so a geometry gives the same
+// result alone or packed with others.
+//
+// T is the solver type: double (FP64 mode) or float (mixed mode). State
+// arrays (u, v, p, u*, v*, p', dE, dN, alpha faces, gamma, mass imbalance)
+// are double in both modes; reductions always accumulate in double.
+#include <hip/hip_runtime.h>
+
+#include "cp_device.h"
+
+namespace cpdev {
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+// max that propagates NaN (a non-finite residual must reach the health check)
+__device__ __forceinline__ double nanmax(double a, double b) { return (a != a || a > b) ? a : b; }
+
+template <bool IsMax>
+__device__ __forceinline__ double combine(double a, double b) {
+    if constexpr (IsMax) return nanmax(a, b);
+    else return a + b;
+}
+
+template <bool IsMax>
+__device__ __forceinline__ double waveReduce(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = combine<IsMax>(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Reduces NQ values over the 256-thread block; MaxMask bit q selects max for
+// value q (else sum). Thread 0 writes the results to dst[0..NQ).
+template <int NQ, unsigned MaxMask>
+__device__ __forceinline__ void blockReduceTo(double (&v)[NQ], double* dst) {
+    __shared__ double sh[kBlock / 64][4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+        v[q] = ((MaxMask >> q) & 1u) ? waveReduce<true>(v[q]) : waveReduce<false>(v[q]);
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) sh[w][q] = v[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double r = sh[0][q];
+            for (int k = 1; k < kBlock / 64; ++k)
+                r = ((MaxMask >> q) & 1u) ? nanmax(r, sh[k][q]) : r + sh[k][q];
+            dst[q] = r;
+        }
+    }
+}
+
+// Tile arrival: returns true (in every thread) for the last tile of slot s.
+__device__ __forceinline__ bool arriveLast(unsigned* ctr, int ntiles) {
+    __shared__ int isLast;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned prev = atomicAdd(ctr, 1u);
+        isLast = (prev == unsigned(ntiles - 1)) ? 1 : 0;
+    }
+    __syncthreads();
+    const bool last = isLast != 0;
+    if (last) __threadfence();
+    return last;
+}
+
+__device__ __forceinline__ void resetCounter(unsigned* ctr) {
+    if (threadIdx.x == 0) atomicExch(ctr, 0u);
+}
+
+// Sums (or maxes) the partials of one slot's tiles in tile order; the result
+// is valid in every thread of the block.
+template <int NQ, unsigned MaxMask>
+__device__ __forceinline__ void slotReduce(const double* partials, int tile0, int ntiles, double (&out)[NQ]) {
+    __shared__ double res[4];
+    double v[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) v[q] = 0.0;
+    for (int k = threadIdx.x; k < ntiles; k += kBlock) {
+        const double* src = partials + size_t(tile0 + k) * kNumPartials;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const double x = __hip_atomic_load(src + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            v[q] = ((MaxMask >> q) & 1u) ? nanmax(v[q], x) : v[q] + x;
+        }
+    }
+    __syncthreads();
+    blockReduceTo<NQ, MaxMask>(v, res);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) out[q] = res[q];
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// tile decoding
+// ---------------------------------------------------------------------------
+struct Tile {
+    int s, i, j;
+};
+
+__device__ __forceinline__ Tile decodeTile(const KArgs& a, int t) {
+    Tile tl;
+    tl.s = a.tileSlot[t];
+    const SlotDesc& d = a.desc[tl.s];
+    const int lt = t - d.tile0;
+    const int ty = lt / d.tilesX;
+    const int tx = lt - ty * d.tilesX;
+    tl.i = ty * kTileH + int(threadIdx.x >> 6);
+    tl.j = tx * kTileW + int(threadIdx.x & 63);
+    return tl;
+}
+
+__device__ __forceinline__ double clampv(double x, double m) { return fmax(-m, fmin(m, x)); }
+
+// ---------------------------------------------------------------------------
+// momentum assembly (momentum_solver.cpp:185-298, delta form)
+// ---------------------------------------------------------------------------
+// Second-order upwind deferred correction (convection.cpp:53-186). The index
+// guards are the reference's, including its U/V asymmetry.
+template <bool IsU>
+__device__ __forceinline__ double souCorrection(const double* f, size_t k, int i, int j, int P, int M, int N,
+                                                double Fe, double Fw, double Fn, double Fs) {
+    double S = 0.0;
+    const bool eOK = IsU ? (j + 2 < N) : (j + 2 <= N);
+    const bool wOK = IsU ? (j + 1 < N) : (j + 1 <= N);
+    const bool nOK = IsU ? (i + 2 <= M) : (i + 2 < M);
+    const bool sOK = IsU ? (i + 1 <= M) : (i + 1 < M);
+    if (Fe >= 0.0) {
+        if (j >= 2) S -= Fe * ((1.5 * f[k] - 0.5 * f[k - 1]) - f[k]);
+    } else {
+        if (eOK) S -= Fe * ((1.5 * f[k + 1] - 0.5 * f[k + 2]) - f[k + 1]);
+    }
+    if (Fw >= 0.0) {
+        if (j >= 2) S += Fw * ((1.5 * f[k - 1] - 0.5 * f[k - 2]) - f[k - 1]);
+    } else {
+        if (wOK) S += Fw * ((1.5 * f[k] - 0.5 * f[k + 1]) - f[k]);
+    }
+    if (Fn >= 0.0) {
+        if (i >= 1) S -= Fn * ((1.5 * f[k] - 0.5 * f[k - P]) - f[k]);
+    } else {
+        if (nOK) S -= Fn * ((1.5 * f[k + P] - 0.5 * f[k + 2 * size_t(P)]) - f[k + P]);
+    }
+    if (Fs >= 0.0) {
+        if (i >= 2) S += Fs * ((1.5 * f[k - P] - 0.5 * f[k - 2 * size_t(P)]) - f[k - P]);
+    } else {
+        if (sOK) S += Fs * ((1.5 * f[k] - 0.5 * f[k + P]) - f[k]);
+    }
+    return S;
+}
+
+// Assembles one U (IsU) or V row at node k. Writes the relaxed diagonal, the
+// four neighbour coefficients, dE/dN, the initial residual r0 = b - A u of
+// the Jacobi system (so the sweeps iterate on the correction delta = x - u
+// from delta = 0, which is the reference's x0 = u), and delta = 0.
+template <typename T, bool IsU>
+__device__ __forceinline__ void momRow(const SlotDesc& d, size_t k, int i, int j, double maxVel, double cfl) {
+    const int P = d.pitch;
+    const double* u = d.u;
+    const double* v = d.v;
+    double ue, uw, vn, vs;
+    if (IsU) {
+        ue = clampv(0.5 * (u[k] + u[k + 1]), maxVel);
+        uw = clampv(0.5 * (u[k - 1] + u[k]), maxVel);
+        vn = clampv(0.5 * (v[k] + v[k + 1]), maxVel);
+        vs = clampv(0.5 * (v[k - P] + v[k - P + 1]), maxVel);
+    } else {
+        ue = clampv(0.5 * (u[k] + u[k + P]), maxVel);
+        uw = clampv(0.5 * (u[k - 1] + u[k + P - 1]), maxVel);
+        vn = clampv(0.5 * (v[k] + v[k + P]), maxVel);
+        vs = clampv(0.5 * (v[k - P] + v[k]), maxVel);
+    }
+    const double cs = d.convScale;
+    const double Fe = cs * d.rho * d.hy * ue, Fw = cs * d.rho * d.hy * uw;
+    const double Fn = cs * d.rho * d.hx * vn, Fs = cs * d.rho * d.hx * vs;
+    const double aE = d.De + fmax(0.0, -Fe);
+    const double aW = d.De + fmax(0.0, Fw);
+    const double aN = d.Dn + fmax(0.0, -Fn);
+    const double aS = d.Dn + fmax(0.0, Fs);
+    const double sumA = aE + aW + aN + aS;
+    const double* f = IsU ? u : v;
+    const double cur = f[k];
+    double trans = 0.0;
+    if (d.pseudo) {
+        double dt = fmax(d.timeStep, 1e-20);
+        if (d.localPseudo) {
+            const double speed = fmax(fabs(cur), d.minSpeed);
+            dt = fmin(dt, cfl * d.hChar / speed);
+            dt = fmax(dt, 1e-20);
+        }
+        trans = d.rho * d.vol / dt;
+    }
+    const double alphaLocal = IsU ? d.aU[k] : d.aV[k];
+    double aP0 = sumA + (Fe - Fw + Fn - Fs) + trans + d.sinkDiag + alphaLocal * d.vol;
+    aP0 = fmax(aP0, 1e-10);
+    const double aP = aP0 / d.uvAlpha;
+    double Sp = IsU ? (d.p[k] - d.p[k + 1]) * d.hy : (d.p[k] - d.p[k + P]) * d.hx;
+    Sp += trans * cur;  // uOld == u at momentum time (iterations.cpp:118-119)
+    Sp += (1.0 - d.uvAlpha) / d.uvAlpha * aP0 * cur;
+    if (d.convScheme == 1) Sp += souCorrection<IsU>(f, k, i, j, P, d.M, d.N, Fe, Fw, Fn, Fs);
+    if (IsU) d.dE[k] = d.hy / aP0;
+    else d.dN[k] = d.hx / aP0;
+    const double r0 = Sp + aE * f[k + 1] + aW * f[k - 1] + aN * f[k + P] + aS * f[k - P] - aP * cur;
+    void* const* A = IsU ? d.mu : d.mv;
+    static_cast<T*>(A[M_AP])[k] = T(aP);
+    static_cast<T*>(A[M_AE])[k] = T(aE);
+    static_cast<T*>(A[M_AW])[k] = T(aW);
+    static_cast<T*>(A[M_AN])[k] = T(aN);
+    static_cast<T*>(A[M_AS])[k] = T(aS);
+    static_cast<T*>(A[M_R0])[k] = T(r0);
+    static_cast<T*>(A[M_D0])[k] = T(0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mom_assemble(KArgs a) {
+    const int nT = *a.nTiles;
+    for (int t = blockIdx.x; t < nT; t += gridDim.x) {
+        const Tile tl = decodeTile(a, t);
+        const SlotDesc& d = a.desc[tl.s];
+        SlotState& st = a.state[tl.s];
+        if (st.status != ST_RUNNING) continue;
+        const int M = d.M, N = d.N, i = tl.i, j = tl.j;
+        const double maxVel = 3.0 * fmax(fabs(st.inletVelocity), 0.1);
+        const double cfl = st.pseudoCFL;
+        if (i <= M && j <= N) {
+            const size_t k = size_t(i) * d.pitch + j;
+            if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2) momRow<T, true>(d, k, i, j, maxVel, cfl);
+            if (i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1) momRow<T, false>(d, k, i, j, maxVel, cfl);
+        }
+        if (arriveLast(&a.counters[tl.s], d.ntiles)) {
+            if (threadIdx.x == 0) {
+                const int cap = d.maxMomIter;
+                st.jk[0] = st.jk[1] = 0;
+                st.jiters[0] = st.jiters[1] = 0;
+                st.jdone[0] = st.jdone[1] = (cap <= 0) ? 1 : 0;
+            }
+            resetCounter(&a.counters[tl.s]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi sweep (jacobi_solver.cpp:44-67): r = r0 - A delta,
+// delta' = delta + r / aP, stop after the sweep where sum r^2 < tol^2 n.
+// U and V are independent systems swept in the same launch, each with its
+// own stop test and sweep count.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ double sweepAt(void* const* A, size_t k, int P, int parity) {
+    const T* src = static_cast<const T*>(A[M_D0 + parity]);
+    T* dst = static_cast<T*>(A[M_D0 + (parity ^ 1)]);
+    const T aP = static_cast<const T*>(A[M_AP])[k];
+    const T dc = src[k];
+    const T Ax = aP * dc - static_cast<const T*>(A[M_AE])[k] * src[k + 1] -
+                 static_cast<const T*>(A[M_AW])[k] * src[k - 1] -
+                 static_cast<const T*>(A[M_AN])[k] * src[k + P] -
+                 static_cast<const T*>(A[M_AS])[k] * src[k - P];
+    const T r = static_cast<const T*>(A[M_R0])[k] - Ax;
+    dst[k] = dc + r / aP;
+    return double(r) * double(r);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_jacobi(KArgs a) {
+    const int nT = *a.nTiles;
+    for (int t = blockIdx.x; t < nT; t += gridDim.x) {
+        const Tile tl = decodeTile(a, t);
+        const SlotDesc& d = a.desc[tl.s];
+        SlotState& st = a.state[tl.s];
+        if (st.status != ST_RUNNING) continue;
+        const int doneU = st.jdone[0], doneV = st.jdone[1];
+        if (doneU && doneV) continue;
+        const int M = d.M, N = d.N, i = tl.i, j = tl.j;
+        double v[2] = {0.0, 0.0};
+        if (i <= M && j <= N) {
+            const size_t k = size_t(i) * d.pitch + j;
+            if (!doneU && i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2)
+                v[0] = sweepAt<T>(d.mu, k, d.pitch, st.jk[0] & 1);
+            if (!doneV && i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1)
+                v[1] = sweepAt<T>(d.mv, k, d.pitch, st.jk[1] & 1);
+        }
+        blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials);
+        if (arriveLast(&a.counters[tl.s], d.ntiles)) {
+            double tot[2];
+            slotReduce<2, 0u>(a.partials, d.tile0, d.ntiles, tot);
+            if (threadIdx.x == 0) {
+                const double tol2 = d.momTol * d.momTol;
+                for (int c = 0; c < 2; ++c) {
+                    if (st.jdone[c]) continue;
+                    st.jk[c] += 1;
+                    const double n = double(c == 0 ? d.nU : d.nV);
+                    if (tot[c] < tol2 * n) {  // jacobi_solver.cpp:63
+                        st.jdone[c] = 1;
+                        st.jiters[c] = st.jk[c];
+                    } else if (st.jk[c] >= d.maxMomIter) {
+                        st.jdone[c] = 1;
+                        st.jiters[c] = d.maxMomIter;
+                    }
+                }
+            }
+            resetCounter(&a.counters[tl.s]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// scatter with clamp + max residual (momentum_solver.cpp:318-335) fused with
+// the velocity boundary conditions on (u*, v*) (boundaries.cpp:48-106).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mom_finish(KArgs a) {
+    const int nT = *a.nTiles;
+    for (int t = blockIdx.x; t < nT; t += gridDim.x) {
+        const Tile tl = decodeTile(a, t);
+        const SlotDesc& d = a.desc[tl.s];
+        SlotState& st = a.state[tl.s];
+        if (st.status != ST_RUNNING) continue;
+        const int M = d.M, N = d.N, P = d.pitch, i = tl.i, j = tl.j;
+        const double maxVel = 3.0 * fmax(fabs(st.inletVelocity), 0.1);
+        const double Uin = st.inletVelocity;
+        const T* dU = static_cast<const T*>(d.mu[M_D0 + (st.jiters[0] & 1)]);
+        const T* dV = static_cast<const T*>(d.mv[M_D0 + (st.jiters[1] & 1)]);
+        double v[3] = {0.0, 0.0, 0.0};
+        if (i <= M && j <= N) {
+            const size_t k = size_t(i) * P + j;
+            // u* : (M+1) x N
+            if (j <= N - 1) {
+                double val;
+                if (i == 0 || i == M) {
+                    val = 0.0;
+                } else if (j == 0) {
+                    val = (d.gam[k + 1] > 0.5) ? Uin : 0.0;  // gamma(i-1, 0) > 0.5
+                } else if (j == N - 1) {
+                    val = clampv(d.u[k - 1] + double(dU[k - 1]), maxVel);  // outlet: u(i,N-1) = u(i,N-2)
+                } else {
+                    val = clampv(d.u[k] + double(dU[k]), maxVel);
+                    v[0] = fabs(val - d.u[k]);
+                }
+                d.us[k] = val;
+                if (!(val == val)) v[2] = 1.0;
+            }
+            // v* : M x (N+1)
+            if (i <= M - 1) {
+                double val;
+                if (i == 0 || i == M - 1 || j == 0) {
+                    val = 0.0;
+                } else if (j == N) {
+                    val = clampv(d.v[k - 1] + double(dV[k - 1]), maxVel);  // outlet: v(i,N) = v(i,N-1)
+                } else {
+                    val = clampv(d.v[k] + double(dV[k]), maxVel);
+                    v[1] = fabs(val - d.v[k]);
+                }
+                d.vs[k] = val;
+                if (!(val == val)) v[2] = 1.0;
+            }
+        }
+        blockReduceTo<3, 7u>(v, a.partials + size_t(t) * kNumPartials);
+        if (arriveLast(&a.counters[tl.s], d.ntiles)) {
+            double tot[3];
+            slotReduce<3, 7u>(a.partials, d.tile0, d.ntiles, tot);
+            if (threadIdx.x == 0) {
+                st.residU = tot[0];
+                st.residV = tot[1];
+                if (tot[2] != 0.0) st.health = 1;
+                st.itU = st.jiters[0];
+                st.itV = st.jiters[1];
+            }
+            resetCounter(&a.counters[tl.s]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pressure correction: mass residual, coefficients, PCG start
+// (pressure_solver.cpp:463-486, 648-653, 721-732)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pres_assemble(KArgs a) {
+    const int nT = *a.nTiles;
+    for (int t = blockIdx.x; t < nT; t += gridDim.x) {
+        const Tile tl = decodeTile(a, t);
+        const SlotDesc& d = a.desc[tl.s];
+        SlotState& st = a.state[tl.s];
+        if (st.status != ST_RUNNING) continue;
+        const int M = d.M, N = d.N, P = d.pitch, i = tl.i, j = tl.j;
+        const int jMass = d.massScope ? N - 2 : N - 1;
+        double v[2] = {0.0, 0.0};
+        if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 1) {
+            const size_t k = size_t(i) * P + j;
+            const double rho = d.rho, hx = d.hx, hy = d.hy;
+            const double mE = rho * d.us[k] * hy, mW = rho * d.us[k - 1] * hy;
+            const double mN = rho * d.vs[k] * hx, mS = rho * d.vs[k - P] * hx;
+            const double b = mW - mE + mS - mN;
+            if (j <= jMass) v[0] = fabs(b);
+            if (j <= N - 2) {
+                const double aE = rho * d.dE[k] * hy;
+                const double aW = (j > 1) ? rho * d.dE[k - 1] * hy : 0.0;
+                const double aN = (i < M - 1) ? rho * d.dN[k] * hx : 0.0;
+                const double aS = (i > 1) ? rho * d.dN[k - P] * hx : 0.0;
+                double aP = aE + aW + aN + aS;
+                if (aP < 1e-20) aP = 1.0;
+                static_cast<T*>(d.pr[P_CE])[k] = T(aE);
+                static_cast<T*>(d.pr[P_CN])[k] = T(aN);
+                static_cast<T*>(d.pr[P_AP])[k] = T(aP);
+                const T r = T(b);
+                static_cast<T*>(d.pr[P_R])[k] = r;
+                static_cast<T*>(d.pr[P_X])[k] = T(0);
+                d.bm[k] = b;
+                d.pc[k] = 0.0;
+                const T z = r / T(aP);
+                v[1] = double(r) * double(z);
+            }
+        }
+        blockReduceTo<2, 1u>(v, a.partials + size_t(t) * kNumPartials);
+        if (arriveLast(&a.counters[tl.s], d.ntiles)) {
+            double tot[2];
+            slotReduce<2, 1u>(a.partials, d.tile0, d.ntiles, tot);
+            if (threadIdx.x == 0) {
+                st.residMass = tot[0];
+                st.rz = tot[1];
+                st.beta = 0.0;
+                st.pk = 0;
+                st.piters = 0;
+                st.pitTotal = 0;
+                st.pcycle = 0;
+                st.pdone = (d.maxPresIter <= 0) ? 1 : 0;
+                st.stepPhase = 0;  // refinement not finished
+            }
+            resetCounter(&a.counters[tl.s]);
+        }
+    }
+}
+
+// p' DOF test: i in [1, M-1], j in [1, N-2] (pressure_solver.cpp:445-455)
+__device__ __forceinline__ bool isPDof(int i, int j, int M, int N) {
+    return i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2;
+}
+
+// ---------------------------------------------------------------------------
+// PCG iteration, part 1: p_k = z_k + beta p_{k-1} (z = r / aP, formed on the
+// fly for the node and its four neighbours), q = A p_k, partial p.q
+// (pressure_solver.cpp:153-159, 173-181)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_q(KArgs a) {
+    const int nT = *a.nTiles;
+    for (int t = blockIdx.x; t < nT; t += gridDim.x) {
+        const Tile tl = decodeTile(a, t);
+        const SlotDesc& d = a.desc[tl.s];
+        SlotState& st = a.state[tl.s];
+        if (st.status != ST_RUNNING || st.pdone || st.stepPhase) continue;
+        const int M = d.M, N = d.N, P = d.pitch, i = tl.i, j = tl.j;
+        const int kk = st.pk;
+        const T beta = T(st.beta);
+        double v[1] = {0.0};
+        if (isPDof(i, j, M, N)) {
+            const size_t k = size_t(i) * P + j;
+            const T* r = static_cast<const T*>(d.pr[P_R]);
+            const T* aP = static_cast<const T*>(d.pr[P_AP]);
+            const T* cE = static_cast<const T*>(d.pr[P_CE]);
+            const T* cN = static_cast<const T*>(d.pr[P_CN]);
+            const T* pold = static_cast<const T*>(d.pr[P_PK0 + (kk & 1)]);
+            T* pnew = static_cast<T*>(d.pr[P_PK0 + ((kk & 1) ^ 1)]);
+            auto pAt = [&](int ii, int jj, size_t kn) -> T {
+                if (!isPDof(ii, jj, M, N)) return T(0);
+                const T z = r[kn] / aP[kn];
+                return kk == 0 ? z : z + beta * pold[kn];
+            };
+            const T pC = pAt(i, j, k);
+            const T pS = pAt(i - 1, j, k - P), pW = pAt(i, j - 1, k - 1);
+            const T pE = pAt(i, j + 1, k + 1), pN = pAt(i + 1, j, k + P);
+            // CSR column order S, W, P, E, N (pressure_solver.cpp:108-118)
+            const T q = -cN[k - P] * pS - cE[k - 1] * pW + aP[k] * pC - cE[k] * pE - cN[k] * pN;
+            pnew[k] = pC;
+            static_cast<T*>(d.pr[P_Q])[k] = q;
+            v[0] = double(pC) * double(q);
+        }
+        blockReduceTo<1, 0u>(v, a.partials + size_t(t) * kNumPartials);
+        if (arriveLast(&a.counters[tl.s], d.ntiles)) {
+            double tot[1];
+            slotReduce<1, 0u>(a.partials, d.tile0, d.ntiles, tot);
+            if (threadIdx.x == 0) {
+                if (fabs(tot[0]) < 1e-20) {  // pressure_solver.cpp:158
+                    st.pdone = 1;
+                    st.piters = st.pk;
+                } else {
+                    st.alpha = st.rz / tot[0];
+                }
+            }
+            resetCounter(&a.counters[tl.s]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PCG iteration, part 2: x += alpha p, r -= alpha q, partial r.r and r.z;
+// stop on ||r||_2 < pTol or the iteration cap (pressure_solver.cpp:159-182)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
+    const int nT = *a.nTiles;
+    for (int t = blockIdx.x; t < nT; t += gridDim.x) {
+        const Tile tl = decodeTile(a, t);
+        const SlotDesc& d = a.desc[tl.s];
+        SlotState& st = a.state[tl.s];
+        if (st.status != ST_RUNNING || st.pdone || st.stepPhase) continue;
+        const int M = d.M, N = d.N, P = d.pitch, i = tl.i, j = tl.j;
+        const int kk = st.pk;
+        const T alpha = T(st.alpha);
+        double v[2] = {0.0, 0.0};
+        if (isPDof(i, j, M, N)) {
+            const size_t k = size_t(i) * P + j;
+            const T p = static_cast<const T*>(d.pr[P_PK0 + ((kk & 1) ^ 1)])[k];
+            T* x = static_cast<T*>(d.pr[P_X]);
+            T* r = static_cast<T*>(d.pr[P_R]);
+            const T q = static_cast<const T*>(d.pr[P_Q])[k];
+            const T aP = static_cast<const T*>(d.pr[P_AP])[k];
+            x[k] += alpha * p;
+            const T rn = r[k] - alpha * q;
+            r[k] = rn;
+            v[0] = double(rn) * double(rn);
+            v[1] = double(rn) * double(rn / aP);
+        }
+        blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials);
+        if (arriveLast(&a.counters[tl.s], d.ntiles)) {
+            double tot[2];
+            slotReduce<2, 0u>(a.partials, d.tile0, d.ntiles, tot);
+            if (threadIdx.x == 0) {
+                st.rr = tot[0];
+                if (sqrt(tot[0]) < d.pTol) {
+                    st.pdone = 1;
+                    st.piters = st.pk;
+                } else {
+                    st.beta = tot[1] / st.rz;
+                    st.rz = tot[1];
+                    st.pk += 1;
+                    if (st.pk >= d.maxPresIter) {
+                        st.pdone = 1;
+                        st.piters = d.maxPresIter;
+                    }
+                }
+            }
+            resetCounter(&a.counters[tl.s]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed mode, FP64 iterative refinement of p': p' += x (FP32 correction),
+// then (if another cycle follows) r64 = b - A p' in FP64, r = r64, x = 0.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_refine_accum(KArgs a) {
+    const int nT = *a.nTiles;
+    for (int t = blockIdx.x; t < nT; t += gridDim.x) {
+        const Tile tl = decodeTile(a, t);
+        const SlotDesc& d = a.desc[tl.s];
+        SlotState& st = a.state[tl.s];
+        if (st.status != ST_RUNNING || st.stepPhase) continue;
+        const int M = d.M, N = d.N, P = d.pitch, i = tl.i, j = tl.j;
+        if (isPDof(i, j, M, N)) {
+            const size_t k = size_t(i) * P + j;
+            d.pc[k] += double(static_cast<const T*>(d.pr[P_X])[k]);
+        }
+        if (arriveLast(&a.counters[tl.s], d.ntiles)) {
+            if (threadIdx.x == 0) {
+                st.pitTotal += st.piters;
+                if (st.pcycle + 1 >= d.refineCycles) st.stepPhase = 1;  // refinement finished
+            }
+            resetCounter(&a.counters[tl.s]);
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_refine_resid(KArgs a) {
+    const int nT = *a.nTiles;
+    for (int t = blockIdx.x; t < nT; t += gridDim.x) {
+        const Tile tl = decodeTile(a, t);
+        const SlotDesc& d = a.desc[tl.s];
+        SlotState& st = a.state[tl.s];
+        if (st.status != ST_RUNNING || st.stepPhase) continue;
+        const int M = d.M, N = d.N, P = d.pitch, i = tl.i, j = tl.j;
+        double v[2] = {0.0, 0.0};
+        if (isPDof(i, j, M, N)) {
+            const size_t k = size_t(i) * P + j;
+            const T* aP = static_cast<const T*>(d.pr[P_AP]);
+            const T* cE = static_cast<const T*>(d.pr[P_CE]);
+            const T* cN = static_cast<const T*>(d.pr[P_CN]);
+            const double* pc = d.pc;
+            auto pAt = [&](int ii, int jj, size_t kn) -> double { return isPDof(ii, jj, M, N) ? pc[kn] : 0.0; };
+            const double Ap = -double(cN[k - P]) * pAt(i - 1, j, k - P) - double(cE[k - 1]) * pAt(i, j - 1, k - 1) +
+                              double(aP[k]) * pc[k] - double(cE[k]) * pAt(i, j + 1, k + 1) -
+                              double(cN[k]) * pAt(i + 1, j, k + P);
+            const double r64 = d.bm[k] - Ap;
+            const T r = T(r64);
+            static_cast<T*>(d.pr[P_R])[k] = r;
+            static_cast<T*>(d.pr[P_X])[k] = T(0);
+            v[0] = r64 * r64;
+            v[1] = double(r) * double(r / aP[k]);
+        }
+        blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials);
+        if (arriveLast(&a.counters[tl.s], d.ntiles)) {
+            double tot[2];
+            slotReduce<2, 0u>(a.partials, d.tile0, d.ntiles, tot);
+            if (threadIdx.x == 0) {
+                st.pcycle += 1;
+                if (sqrt(tot[0]) < d.pTol) {
+                    st.stepPhase = 1;
+                } else {
+                    st.rz = tot[1];
+                    st.beta = 0.0;
+                    st.pk = 0;
+                    st.piters = 0;
+                    st.pdone = 0;
+                }
+            }
+            resetCounter(&a.counters[tl.s]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Outer-loop scalar logic of one slot: runIterations' loop test and the SER /
+// line-search / CFL-ramp updates (SIMPLE.cpp:299-327, time_control.cpp:43-165).
+// ---------------------------------------------------------------------------
+__device__ void controller(const SlotDesc& d, SlotState& st) {
+    const double error = nanmax(nanmax(st.residU, st.residV), st.residMass);
+    st.error = error;
+    const double mon = d.serUseMax ? error : st.residMass;
+    double cfl = st.pseudoCFL;
+    if (d.pseudo && d.enableSER && st.iter >= d.serMinIter) {
+        const double cur = fmax(mon, 1e-15);
+        if (st.serResidPrev <= 0.0) {
+            st.serResidPrev = cur;
+        } else {
+            double ratio = st.serResidPrev / cur;
+            ratio = fmax(ratio, d.serMaxDec);
+            ratio = fmin(ratio, d.serMaxInc);
+            double target = cfl * ratio;
+            target = fmax(target, d.serCFLMin);
+            target = fmin(target, d.serCFLMax);
+            cfl = (1.0 - d.serSmooth) * cfl + d.serSmooth * target;
+            if (!isfinite(cfl) || cfl <= 0.0) cfl = d.serCFLMin;
+            st.serResidPrev = cur;
+        }
+    }
+    if (d.pseudo && d.enableLS && d.enableSER) {
+        if (st.serResidPrev <= 0.0) {
+            st.lsAlpha = 1.0;
+            st.lsBacktrack = 0;
+        } else {
+            const double ratio = mon / st.serResidPrev;
+            if (ratio > d.lsTol) {
+                st.lsBacktrack += 1;
+                if (st.lsBacktrack <= d.lsMaxTries) {
+                    st.lsAlpha = fmax(st.lsAlpha * d.lsAlphaReduce, d.lsAlphaMin);
+                    cfl = fmax(cfl * d.lsAlphaReduce, d.serCFLMin);
+                }
+            } else {
+                if (st.lsAlpha < 1.0) st.lsAlpha = fmin(st.lsAlpha * 1.1, 1.0);
+                st.lsBacktrack = 0;
+            }
+        }
+    }
+    if (d.pseudo && d.enableRamp && !d.enableSER) {
+        const double cur = fmax(mon, 1e-12);
+        if (cur > d.rampStart) {
+            cfl = d.cflInit;
+        } else {
+            const double ratio = d.rampStart / cur;
+            const double raw = pow(ratio, d.rampExp);
+            const double target = fmin(d.cflInit * raw, d.cflMax);
+            cfl = (1.0 - d.rampSmooth) * cfl + d.rampSmooth * target;
+            if (!isfinite(cfl) || cfl <= 0.0) cfl = d.cflInit;
+        }
+    }
+    st.pseudoCFL = cfl;
+    st.iter += 1;
+    if (st.health || !isfinite(error)) {
+        st.health = 1;
+        st.status = ST_FAILED;
+    } else if (error <= d.epsilon) {
+        st.status = ST_CONVERGED;
+    } else if (st.iter >= d.maxIterations) {
+        st.status = ST_MAXITER;
+    }
+    st.inletVelocity = (d.inletRamp && d.rampSteps > 0 && st.iter < d.rampSteps)
+                           ? (double(st.iter + 1) / double(d.rampSteps)) * d.targetVel
+                           : d.targetVel;
+}
+
+// ---------------------------------------------------------------------------
+// p += pAlpha p', pressure BCs, velocity correction, velocity BCs
+// (iterations.cpp:456-500, boundaries.cpp:48-134), then (when `outer`) the
+// slot's outer-loop scalar logic.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_correct(KArgs a, int outer) {
+    const int nT = *a.nTiles;
+    for (int t = blockIdx.x; t < nT; t += gridDim.x) {
+        const Tile tl = decodeTile(a, t);
+        const SlotDesc& d = a.desc[tl.s];
+        SlotState& st = a.state[tl.s];
+        if (st.status != ST_RUNNING) continue;
+        const int M = d.M, N = d.N, P = d.pitch, i = tl.i, j = tl.j;
+        const double maxVel = 3.0 * fmax(fabs(st.inletVelocity), 0.1);
+        const double Uin = st.inletVelocity;
+        double bad = 0.0;
+        if (i <= M && j <= N) {
+            const size_t k = size_t(i) * P + j;
+            const double* pc = d.pc;
+            // pressure update + ghosts; each ghost has exactly one writer
+            if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 1) {
+                const double pn = d.p[k] + d.pAlpha * pc[k];
+                d.p[k] = pn;
+                if (i == 1) d.p[k - P] = pn;      // p(0,j) = p(1,j)
+                if (i == M - 1) d.p[k + P] = pn;  // p(M,j) = p(M-1,j)
+                if (j == 1) d.p[k - 1] = pn;      // p(i,0) = p(i,1)
+                if (j == N - 1) d.p[k + 1] = 0.0; // p(i,N) = 0
+                if (!(pn == pn)) bad = 1.0;
+            }
+            auto uCorr = [&](size_t kk) {
+                const double du = clampv(d.dE[kk] * (pc[kk] - pc[kk + 1]), maxVel);
+                return clampv(d.us[kk] + du, maxVel);
+            };
+            auto vCorr = [&](size_t kk) {
+                const double dv = clampv(d.dN[kk] * (pc[kk] - pc[kk + P]), maxVel);
+                return clampv(d.vs[kk] + dv, maxVel);
+            };
+            if (j <= N - 1) {
+                double val;
+                if (i == 0 || i == M) val = 0.0;
+                else if (j == 0) val = (d.gam[k + 1] > 0.5) ? Uin : 0.0;
+                else if (j == N - 1) val = uCorr(k - 1);
+                else val = uCorr(k);
+                d.u[k] = val;
+            }
+            if (i <= M - 1) {
+                double val;
+                if (i == 0 || i == M - 1 || j == 0) val = 0.0;
+                else if (j == N) val = vCorr(k - 1);
+                else val = vCorr(k);
+                d.v[k] = val;
+            }
+        }
+        double vb[1] = {bad};
+        blockReduceTo<1, 1u>(vb, a.partials + size_t(t) * kNumPartials);
+        if (arriveLast(&a.counters[tl.s], d.ntiles)) {
+            double tot[1];
+            slotReduce<1, 1u>(a.partials, d.tile0, d.ntiles, tot);
+            if (threadIdx.x == 0) {
+                if (tot[0] != 0.0) st.health = 1;
+                st.itP = d.mixed ? st.pitTotal : st.piters;
+                if (outer) controller(d, st);
+            }
+            resetCounter(&a.counters[tl.s]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// samplePlaneAtX (postprocessing.cpp:53-231): one block per request.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_plane(const SlotDesc* desc, const PlaneReq* req, PlaneOut* out) {
+    const PlaneReq rq = req[blockIdx.x];
+    const SlotDesc& d = desc[rq.slot];
+    const int M = d.M, N = d.N, P = d.pitch;
+    const double hx = d.hx, hy = d.hy, rho = d.rho;
+    const double xMin = 0.5 * hx, xMax = (double(N) - 0.5) * hx;
+    const double xc = fmax(xMin, fmin(rq.x, xMax));
+    const double jf = xc / hx - 0.5;
+    int cl = int(floor(jf));
+    cl = max(0, min(cl, N - 2));
+    const int cr = cl + 1;
+    const double xl = (double(cl) + 0.5) * hx, xr = (double(cr) + 0.5) * hx;
+    double tt = (fabs(xr - xl) < 1e-12) ? 0.0 : (xc - xl) / (xr - xl);
+    tt = fmax(0.0, fmin(1.0, tt));
+    const int pl = min(cl + 1, N), pr = min(cr + 1, N);
+    const int ul = min(cl, N - 1), ur = min(cr, N - 1);
+    const int vl = min(cl, N), vr = min(cr, N);
+    double acc[5] = {0, 0, 0, 0, 0};  // staticArea, dynArea, totalWeighted, area, flux
+    for (int i = 1 + int(threadIdx.x); i < M; i += kBlock) {
+        // gamma(i-1, c) lives at gam(i, c+1)
+        if (d.gam[size_t(i) * P + cl + 1] < rq.gammaMin || d.gam[size_t(i) * P + cr + 1] < rq.gammaMin) continue;
+        const double fa = hy;
+        acc[3] += fa;
+        const double sp = (1.0 - tt) * d.p[size_t(i) * P + pl] + tt * d.p[size_t(i) * P + pr];
+        const double un = (1.0 - tt) * d.u[size_t(i) * P + ul] + tt * d.u[size_t(i) * P + ur];
+        const int vt = min(i - 1, M - 1), vb = min(i, M - 1);
+        const double vT = (1.0 - tt) * d.v[size_t(vt) * P + vl] + tt * d.v[size_t(vt) * P + vr];
+        const double vB = (1.0 - tt) * d.v[size_t(vb) * P + vl] + tt * d.v[size_t(vb) * P + vr];
+        const double vtan = 0.5 * (vT + vB);
+        const double vm = sqrt(un * un + vtan * vtan);
+        const double dyn = 0.5 * rho * vm * vm;
+        acc[0] += sp * fa;
+        acc[1] += dyn * fa;
+        const double fl = fabs(rho * un * fa);
+        if (fl > 0.0) {
+            acc[4] += fl;
+            acc[2] += (sp + dyn) * fl;
+        }
+    }
+    __shared__ double res[5];
+    double a4[4] = {acc[0], acc[1], acc[2], acc[3]};
+    blockReduceTo<4, 0u>(a4, res);
+    __syncthreads();
+    double a1[1] = {acc[4]};
+    __shared__ double res2[1];
+    blockReduceTo<1, 0u>(a1, res2);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        PlaneOut o;
+        const double area = res[3], fs = res2[0];
+        o.flowArea = area;
+        o.massFlux = fs;
+        o.avgStatic = 0.0;
+        o.avgDynamic = 0.0;
+        o.avgTotal = 0.0;
+        o.valid = 0;
+        if (area > 0.0) {
+            o.avgStatic = res[0] / area;
+            o.avgDynamic = res[1] / area;
+            o.valid = 1;
+        }
+        if (fs > 0.0) o.avgTotal = res[2] / fs;
+        else if (area > 0.0) o.avgTotal = o.avgStatic + o.avgDynamic;
+        out[blockIdx.x] = o;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+// ---------------------------------------------------------------------------
+template <typename T>
+static void launchT(Kern k, int grid, hipStream_t s, const KArgs& a) {
+    const dim3 g(grid), b(kBlock);
+    switch (k) {
+        case K_MOM_ASM: hipLaunchKernelGGL(k_mom_assemble<T>, g, b, 0, s, a); break;
+        case K_JAC: hipLaunchKernelGGL(k_jacobi<T>, g, b, 0, s, a); break;
+        case K_MOM_FIN: hipLaunchKernelGGL(k_mom_finish<T>, g, b, 0, s, a); break;
+        case K_PRES_ASM: hipLaunchKernelGGL(k_pres_assemble<T>, g, b, 0, s, a); break;
+        case K_PCG_Q: hipLaunchKernelGGL(k_pcg_q<T>, g, b, 0, s, a); break;
+        case K_PCG_U: hipLaunchKernelGGL(k_pcg_update<T>, g, b, 0, s, a); break;
+        case K_REF_ACC: hipLaunchKernelGGL(k_refine_accum<T>, g, b, 0, s, a); break;
+        case K_REF_RES: hipLaunchKernelGGL(k_refine_resid<T>, g, b, 0, s, a); break;
+        case K_CORR: hipLaunchKernelGGL(k_correct, g, b, 0, s, a, 0); break;
+        case K_CORR_OUTER: hipLaunchKernelGGL(k_correct, g, b, 0, s, a, 1); break;
+    }
+}
+
+hipError_t launchKernel(Kern k, bool f32, int grid, hipStream_t s, const KArgs& a) {
+    if (f32) launchT<float>(k, grid, s, a);
+    else launchT<double>(k, grid, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launchPlane(int nreq, hipStream_t s, const SlotDesc* desc, const PlaneReq* req, PlaneOut* out) {
+    hipLaunchKernelGGL(k_plane, dim3(nreq), dim3(kBlock), 0, s, desc, req, out);
+    return hipGetLastError();
+}
+
+}  // namespace cpdev

@@ -1,0 +1,41 @@
+/*
+ * coldplate_defaults.h — the reference's parameter defaults as a table.
+ *
+ * Values are the member initialisers of class SIMPLE (SIMPLE.h:68-358):
+ * they are data, kept identical on purpose so a caller switching backends
+ * gets the same behaviour. Used by cp_params_default() and by the CPU oracle.
+ */
+#ifndef COLDPLATE_DEFAULTS_H
+#define COLDPLATE_DEFAULTS_H
+
+#include <string.h>
+#include "coldplate_hip.h"
+
+static inline void cp_fill_reference_defaults(cp_params* p) {
+    memset(p, 0, sizeof(*p));
+    p->epsilon = 1e-5;            p->maxIterations = 10000;       /* SIMPLE.h:68-69 */
+    p->maxPressureIter = 20;      p->pTol = 1e-4;                 /* SIMPLE.h:152-153 */
+    p->momentumTol = 1e-5;        p->maxMomentumIter = 200;       /* SIMPLE.h:176-177 */
+    p->convectionScheme = 0;                                      /* SIMPLE.h:193 */
+    p->enableTwoPointFiveD = 0;   p->enableConvectionScaling = 0; /* SIMPLE.h:205-206 */
+    p->Ht = 0.0;                  p->sinkMultiplier = 4.8;        /* SIMPLE.h:207-208 */
+    p->KMin = 1e-14;              p->alphaMax = 0.0;  p->brinkmanQ = 1.0;  /* SIMPLE.h:242-244 */
+    p->timeStepMultiplier = 0.01; p->enablePseudoTime = 0;        /* SIMPLE.h:260-262 */
+    p->minPseudoSpeed = 0.05;     p->useLocalPseudoTime = 1;      /* SIMPLE.h:264-265 */
+    p->enableSER = 1;  p->serCFLMin = 0.1;  p->serCFLMax = 5.0;   /* SIMPLE.h:278-280 */
+    p->serSmooth = 0.5; p->serMinIter = 10;                        /* SIMPLE.h:282-283 */
+    p->serMaxIncrease = 1.1; p->serMaxDecrease = 0.7; p->serUseMaxResid = 0; /* SIMPLE.h:284-286 */
+    p->enableLineSearch = 1; p->lsAlphaMin = 0.1; p->lsAlphaReduce = 0.3;    /* SIMPLE.h:291-293 */
+    p->lsMaxTries = 4; p->lsResidIncreaseTol = 1.01;               /* SIMPLE.h:294-295 */
+    p->enableCflRamp = 1; p->pseudoCFLInitial = 0.1; p->pseudoCFLMax = 30.0; /* SIMPLE.h:303-305 */
+    p->cflRampStartRes = 1e-4; p->cflRampExponent = 0.8; p->cflRampSmooth = 0.1; /* SIMPLE.h:306-308 */
+    p->rho = 997.0;  p->mu = 0.00089;                              /* SIMPLE.h:334-335 */
+    p->U_in = 0.0;   p->enableInletRamp = 0;  p->rampSteps = 1000; /* SIMPLE.h:343-346 */
+    p->uvAlpha = 0.7; p->pAlpha = 0.2;                             /* SIMPLE.h:357-358 */
+    p->precision = CP_PREC_FP64;
+    p->refineCycles = 2;
+    p->fixedMomentumSweeps = 0;
+    p->massResidualScope = 0;
+}
+
+#endif /* COLDPLATE_DEFAULTS_H */

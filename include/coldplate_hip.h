@@ -1,0 +1,265 @@
+/*
+ * coldplate_hip.h — C ABI of the MI355X (gfx950) SIMPLE backend.
+ *
+ * The library keeps every field of a solve (staggered u/v/p, u*, v*, p',
+ * dE/dN, the Brinkman alpha faces and the gamma density field) resident in
+ * HBM for the whole run. The host only hands over geometry and parameters,
+ * and reads fields back on request (checkpoint, result).
+ *
+ * Shapes follow the reference's staggered MAC layout (SIMPLE.h:393-440,
+ * SIMPLE.cpp:184-201), stored row-major here:
+ *     cellType / gamma : M x N
+ *     u, u*, dE        : (M+1) x N
+ *     v, v*, dN        : M x (N+1)
+ *     p, p'            : (M+1) x (N+1)
+ *
+ * Error convention: every entry point returns CP_OK (0) or a negative
+ * CP_E_* code; cp_last_error() gives the message. Nothing throws across
+ * the boundary.
+ *
+ * The reference's device-library seam being replaced is
+ * namespace amgcl_cuda (Utilities/solvers/gpu/cuda_solver.hpp:15-59), which
+ * re-uploads a host CSR system on every call. This ABI is session-scoped
+ * instead: one context owns a geometry's device state, and the
+ * SIMPLE sub-steps run on it without host round trips.
+ */
+#ifndef COLDPLATE_HIP_H
+#define COLDPLATE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CP_OK          0
+#define CP_E_ARG      -1   /* invalid argument / shape */
+#define CP_E_HIP      -2   /* HIP runtime error */
+#define CP_E_NOMEM    -3   /* device arena exhausted */
+#define CP_E_STATE    -4   /* call not valid in the current state */
+#define CP_E_CAPACITY -5   /* batch has no free slot or HBM budget */
+#define CP_E_NODEV    -6   /* no usable gfx950 device */
+#define CP_E_DIVERGED -7   /* non-finite field detected (health word) */
+
+/* Arithmetic modes. */
+#define CP_PREC_FP64   0   /* plain FP64: the drop-in single-geometry solver */
+#define CP_PREC_MIXED  1   /* FP32 stencil/Jacobi/PCG + FP64 refinement,
+                              FP64 reductions and convergence checks */
+
+/* Fields for cp_get_field / cp_set_field. */
+#define CP_F_U      0
+#define CP_F_V      1
+#define CP_F_P      2
+#define CP_F_USTAR  3
+#define CP_F_VSTAR  4
+#define CP_F_PSTAR  5   /* pressure correction p' */
+#define CP_F_DE     6
+#define CP_F_DN     7
+#define CP_F_GAMMA  8   /* M x N */
+#define CP_F_ALPHA  9   /* M x N */
+
+/*
+ * Solver parameters. Field names and meaning follow the public members of
+ * class SIMPLE (SIMPLE.h:68-358); cp_params_default() gives the reference
+ * defaults. rho and mu are parameters here (the reference hard-codes
+ * 997 / 8.9e-4 at SIMPLE.h:334-335 and ignores the file values,
+ * SIMPLE.cpp:48-54).
+ */
+typedef struct cp_params {
+    /* grid (SIMPLE.h:321-326) */
+    int32_t M, N;              /* cell rows (y), cell columns (x) */
+    double  hx, hy;            /* cell width, height [m] */
+    /* fluid (SIMPLE.h:334-335) */
+    double  rho, mu;
+    /* inlet (SIMPLE.h:343-346) */
+    double  U_in;              /* targetVel */
+    int32_t enableInletRamp, rampSteps;
+    /* relaxation (SIMPLE.h:357-358) */
+    double  uvAlpha, pAlpha;
+    /* convection scheme (SIMPLE.h:193): 0 FOU, 1 SOU deferred correction */
+    int32_t convectionScheme;
+    /* 2.5D model (SIMPLE.h:205-208) */
+    int32_t enableTwoPointFiveD, enableConvectionScaling;
+    double  Ht, sinkMultiplier;
+    /* Brinkman (SIMPLE.h:242-244). alphaMax <= 0 means mu / max(KMin,1e-20)
+       (SIMPLE.cpp:65-66). */
+    double  alphaMax, KMin, brinkmanQ;
+    /* pseudo-transient continuation (SIMPLE.h:260-308) */
+    int32_t enablePseudoTime, useLocalPseudoTime;
+    double  timeStepMultiplier, minPseudoSpeed;
+    int32_t enableCflRamp;
+    double  pseudoCFLInitial, pseudoCFLMax, cflRampStartRes, cflRampExponent, cflRampSmooth;
+    int32_t enableSER, serMinIter, serUseMaxResid;
+    double  serCFLMin, serCFLMax, serSmooth, serMaxIncrease, serMaxDecrease;
+    int32_t enableLineSearch, lsMaxTries;
+    double  lsAlphaMin, lsAlphaReduce, lsResidIncreaseTol;
+    /* inner solvers (SIMPLE.h:152-153, 176-177) */
+    double  momentumTol;  int32_t maxMomentumIter;
+    double  pTol;         int32_t maxPressureIter;
+    /* outer loop (SIMPLE.h:68-69) */
+    double  epsilon;      int32_t maxIterations;
+    /* arithmetic mode, CP_PREC_* */
+    int32_t precision;
+    /* CP_PREC_MIXED only: FP64 refinement cycles of the pressure
+       correction (each one FP32 PCG of at most maxPressureIter iterations),
+       and fixed Jacobi sweeps per momentum solve (0 = reference stop test). */
+    int32_t refineCycles;
+    int32_t fixedMomentumSweeps;
+    /* Columns of the mass residual residMass: 0 = j in [1, N-1] as the
+       reference (pressure_solver.cpp:471-483); 1 = j in [1, N-2], the p'
+       unknowns only. Column N-1 carries no p' unknown, so its imbalance is
+       never corrected and floors residMass (1.655e-6 kg/s/m on the 200x50
+       straight channel); scope 1 lets the stop test see convergence. */
+    int32_t massResidualScope;
+} cp_params;
+
+/* Statistics of one outer iteration (one pass of SIMPLE.cpp:299-439). */
+typedef struct cp_step_stats {
+    double  error;       /* max(residU, residV, residMass), iterations.cpp:525 */
+    double  residU, residV, residMass;
+    double  pseudoCFL;   /* after SER / line search / ramp */
+    int32_t itU, itV;    /* Jacobi sweeps of the U / V momentum solves */
+    int32_t itP;         /* PCG iterations (return value of solvePCG_CSR) */
+    int32_t iter;        /* outer iterations completed */
+    int32_t converged;   /* error <= epsilon */
+    int32_t health;      /* 0 ok, else CP_E_DIVERGED */
+} cp_step_stats;
+
+/* PlaneMetrics (SIMPLE.h:679-686). */
+typedef struct cp_plane_metrics {
+    double  flowArea, massFlux, avgStatic, avgDynamic, avgTotal;
+    int32_t valid;
+} cp_plane_metrics;
+
+/* ------------------------------------------------------------------ */
+/* parameters and diagnostics                                          */
+/* ------------------------------------------------------------------ */
+
+/* Fill p with the reference defaults of class SIMPLE (SIMPLE.h:68-358). */
+int cp_params_default(cp_params* p);
+
+/* Message of the last error of ctx (NULL: the library-wide last error). */
+const char* cp_last_error(const void* ctx);
+
+/* Number of usable gfx950 devices (0 without a GPU). */
+int cp_device_count(void);
+
+/* ------------------------------------------------------------------ */
+/* single-geometry context: the drop-in backend                        */
+/* ------------------------------------------------------------------ */
+typedef struct cp_ctx cp_ctx;
+
+/*
+ * Create a solve on the current device from parameters and the cellType
+ * field (M x N, 0 = fluid, 1 = solid, intermediate = buffer).
+ * Replaces SIMPLE::SIMPLE's loadParameters / initializeMemory /
+ * loadTopology / loadDensityField / buildAlphaFromDensity
+ * (SIMPLE.cpp:18-29, 31-233, 517-630) for the device-resident state.
+ */
+int cp_create(const cp_params* prm, const double* cellType, cp_ctx** out);
+int cp_destroy(cp_ctx* ctx);
+
+/* Restart / checkpoint (SIMPLE.cpp:203-232, output.cpp:73-82). Any pointer
+   may be NULL to skip that field. cp_set_fields also sets u* = uOld = u and
+   v* = vOld = v as the reference restart does. */
+int cp_set_fields(cp_ctx* ctx, const double* u, const double* v, const double* p);
+int cp_get_fields(cp_ctx* ctx, double* u, double* v, double* p);
+/* Read any field (CP_F_*) in its reference shape. */
+int cp_get_field(cp_ctx* ctx, int which, double* out);
+
+/* U and V implicit momentum solves followed by the velocity boundary
+   conditions on (u*, v*): replaces solveUMomentumImplicit /
+   solveVMomentumImplicit (momentum_solver.cpp:346-352, SIMPLE.h:705-709)
+   and iterations.cpp:118-135. Fills itU/itV/residU/residV of st. */
+int cp_momentum(cp_ctx* ctx, cp_step_stats* st);
+
+/* Pressure-correction solve: replaces SIMPLE::solvePressureSystem
+   (pressure_solver.cpp:462-806, pressureSolverType 1). Fills itP and
+   residMass of st. */
+int cp_pressure(cp_ctx* ctx, cp_step_stats* st);
+
+/* p += pAlpha p', pressure BCs, velocity correction and velocity BCs
+   (iterations.cpp:456-500, boundaries.cpp:48-134). */
+int cp_correct(cp_ctx* ctx);
+
+/* One outer iteration: inlet ramp, calculateStep (iterations.cpp:82-526),
+   then the SER / line-search / CFL-ramp updates (SIMPLE.cpp:303-327). */
+int cp_step(cp_ctx* ctx, cp_step_stats* st);
+
+/* runIterations without I/O (SIMPLE.cpp:238-439): steps until
+   error <= epsilon or maxIterations. The pressure-drop-window exit is off.
+   check_every: outer iterations between host convergence checks
+   (<= 0: choose automatically). */
+int cp_run(cp_ctx* ctx, int check_every, cp_step_stats* last);
+
+/* samplePlaneAtX (postprocessing.cpp:97-231) at physical x. A row is kept
+   when both bracketing cells have gamma >= gamma_min; the reference uses
+   1 - 1e-6 (postprocessing.cpp:145-149). */
+int cp_plane(cp_ctx* ctx, double x, double gamma_min, cp_plane_metrics* out);
+
+/* Wait for all device work of ctx. */
+int cp_sync(cp_ctx* ctx);
+
+/* ------------------------------------------------------------------ */
+/* batched / served form: many geometries packed in one launch         */
+/* ------------------------------------------------------------------ */
+typedef struct cp_batch cp_batch;
+
+/* Slot status returned by cp_batch_poll. */
+#define CP_SLOT_FREE      0
+#define CP_SLOT_RUNNING   1
+#define CP_SLOT_CONVERGED 2
+#define CP_SLOT_MAXITER   3
+#define CP_SLOT_FAILED    4
+
+typedef struct cp_slot_info {
+    int32_t status;      /* CP_SLOT_* */
+    int32_t iter;        /* outer iterations done */
+    double  error, residU, residV, residMass;
+    int32_t health;
+    int32_t M, N;
+} cp_slot_info;
+
+/*
+ * A batch owns one HBM arena of hbm_budget bytes and at most max_slots
+ * concurrent geometries, all in one precision mode. Each cp_batch_run
+ * step is one persistent launch per SIMPLE sub-step over every running
+ * slot; converged slots are skipped on the device.
+ */
+int cp_batch_create(int precision, size_t hbm_budget, int max_slots, cp_batch** out);
+int cp_batch_destroy(cp_batch* b);
+/* Bytes of arena a geometry of M x N cells needs in this batch. */
+size_t cp_batch_footprint(const cp_batch* b, int M, int N);
+/* Admit a geometry; CP_E_CAPACITY when no slot or arena space is free. */
+int cp_batch_admit(cp_batch* b, const cp_params* prm, const double* cellType, int* slot);
+/* Run `steps` outer iterations of every running slot. */
+int cp_batch_run(cp_batch* b, int steps);
+/* Per-slot status; info has max_slots entries. */
+int cp_batch_poll(cp_batch* b, cp_slot_info* info);
+/* Read fields / plane metrics of a slot (same shapes as cp_get_fields). */
+int cp_batch_get_fields(cp_batch* b, int slot, double* u, double* v, double* p);
+int cp_batch_plane(cp_batch* b, int slot, double x, double gamma_min, cp_plane_metrics* out);
+/* Release a slot's arena space. */
+int cp_batch_evict(cp_batch* b, int slot);
+
+/* ------------------------------------------------------------------ */
+/* seeded geometry generators (cellType, M x N row-major)              */
+/* ------------------------------------------------------------------ */
+/* The reference ships Python generators (GeometryGeneratorV1.py ...
+   GeometryGeneratorV5.py) writing geometry_fluid.txt; these produce the
+   same format for the solver's workloads. */
+/* Straight channel: fluid on cell rows [row0, row1], solid elsewhere. */
+int cp_geom_straight(int M, int N, int row0, int row1, double* cellType);
+/* Seeded network of 6-10 Manhattan/diagonal channels 8-16 cells wide with
+   split/merge junctions, carved into solid. */
+int cp_geom_network(int M, int N, uint64_t seed, double* cellType);
+/* Seeded Gaussian random field (correlation length corr cells), mapped to
+   [0,1] and tanh-projected with (beta, eta); cellType = 1 - gamma. */
+int cp_geom_gamma(int M, int N, uint64_t seed, double corr, double beta, double eta, double* cellType);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* COLDPLATE_HIP_H */

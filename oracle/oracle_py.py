@@ -1,0 +1,224 @@
+"""ctypes binding of the CPU oracle (oracle/lib/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg, as the checker. The product path never imports
+this module.
+
+Also binds oracle/_ref/libref.so, the reference's own Eigen-free solver
+leaves (Utilities/solvers/cpu/{jacobi,pcg}_solver.cpp) built in place by
+``make -C oracle ref``, when it has been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ORACLE_DIR = Path(__file__).resolve().parent
+REPO = ORACLE_DIR.parent
+sys.path.insert(0, str(REPO / "coldplate-solver_amd"))
+from cpbind import Params, PlaneMetrics, StepStats, field_shape  # noqa: E402
+
+LIB = ORACLE_DIR / "lib" / "liboracle.so"
+REF_LIB = ORACLE_DIR / "_ref" / "libref.so"
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+_lib = None
+_ref = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise RuntimeError(f"{LIB} missing: run `make -C oracle`")
+        lib = C.CDLL(str(LIB))
+        sig = {
+            "or_params_default": (None, [C.POINTER(Params)]),
+            "or_create": (_vp, [C.POINTER(Params), _dp, C.c_int]),
+            "or_destroy": (None, [_vp]),
+            "or_step": (C.c_int, [_vp, C.POINTER(StepStats)]),
+            "or_run": (C.c_int, [_vp, C.c_int, C.POINTER(StepStats), _dp]),
+            "or_momentum": (C.c_int, [_vp, C.POINTER(StepStats)]),
+            "or_pressure": (C.c_int, [_vp, C.POINTER(StepStats)]),
+            "or_correct": (C.c_int, [_vp]),
+            "or_get_field": (C.c_int, [_vp, C.c_int, _dp]),
+            "or_set_fields": (C.c_int, [_vp, _dp, _dp, _dp]),
+            "or_set_scalars": (C.c_int, [_vp, C.c_double, C.c_double]),
+            "or_plane": (C.c_int, [_vp, C.c_double, C.c_double, C.POINTER(PlaneMetrics)]),
+            "or_get_system": (C.c_int, [_vp, C.c_int, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _ip, _ip]),
+            "or_jacobi_f32": (C.c_int, [C.c_int, _ip, _ip, _fp, _fp, _fp, _ip, C.c_float, C.c_int]),
+            "or_jacobi_f64": (C.c_int, [C.c_int, _ip, _ip, _dp, _dp, _dp, _ip, C.c_double, C.c_int]),
+            "or_pcg_f32": (C.c_int, [C.c_int, _ip, _ip, _fp, _fp, _fp, C.c_float, C.c_int]),
+            "or_pcg_f64": (C.c_int, [C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_double, C.c_int]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(lib, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = lib
+    return _lib
+
+
+def load_ref() -> C.CDLL | None:
+    """The reference's compiled solver leaves, or None when not built."""
+    global _ref
+    if _ref is None and REF_LIB.exists():
+        lib = C.CDLL(str(REF_LIB))
+        lib.ref_jacobi.restype = C.c_int
+        lib.ref_jacobi.argtypes = [C.c_int, _ip, _ip, _fp, _fp, _fp, _ip, C.c_float, C.c_int]
+        lib.ref_pcg.restype = C.c_int
+        lib.ref_pcg.argtypes = [C.c_int, _ip, _ip, _fp, _fp, _fp, C.c_float, C.c_int]
+        _ref = lib
+    return _ref
+
+
+def _ptr(a: np.ndarray, ty):
+    return a.ctypes.data_as(ty)
+
+
+def default_params() -> Params:
+    p = Params()
+    load().or_params_default(C.byref(p))
+    return p
+
+
+class Oracle:
+    """One oracle solve. fp32=False: FP64 oracle; True: reference float arithmetic."""
+
+    def __init__(self, prm: Params, cell_type: np.ndarray, fp32: bool = False):
+        ct = np.ascontiguousarray(cell_type, dtype=np.float64)
+        assert ct.shape == (prm.M, prm.N)
+        self.prm = prm.copy()
+        self.M, self.N = prm.M, prm.N
+        self.h = load().or_create(C.byref(self.prm), _ptr(ct, _dp), int(fp32))
+        if not self.h:
+            raise RuntimeError("or_create failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().or_destroy(self.h)
+            self.h = None
+
+    def _st(self, fn) -> dict:
+        st = StepStats()
+        fn(self.h, C.byref(st))
+        return st.as_dict()
+
+    def step(self) -> dict:
+        return self._st(load().or_step)
+
+    def momentum(self) -> dict:
+        return self._st(load().or_momentum)
+
+    def pressure(self) -> dict:
+        return self._st(load().or_pressure)
+
+    def correct(self) -> None:
+        load().or_correct(self.h)
+
+    def run(self, max_steps: int) -> tuple[int, dict, float]:
+        st = StepStats()
+        w = C.c_double(0)
+        n = load().or_run(self.h, max_steps, C.byref(st), C.byref(w))
+        return n, st.as_dict(), w.value
+
+    def field(self, which: int) -> np.ndarray:
+        out = np.empty(field_shape(which, self.M, self.N), dtype=np.float64)
+        rc = load().or_get_field(self.h, which, _ptr(out, _dp))
+        assert rc == 0
+        return out
+
+    def set_fields(self, u=None, v=None, p=None) -> None:
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, p)]
+        load().or_set_fields(self.h, *[None if a is None else _ptr(a, _dp) for a in arrs])
+
+    def set_scalars(self, inlet_velocity: float, pseudo_cfl: float) -> None:
+        load().or_set_scalars(self.h, inlet_velocity, pseudo_cfl)
+
+    def plane(self, x: float, gamma_min: float = 1.0 - 1e-6) -> dict:
+        m = PlaneMetrics()
+        load().or_plane(self.h, x, gamma_min, C.byref(m))
+        return {k: getattr(m, k) for k, _ in m._fields_}
+
+    def full_dp(self, gamma_min: float = 1.0 - 1e-6) -> float:
+        return (self.plane(0.0, gamma_min)["avgTotal"]
+                - self.plane(self.N * self.prm.hx, gamma_min)["avgTotal"])
+
+    def system(self, which: int) -> dict:
+        """Last assembled system: 0 = U momentum, 1 = V momentum, 2 = p'."""
+        n = C.c_int(0)
+        nnz = C.c_int(0)
+        load().or_get_system(self.h, which, C.byref(n), C.byref(nnz), None, None, None, None, None, None, None)
+        rp = np.empty(n.value + 1, np.int32)
+        ci = np.empty(nnz.value, np.int32)
+        va = np.empty(nnz.value)
+        rhs = np.empty(n.value)
+        sol = np.empty(n.value)
+        ri = np.empty(n.value, np.int32)
+        rj = np.empty(n.value, np.int32)
+        load().or_get_system(self.h, which, None, None, _ptr(rp, _ip), _ptr(ci, _ip), _ptr(va, _dp),
+                             _ptr(rhs, _dp), _ptr(sol, _dp), _ptr(ri, _ip), _ptr(rj, _ip))
+        return dict(n=n.value, row_ptr=rp, col_idx=ci, values=va, rhs=rhs, sol=sol, ri=ri, rj=rj)
+
+
+def diag_offsets(row_ptr: np.ndarray, col_idx: np.ndarray) -> np.ndarray:
+    n = len(row_ptr) - 1
+    d = np.full(n, -1, np.int32)
+    for i in range(n):
+        for k in range(row_ptr[i], row_ptr[i + 1]):
+            if col_idx[k] == i:
+                d[i] = k
+                break
+    return d
+
+
+def jacobi(row_ptr, col_idx, values, b, x0, diag_off, tol, max_iter, fp32=True, impl="oracle"):
+    """Run a Jacobi solve: impl 'oracle' (restatement) or 'ref' (reference leaf)."""
+    n = len(row_ptr) - 1
+    rp = np.ascontiguousarray(row_ptr, np.int32)
+    ci = np.ascontiguousarray(col_idx, np.int32)
+    do = np.ascontiguousarray(diag_off, np.int32)
+    if fp32 or impl == "ref":
+        va = np.ascontiguousarray(values, np.float32)
+        bb = np.ascontiguousarray(b, np.float32)
+        x = np.array(x0, np.float32)
+        lib = load_ref() if impl == "ref" else load()
+        fn = lib.ref_jacobi if impl == "ref" else lib.or_jacobi_f32
+        it = fn(n, _ptr(rp, _ip), _ptr(ci, _ip), _ptr(va, _fp), _ptr(bb, _fp), _ptr(x, _fp), _ptr(do, _ip),
+                float(tol), int(max_iter))
+    else:
+        va = np.ascontiguousarray(values, np.float64)
+        bb = np.ascontiguousarray(b, np.float64)
+        x = np.array(x0, np.float64)
+        it = load().or_jacobi_f64(n, _ptr(rp, _ip), _ptr(ci, _ip), _ptr(va, _dp), _ptr(bb, _dp), _ptr(x, _dp),
+                                  _ptr(do, _ip), float(tol), int(max_iter))
+    return it, x
+
+
+def pcg(row_ptr, col_idx, values, b, x0, tol, max_iter, fp32=True, impl="oracle"):
+    """Run a PCG solve: impl 'oracle' (restatement) or 'ref' (reference leaf)."""
+    n = len(row_ptr) - 1
+    rp = np.ascontiguousarray(row_ptr, np.int32)
+    ci = np.ascontiguousarray(col_idx, np.int32)
+    if fp32 or impl == "ref":
+        va = np.ascontiguousarray(values, np.float32)
+        bb = np.ascontiguousarray(b, np.float32)
+        x = np.array(x0, np.float32)
+        lib = load_ref() if impl == "ref" else load()
+        fn = lib.ref_pcg if impl == "ref" else lib.or_pcg_f32
+        it = fn(n, _ptr(rp, _ip), _ptr(ci, _ip), _ptr(va, _fp), _ptr(bb, _fp), _ptr(x, _fp), float(tol),
+                int(max_iter))
+    else:
+        va = np.ascontiguousarray(values, np.float64)
+        bb = np.ascontiguousarray(b, np.float64)
+        x = np.array(x0, np.float64)
+        it = load().or_pcg_f64(n, _ptr(rp, _ip), _ptr(ci, _ip), _ptr(va, _dp), _ptr(bb, _dp), _ptr(x, _dp),
+                               float(tol), int(max_iter))
+    return it, x

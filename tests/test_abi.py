@@ -1,0 +1,103 @@
+"""C ABI checks that need no GPU: the library loads, exports every symbol
+declared in include/*.h, parameter defaults match the reference, host-only
+entry points behave, and errors come back as codes (no GPU calls)."""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+
+import cpbind as cb
+import workloads as wl
+
+REPO = Path(__file__).resolve().parent.parent
+
+
+def declared_symbols() -> set[str]:
+    names = set()
+    for h in (REPO / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[\w\s\*]+?\b(cp_\w+)\s*\(", text, flags=re.M):
+            if not m.group(1).startswith("cp_fill_"):
+                names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = cb.load()
+    names = declared_symbols()
+    assert len(names) >= 25
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    bound = {n for n, _, _ in cb.EXPORTS}
+    assert names == bound, (names ^ bound)
+
+
+def test_params_struct_layout_matches_header():
+    text = (REPO / "include" / "coldplate_hip.h").read_text()
+    body = text[text.index("typedef struct cp_params {") + len("typedef struct cp_params {"):text.index("} cp_params;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl or decl.startswith("typedef"):
+            continue
+        parts = decl.split(None, 1)
+        fields += [f.strip() for f in parts[1].split(",")]
+    assert [f for f, _ in cb.Params._fields_] == fields
+
+
+def test_reference_defaults():
+    p = cb.default_params()
+    # SIMPLE.h:68-358
+    assert p.epsilon == 1e-5 and p.maxIterations == 10000
+    assert p.maxPressureIter == 20 and p.pTol == 1e-4
+    assert p.momentumTol == 1e-5 and p.maxMomentumIter == 200
+    assert p.uvAlpha == 0.7 and p.pAlpha == 0.2
+    assert p.rho == 997.0 and p.mu == 0.00089
+    assert p.sinkMultiplier == 4.8 and p.KMin == 1e-14 and p.brinkmanQ == 1.0
+    assert p.enableSER == 1 and p.serCFLMax == 5.0 and p.serMinIter == 10
+    assert p.pseudoCFLInitial == 0.1 and p.pseudoCFLMax == 30.0
+    assert p.precision == cb.CP_PREC_FP64 and p.massResidualScope == 0
+
+
+def test_create_rejects_bad_arguments_without_gpu_work():
+    lib = cb.load()
+    p = cb.default_params()
+    p.M, p.N = 2, 2
+    h = C.c_void_p()
+    ct = np.zeros((2, 2))
+    rc = lib.cp_create(C.byref(p), cb.dptr(ct), C.byref(h))
+    assert rc == -1
+    assert b"at least 4" in lib.cp_last_error(None)
+
+
+def test_geometry_generators_deterministic_and_valid():
+    a = wl.geom_network(128, 256, 5)
+    b = wl.geom_network(128, 256, 5)
+    c = wl.geom_network(128, 256, 6)
+    np.testing.assert_array_equal(a, b)
+    assert np.any(a != c)
+    assert set(np.unique(a)) <= {0.0, 1.0}
+    assert np.all(a[0] == 1) and np.all(a[-1] == 1)
+    assert (a[:, 0] == 0).sum() >= 8 and (a[:, -1] == 0).sum() >= 8
+    g = wl.geom_gamma(96, 128, 3, corr=8.0)
+    np.testing.assert_array_equal(g, wl.geom_gamma(96, 128, 3, corr=8.0))
+    assert g.min() >= 0 and g.max() <= 1
+    assert 0.2 < (g < 0.5).mean() < 0.8  # roughly half fluid
+    s = wl.geom_straight(50, 200, 10, 39)
+    assert s[10:40].sum() == 0 and s[:10].min() == 1 and s[40:].min() == 1
+
+
+def test_stream_and_batch_workload_shapes():
+    reqs = wl.stream_2000(50)
+    assert len(reqs) == 50
+    for r in reqs:
+        assert 32 <= r["M"] <= 1024 and 32 <= r["N"] <= 2048
+    b64 = wl.batch_64()
+    assert len(b64) == 64
+    for p, ct in b64:
+        assert ct.shape == (p.M, p.N)
+        assert 128 <= p.N <= 1024 and p.M == max(16, (p.N // 2 // 8) * 8)

@@ -1,0 +1,127 @@
+"""GPU tests of the batched form: packing, graph replay, identity alone vs
+mixed, admission by footprint, eviction."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import cpbind as cb
+import oracle_py as op
+import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg: str) -> None:
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def geoms():
+    a = wl.network(64, 128, 21, cb.CP_PREC_MIXED)
+    b = wl.gamma_field(40, 96, 5, cb.CP_PREC_MIXED, corr=5.0)
+    c = wl.straight_200x50(cb.CP_PREC_MIXED)
+    return [a, b, c]
+
+
+@pytest.mark.parametrize("precision", [cb.CP_PREC_FP64, cb.CP_PREC_MIXED], ids=["fp64", "mixed"])
+def test_alone_equals_mixed_bitwise(precision):
+    """The same geometry gives bit-identical fields alone and packed with
+    others (per-slot reduction order depends only on the slot's shape)."""
+    gs = []
+    for p, ct in geoms():
+        q = wl.fast_mode(p) if precision == cb.CP_PREC_MIXED else p.copy()
+        q.precision = precision
+        gs.append((q, ct))
+    say(f"[batch] alone vs mixed, precision {precision}")
+    alone = cb.Batch(precision, 256 << 20, 4)
+    s0 = alone.admit(*gs[0])
+    alone.run(25)
+    ua, va, pa = alone.fields(s0)
+    mixed = cb.Batch(precision, 256 << 20, 4)
+    order = [mixed.admit(*gs[2]), mixed.admit(*gs[0]), mixed.admit(*gs[1])]
+    mixed.run(25)
+    ub, vb, pb = mixed.fields(order[1])
+    np.testing.assert_array_equal(ua, ub)
+    np.testing.assert_array_equal(va, vb)
+    np.testing.assert_array_equal(pa, pb)
+    info = mixed.poll()
+    assert info[order[1]]["iter"] == 25
+
+
+def test_graph_batch_matches_polled_drop_in():
+    """FP64, reference budgets: 20 graph-replayed outer iterations of a
+    one-slot batch equal 20 polled cp_step calls."""
+    p, ct = wl.straight_200x50()
+    say("[batch] graph vs polled FP64")
+    b = cb.Batch(cb.CP_PREC_FP64, 64 << 20, 1)
+    s = b.admit(p, ct)
+    b.run(20)
+    g = cb.Solver(p, ct)
+    for _ in range(20):
+        g.step()
+    u, v, pp = b.fields(s)
+    np.testing.assert_allclose(u, g.field(cb.F_U), rtol=0, atol=1e-12 * np.abs(u).max())
+    np.testing.assert_allclose(pp, g.field(cb.F_P), rtol=0, atol=1e-12 * np.abs(pp).max())
+
+
+def test_batch_fp64_matches_oracle():
+    p, ct = wl.network(64, 128, 3)
+    b = cb.Batch(cb.CP_PREC_FP64, 64 << 20, 2)
+    s = b.admit(p, ct)
+    b.run(12)
+    o = op.Oracle(p, ct)
+    o.run(12)
+    u, v, pp = b.fields(s)
+    for a, f in ((u, cb.F_U), (v, cb.F_V), (pp, cb.F_P)):
+        ref = o.field(f)
+        assert np.abs(a - ref).max() <= 1e-8 * np.abs(ref).max()
+
+
+def test_admission_by_footprint_and_evict():
+    p, ct = wl.network(64, 128, 9, cb.CP_PREC_MIXED)
+    q = wl.fast_mode(p)
+    b = cb.Batch(cb.CP_PREC_MIXED, 64 << 20, 8)
+    fp = b.footprint(q.M, q.N)
+    cap = (64 << 20) // fp
+    slots = []
+    while True:
+        s = b.admit(q, ct)
+        if s is None:
+            break
+        slots.append(s)
+    assert 1 <= len(slots) <= min(8, cap)
+    b.run(3)
+    info = b.poll()
+    assert all(info[s]["status"] == cb.SLOT_RUNNING for s in slots)
+    b.evict(slots[0])
+    assert b.admit(q, ct) == slots[0]
+    b.run(2)
+    info = b.poll()
+    assert info[slots[0]]["iter"] == 2 and info[slots[1]]["iter"] == 5
+
+
+def test_converged_slot_is_frozen():
+    """A converged slot is skipped by later launches: its fields stop
+    changing while other slots keep running."""
+    p, ct = wl.straight_200x50(cb.CP_PREC_MIXED)
+    q = wl.fast_mode(p)
+    q.epsilon = 1e-3
+    r = wl.fast_mode(p)
+    r.epsilon = 1e-12
+    b = cb.Batch(cb.CP_PREC_MIXED, 128 << 20, 2)
+    s0, s1 = b.admit(q, ct), b.admit(r, ct)
+    for _ in range(40):
+        b.run(5)
+        if b.poll()[s0]["status"] == cb.SLOT_CONVERGED:
+            break
+    info = b.poll()
+    assert info[s0]["status"] == cb.SLOT_CONVERGED
+    it0 = info[s0]["iter"]
+    u0 = b.fields(s0)[0]
+    b.run(10)
+    info = b.poll()
+    assert info[s0]["iter"] == it0 and info[s1]["iter"] > it0
+    np.testing.assert_array_equal(b.fields(s0)[0], u0)

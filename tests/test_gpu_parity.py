@@ -1,0 +1,204 @@
+"""GPU parity: the HIP backend against the FP64 oracle, through the C ABI.
+
+Every test runs the same seeded inputs through the oracle (CPU, FP64) and
+the device library, and compares sub-step by sub-step or at convergence.
+Tolerances:
+  * FP64 mode: fields within 1e-9 of their scale per step (different
+    summation orders only), identical inner iteration counts;
+  * mixed mode: the converged full-system pressure drop within 1e-4
+    relative (the target accuracy of the mixed-precision form).
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import cpbind as cb
+import oracle_py as op
+import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+F_ALL = [cb.F_U, cb.F_V, cb.F_P, cb.F_USTAR, cb.F_VSTAR, cb.F_PSTAR, cb.F_DE, cb.F_DN]
+NAMES = {cb.F_U: "u", cb.F_V: "v", cb.F_P: "p", cb.F_USTAR: "u*", cb.F_VSTAR: "v*", cb.F_PSTAR: "p'",
+         cb.F_DE: "dE", cb.F_DN: "dN"}
+
+
+def say(msg: str) -> None:
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    cb.load()
+    if cb.load().cp_device_count() < 1:
+        pytest.fail("no HIP device: GPU tests need an MI355X")
+
+
+def rel_err(a: np.ndarray, b: np.ndarray) -> float:
+    scale = max(np.abs(b).max(), 1e-300)
+    return float(np.abs(a - b).max() / scale)
+
+
+def compare_fields(gpu: cb.Solver, orc: op.Oracle, fields, tol: float, tag: str) -> None:
+    for f in fields:
+        g, o = gpu.field(f), orc.field(f)
+        e = rel_err(g, o)
+        assert e <= tol, f"{tag}: field {NAMES[f]} rel err {e:.3e} > {tol:.1e}"
+
+
+def small_straight():
+    p = wl.base_params(20, 60, 5e-5, 0.1)
+    return p, wl.geom_straight(20, 60, 5, 14)
+
+
+def test_substeps_straight_fp64():
+    """Config-1 geometry, FP64: momentum, pressure and correction sub-steps."""
+    p, ct = wl.straight_200x50()
+    say(f"[parity] straight 200x50 FP64 sub-steps")
+    g = cb.Solver(p, ct)
+    o = op.Oracle(p, ct)
+    for step in range(6):
+        sg, so = g.momentum(), o.momentum()
+        assert (sg["itU"], sg["itV"]) == (so["itU"], so["itV"]), f"step {step}: Jacobi sweeps {sg} vs {so}"
+        np.testing.assert_allclose([sg["residU"], sg["residV"]], [so["residU"], so["residV"]], rtol=1e-9, atol=1e-18)
+        compare_fields(g, o, [cb.F_USTAR, cb.F_VSTAR, cb.F_DE, cb.F_DN], 1e-10, f"step {step} momentum")
+        sg, so = g.pressure(), o.pressure()
+        assert sg["itP"] == so["itP"], f"step {step}: PCG iterations {sg['itP']} vs {so['itP']}"
+        np.testing.assert_allclose(sg["residMass"], so["residMass"], rtol=1e-9)
+        compare_fields(g, o, [cb.F_PSTAR], 1e-9, f"step {step} pressure")
+        g.correct()
+        o.correct()
+        compare_fields(g, o, [cb.F_U, cb.F_V, cb.F_P], 1e-9, f"step {step} correct")
+
+
+def test_outer_steps_network_sou_pseudo_fp64():
+    """SOU + 2.5D + convection scaling + pseudo-time with SER and line
+    search: 16 outer steps (SER starts at step 10)."""
+    p, ct = wl.network(64, 128, 7)
+    say("[parity] network 128x64 SOU/2.5D/pseudo FP64 outer steps")
+    g = cb.Solver(p, ct)
+    o = op.Oracle(p, ct)
+    for step in range(16):
+        sg, so = g.step(), o.step()
+        for k in ("itU", "itV", "itP", "iter"):
+            assert sg[k] == so[k], f"step {step}: {k} {sg[k]} vs {so[k]}"
+        for k in ("residU", "residV", "residMass", "pseudoCFL", "error"):
+            np.testing.assert_allclose(sg[k], so[k], rtol=1e-8, err_msg=f"step {step}: {k}")
+        compare_fields(g, o, F_ALL, 1e-8, f"step {step}")
+
+
+def test_gamma_field_fp64():
+    """Continuous gamma field with alpha_max = 1e9 and the 2.5D sink."""
+    p, ct = wl.gamma_field(48, 80, 11, cb.CP_PREC_FP64, corr=6.0)
+    say("[parity] gamma field 80x48 FP64")
+    g = cb.Solver(p, ct)
+    o = op.Oracle(p, ct)
+    np.testing.assert_array_equal(g.field(cb.F_GAMMA), o.field(cb.F_GAMMA))
+    np.testing.assert_allclose(g.field(cb.F_ALPHA), o.field(cb.F_ALPHA), rtol=1e-15)
+    for step in range(10):
+        sg, so = g.step(), o.step()
+        assert (sg["itU"], sg["itV"], sg["itP"]) == (so["itU"], so["itV"], so["itP"]), f"step {step}"
+    compare_fields(g, o, F_ALL, 1e-8, "after 10 steps")
+
+
+def test_long_run_straight_fp64():
+    """600 outer iterations of the config-1 case: fields and full-system
+    pressure drop stay on the oracle's trajectory."""
+    p, ct = wl.straight_200x50()
+    say("[parity] straight 200x50 FP64 600 steps")
+    g = cb.Solver(p, ct)
+    o = op.Oracle(p, ct)
+    for _ in range(600):
+        sg = g.step()
+    n, so, _ = o.run(600)
+    assert sg["iter"] == so["iter"] == 600
+    compare_fields(g, o, [cb.F_U, cb.F_V, cb.F_P], 1e-7, "600 steps")
+    np.testing.assert_allclose(g.full_dp(), o.full_dp(), rtol=1e-7)
+
+
+def test_plane_metrics_fp64():
+    p, ct = wl.straight_200x50()
+    g = cb.Solver(p, ct)
+    o = op.Oracle(p, ct)
+    for _ in range(40):
+        g.step()
+        o.step()
+    for x in (0.0, 1.23e-3, 5e-3, 7.77e-3, 200 * 5e-5, 0.02):
+        mg, mo = g.plane(x), o.plane(x)
+        assert mg["valid"] == mo["valid"]
+        for k in ("flowArea", "massFlux", "avgStatic", "avgDynamic", "avgTotal"):
+            np.testing.assert_allclose(mg[k], mo[k], rtol=1e-9, atol=1e-14, err_msg=f"x={x} {k}")
+
+
+def test_restart_fields_roundtrip():
+    """cp_set_fields / cp_get_fields restart semantics (SIMPLE.cpp:203-232)."""
+    p, ct = small_straight()
+    g = cb.Solver(p, ct)
+    o = op.Oracle(p, ct)
+    rng = np.random.default_rng(3)
+    u = rng.normal(0, 0.05, (21, 60))
+    v = rng.normal(0, 0.01, (20, 61))
+    pp = rng.normal(0, 1.0, (21, 61))
+    g.set_fields(u, v, pp)
+    o.set_fields(u, v, pp)
+    np.testing.assert_array_equal(g.field(cb.F_U), u)
+    np.testing.assert_array_equal(g.field(cb.F_USTAR), u)
+    np.testing.assert_array_equal(g.field(cb.F_P), pp)
+    for step in range(3):
+        sg, so = g.step(), o.step()
+        assert (sg["itU"], sg["itV"], sg["itP"]) == (so["itU"], so["itV"], so["itP"])
+    compare_fields(g, o, F_ALL, 1e-8, "restart")
+
+
+def test_zero_inflow_gpu():
+    p, ct = small_straight()
+    p.U_in = 0.0
+    g = cb.Solver(p, ct)
+    for _ in range(5):
+        st = g.step()
+    assert st["error"] == 0.0
+    for f in (cb.F_U, cb.F_V, cb.F_P):
+        assert np.abs(g.field(f)).max() == 0.0
+
+
+def test_mixed_converges_to_oracle_dp():
+    """Mixed precision (FP32 stencil/Jacobi/PCG + FP64 refinement) on the
+    config-1 case: converged pressure drop within 1e-4 of the FP64 oracle's
+    converged value."""
+    p, ct = wl.straight_200x50()
+    ref = p.copy()
+    ref.massResidualScope = 1
+    ref.epsilon = 1e-10
+    ref.pTol, ref.maxPressureIter = 1e-12, 500
+    o = op.Oracle(ref, ct)
+    o.run(20000)
+    dp_ref = o.full_dp()
+    q = wl.fast_mode(p)
+    q.epsilon = 1e-7
+    say("[parity] straight 200x50 mixed run to convergence")
+    g = cb.Solver(q, ct)
+    st = g.run()
+    assert st["converged"] == 1, st
+    dp = g.full_dp()
+    assert abs(dp / dp_ref - 1) <= 1e-4, (dp, dp_ref)
+    u_ref = o.field(cb.F_U)
+    assert rel_err(g.field(cb.F_U), u_ref) <= 1e-3
+
+
+def test_fp64_health_flag_on_nonfinite():
+    """A NaN written into the state is caught by the health word and the
+    solve stops as failed (the reference would run on to maxIterations)."""
+    p, ct = small_straight()
+    g = cb.Solver(p, ct)
+    g.step()
+    u = g.field(cb.F_U)
+    u[5, 10] = np.nan
+    g.set_fields(u=u)
+    st = g.step()
+    assert st["health"] != 0
+    with pytest.raises(cb.CpError):
+        g.step()
