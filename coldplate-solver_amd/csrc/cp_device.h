@@ -91,8 +91,10 @@ struct KArgs {
     const int* tileSlot;
     const int* nTiles;
     double* partials;
-    unsigned* counters;
 };
+
+// Finalisation phases (k_finalize), one per reducing work kernel.
+enum Phase { PH_JAC = 0, PH_MOM_FIN, PH_PRES_ASM, PH_PCG_Q, PH_PCG_U, PH_REF_ACC, PH_REF_RES, PH_CORR };
 
 // Kernels of one SIMPLE step, in launch order.
 enum Kern {
@@ -108,8 +110,10 @@ enum Kern {
     K_CORR_OUTER,   // K_CORR + outer-loop scalar logic
 };
 
-// Host-side launchers (defined next to the kernels).
-hipError_t launchKernel(Kern k, bool f32, int grid, hipStream_t s, const KArgs& a);
+// Host-side launchers (defined next to the kernels). launchKernel enqueues
+// the work kernel on `grid` workgroups and, when it reduces, the per-slot
+// finalisation on `nslots` workgroups.
+hipError_t launchKernel(Kern k, bool f32, int grid, int nslots, hipStream_t s, const KArgs& a);
 hipError_t launchPlane(int nreq, hipStream_t s, const SlotDesc* desc, const PlaneReq* req, PlaneOut* out);
 
 }  // namespace cpdev

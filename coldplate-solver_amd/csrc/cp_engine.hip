@@ -148,7 +148,6 @@ struct cp_batch {
     int* dTileSlot = nullptr;
     int* dNTiles = nullptr;
     double* dPartials = nullptr;
-    unsigned* dCounters = nullptr;
     PlaneReq* dPlaneReq = nullptr;
     PlaneOut* dPlaneOut = nullptr;
     // host mirrors
@@ -194,12 +193,11 @@ KArgs kargs(cp_batch* b) {
     a.tileSlot = b->dTileSlot;
     a.nTiles = b->dNTiles;
     a.partials = b->dPartials;
-    a.counters = b->dCounters;
     return a;
 }
 
 int launch(cp_batch* b, Kern k, int grid) {
-    const hipError_t e = launchKernel(k, b->precision == CP_PREC_MIXED, grid, b->stream, kargs(b));
+    const hipError_t e = launchKernel(k, b->precision == CP_PREC_MIXED, grid, b->maxSlots, b->stream, kargs(b));
     if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
     return CP_OK;
 }
@@ -419,7 +417,6 @@ int batchCreate(int precision, size_t budget, int maxSlots, cp_batch** out, std:
     HIP_TRY(hipMalloc(&b->dTileSlot, sizeof(int) * b->maxTiles));
     HIP_TRY(hipMalloc(&b->dNTiles, sizeof(int)));
     HIP_TRY(hipMalloc(&b->dPartials, sizeof(double) * kNumPartials * b->maxTiles));
-    HIP_TRY(hipMalloc(&b->dCounters, sizeof(unsigned) * maxSlots));
     HIP_TRY(hipMalloc(&b->dPlaneReq, sizeof(PlaneReq) * 2 * maxSlots));
     HIP_TRY(hipMalloc(&b->dPlaneOut, sizeof(PlaneOut) * 2 * maxSlots));
     HIP_TRY(hipHostMalloc(&b->hState, sizeof(SlotState) * maxSlots, hipHostMallocDefault));
@@ -429,7 +426,6 @@ int batchCreate(int precision, size_t budget, int maxSlots, cp_batch** out, std:
     b->hDesc.assign(maxSlots, SlotDesc());
     memset(b->hDesc.data(), 0, sizeof(SlotDesc) * maxSlots);
     b->slots.assign(maxSlots, SlotHost());
-    HIP_TRY(hipMemsetAsync(b->dCounters, 0, sizeof(unsigned) * maxSlots, b->stream));
     HIP_TRY(hipMemcpyAsync(b->dState, b->hState, sizeof(SlotState) * maxSlots, hipMemcpyHostToDevice, b->stream));
     const int zero = 0;
     HIP_TRY(hipMemcpyAsync(b->dNTiles, &zero, sizeof(int), hipMemcpyHostToDevice, b->stream));
@@ -448,7 +444,6 @@ void batchDestroy(cp_batch* b) {
     (void)hipFree(b->dTileSlot);
     (void)hipFree(b->dNTiles);
     (void)hipFree(b->dPartials);
-    (void)hipFree(b->dCounters);
     (void)hipFree(b->dPlaneReq);
     (void)hipFree(b->dPlaneOut);
     (void)hipHostFree(b->hState);

@@ -1,14 +1,16 @@
 // cp_kernels.hip — gfx950 kernels of the SIMPLE backend.
 //
-// Every kernel is a persistent grid-stride loop over the batch's tile list
-// (tile -> slot table), so one launch covers every running geometry of a
-// batch, and the launch shape never depends on which slots are admitted
+// Every work kernel is a persistent grid-stride loop over the batch's tile
+// list (tile -> slot table), so one launch covers every running geometry of
+// a batch, and the launch shape never depends on which slots are admitted
 // (the sequence can be captured once in a hipGraph). Per-slot reductions are
-// deterministic: each tile writes its block-reduced partials, and the tile
-// that arrives last for its slot sums that slot's partials in tile order and
-// then runs the slot's scalar logic (stop tests, CFL control). The reduction
-// order depends only on the slot's own shape, so a geometry gives the same
-// result alone or packed with others.
+// deterministic and fence-free: each tile writes its block-reduced partials,
+// and k_finalize (one workgroup per slot, launched next) sums that slot's
+// partials in tile order and runs the slot's scalar logic (stop tests, CG
+// coefficients, CFL control). The kernel boundary makes the partials
+// visible; no tile ever waits on another. The reduction order depends only
+// on the slot's own shape, so a geometry gives the same result alone or
+// packed with others.
 //
 // T is the solver type: double (FP64 mode) or float (mixed mode). State
 // arrays (u, v, p, u*, v*, p', dE, dN, alpha faces, gamma, mass imbalance)
@@ -63,25 +65,6 @@ __device__ __forceinline__ void blockReduceTo(double (&v)[NQ], double* dst) {
     }
 }
 
-// Tile arrival: returns true (in every thread) for the last tile of slot s.
-__device__ __forceinline__ bool arriveLast(unsigned* ctr, int ntiles) {
-    __shared__ int isLast;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned prev = atomicAdd(ctr, 1u);
-        isLast = (prev == unsigned(ntiles - 1)) ? 1 : 0;
-    }
-    __syncthreads();
-    const bool last = isLast != 0;
-    if (last) __threadfence();
-    return last;
-}
-
-__device__ __forceinline__ void resetCounter(unsigned* ctr) {
-    if (threadIdx.x == 0) atomicExch(ctr, 0u);
-}
-
 // Sums (or maxes) the partials of one slot's tiles in tile order; the result
 // is valid in every thread of the block.
 template <int NQ, unsigned MaxMask>
@@ -94,7 +77,7 @@ __device__ __forceinline__ void slotReduce(const double* partials, int tile0, in
         const double* src = partials + size_t(tile0 + k) * kNumPartials;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            const double x = __hip_atomic_load(src + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const double x = src[q];
             v[q] = ((MaxMask >> q) & 1u) ? nanmax(v[q], x) : v[q] + x;
         }
     }
@@ -241,14 +224,13 @@ __global__ __launch_bounds__(kBlock) void k_mom_assemble(KArgs a) {
             if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2) momRow<T, true>(d, k, i, j, maxVel, cfl);
             if (i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1) momRow<T, false>(d, k, i, j, maxVel, cfl);
         }
-        if (arriveLast(&a.counters[tl.s], d.ntiles)) {
-            if (threadIdx.x == 0) {
-                const int cap = d.maxMomIter;
-                st.jk[0] = st.jk[1] = 0;
-                st.jiters[0] = st.jiters[1] = 0;
-                st.jdone[0] = st.jdone[1] = (cap <= 0) ? 1 : 0;
-            }
-            resetCounter(&a.counters[tl.s]);
+        // the workgroup holding the slot's first tile resets the Jacobi state
+        // (no tile of this kernel reads it)
+        if (t == d.tile0 && threadIdx.x == 0) {
+            const int cap = d.maxMomIter;
+            st.jk[0] = st.jk[1] = 0;
+            st.jiters[0] = st.jiters[1] = 0;
+            st.jdone[0] = st.jdone[1] = (cap <= 0) ? 1 : 0;
         }
     }
 }
@@ -294,26 +276,6 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(KArgs a) {
                 v[1] = sweepAt<T>(d.mv, k, d.pitch, st.jk[1] & 1);
         }
         blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials);
-        if (arriveLast(&a.counters[tl.s], d.ntiles)) {
-            double tot[2];
-            slotReduce<2, 0u>(a.partials, d.tile0, d.ntiles, tot);
-            if (threadIdx.x == 0) {
-                const double tol2 = d.momTol * d.momTol;
-                for (int c = 0; c < 2; ++c) {
-                    if (st.jdone[c]) continue;
-                    st.jk[c] += 1;
-                    const double n = double(c == 0 ? d.nU : d.nV);
-                    if (tot[c] < tol2 * n) {  // jacobi_solver.cpp:63
-                        st.jdone[c] = 1;
-                        st.jiters[c] = st.jk[c];
-                    } else if (st.jk[c] >= d.maxMomIter) {
-                        st.jdone[c] = 1;
-                        st.jiters[c] = d.maxMomIter;
-                    }
-                }
-            }
-            resetCounter(&a.counters[tl.s]);
-        }
     }
 }
 
@@ -369,18 +331,6 @@ __global__ __launch_bounds__(kBlock) void k_mom_finish(KArgs a) {
             }
         }
         blockReduceTo<3, 7u>(v, a.partials + size_t(t) * kNumPartials);
-        if (arriveLast(&a.counters[tl.s], d.ntiles)) {
-            double tot[3];
-            slotReduce<3, 7u>(a.partials, d.tile0, d.ntiles, tot);
-            if (threadIdx.x == 0) {
-                st.residU = tot[0];
-                st.residV = tot[1];
-                if (tot[2] != 0.0) st.health = 1;
-                st.itU = st.jiters[0];
-                st.itV = st.jiters[1];
-            }
-            resetCounter(&a.counters[tl.s]);
-        }
     }
 }
 
@@ -426,22 +376,6 @@ __global__ __launch_bounds__(kBlock) void k_pres_assemble(KArgs a) {
             }
         }
         blockReduceTo<2, 1u>(v, a.partials + size_t(t) * kNumPartials);
-        if (arriveLast(&a.counters[tl.s], d.ntiles)) {
-            double tot[2];
-            slotReduce<2, 1u>(a.partials, d.tile0, d.ntiles, tot);
-            if (threadIdx.x == 0) {
-                st.residMass = tot[0];
-                st.rz = tot[1];
-                st.beta = 0.0;
-                st.pk = 0;
-                st.piters = 0;
-                st.pitTotal = 0;
-                st.pcycle = 0;
-                st.pdone = (d.maxPresIter <= 0) ? 1 : 0;
-                st.stepPhase = 0;  // refinement not finished
-            }
-            resetCounter(&a.counters[tl.s]);
-        }
     }
 }
 
@@ -490,19 +424,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_q(KArgs a) {
             v[0] = double(pC) * double(q);
         }
         blockReduceTo<1, 0u>(v, a.partials + size_t(t) * kNumPartials);
-        if (arriveLast(&a.counters[tl.s], d.ntiles)) {
-            double tot[1];
-            slotReduce<1, 0u>(a.partials, d.tile0, d.ntiles, tot);
-            if (threadIdx.x == 0) {
-                if (fabs(tot[0]) < 1e-20) {  // pressure_solver.cpp:158
-                    st.pdone = 1;
-                    st.piters = st.pk;
-                } else {
-                    st.alpha = st.rz / tot[0];
-                }
-            }
-            resetCounter(&a.counters[tl.s]);
-        }
     }
 }
 
@@ -536,26 +457,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
             v[1] = double(rn) * double(rn / aP);
         }
         blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials);
-        if (arriveLast(&a.counters[tl.s], d.ntiles)) {
-            double tot[2];
-            slotReduce<2, 0u>(a.partials, d.tile0, d.ntiles, tot);
-            if (threadIdx.x == 0) {
-                st.rr = tot[0];
-                if (sqrt(tot[0]) < d.pTol) {
-                    st.pdone = 1;
-                    st.piters = st.pk;
-                } else {
-                    st.beta = tot[1] / st.rz;
-                    st.rz = tot[1];
-                    st.pk += 1;
-                    if (st.pk >= d.maxPresIter) {
-                        st.pdone = 1;
-                        st.piters = d.maxPresIter;
-                    }
-                }
-            }
-            resetCounter(&a.counters[tl.s]);
-        }
     }
 }
 
@@ -575,13 +476,6 @@ __global__ __launch_bounds__(kBlock) void k_refine_accum(KArgs a) {
         if (isPDof(i, j, M, N)) {
             const size_t k = size_t(i) * P + j;
             d.pc[k] += double(static_cast<const T*>(d.pr[P_X])[k]);
-        }
-        if (arriveLast(&a.counters[tl.s], d.ntiles)) {
-            if (threadIdx.x == 0) {
-                st.pitTotal += st.piters;
-                if (st.pcycle + 1 >= d.refineCycles) st.stepPhase = 1;  // refinement finished
-            }
-            resetCounter(&a.counters[tl.s]);
         }
     }
 }
@@ -614,23 +508,6 @@ __global__ __launch_bounds__(kBlock) void k_refine_resid(KArgs a) {
             v[1] = double(r) * double(r / aP[k]);
         }
         blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials);
-        if (arriveLast(&a.counters[tl.s], d.ntiles)) {
-            double tot[2];
-            slotReduce<2, 0u>(a.partials, d.tile0, d.ntiles, tot);
-            if (threadIdx.x == 0) {
-                st.pcycle += 1;
-                if (sqrt(tot[0]) < d.pTol) {
-                    st.stepPhase = 1;
-                } else {
-                    st.rz = tot[1];
-                    st.beta = 0.0;
-                    st.pk = 0;
-                    st.piters = 0;
-                    st.pdone = 0;
-                }
-            }
-            resetCounter(&a.counters[tl.s]);
-        }
     }
 }
 
@@ -706,10 +583,10 @@ __device__ void controller(const SlotDesc& d, SlotState& st) {
 
 // ---------------------------------------------------------------------------
 // p += pAlpha p', pressure BCs, velocity correction, velocity BCs
-// (iterations.cpp:456-500, boundaries.cpp:48-134), then (when `outer`) the
-// slot's outer-loop scalar logic.
+// (iterations.cpp:456-500, boundaries.cpp:48-134). Its finalisation runs the
+// slot's outer-loop scalar logic when the step is an outer iteration.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_correct(KArgs a, int outer) {
+__global__ __launch_bounds__(kBlock) void k_correct(KArgs a) {
     const int nT = *a.nTiles;
     for (int t = blockIdx.x; t < nT; t += gridDim.x) {
         const Tile tl = decodeTile(a, t);
@@ -759,16 +636,6 @@ __global__ __launch_bounds__(kBlock) void k_correct(KArgs a, int outer) {
         }
         double vb[1] = {bad};
         blockReduceTo<1, 1u>(vb, a.partials + size_t(t) * kNumPartials);
-        if (arriveLast(&a.counters[tl.s], d.ntiles)) {
-            double tot[1];
-            slotReduce<1, 1u>(a.partials, d.tile0, d.ntiles, tot);
-            if (threadIdx.x == 0) {
-                if (tot[0] != 0.0) st.health = 1;
-                st.itP = d.mixed ? st.pitTotal : st.piters;
-                if (outer) controller(d, st);
-            }
-            resetCounter(&a.counters[tl.s]);
-        }
     }
 }
 
@@ -843,6 +710,153 @@ __global__ __launch_bounds__(kBlock) void k_plane(const SlotDesc* desc, const Pl
 }
 
 // ---------------------------------------------------------------------------
+// Per-slot finalisation of a work kernel: one workgroup per slot sums the
+// slot's tile partials in tile order (deterministic) and updates the slot's
+// scalars. Launched right after the work kernel of `phase`; it applies the
+// same skip rule as that kernel, so it only reads partials written by it.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int outer) {
+    const int s = blockIdx.x;
+    const SlotDesc& d = a.desc[s];
+    SlotState& st = a.state[s];
+    if (d.ntiles == 0 || st.status != ST_RUNNING) return;
+    switch (phase) {
+        case PH_JAC:
+            if (st.jdone[0] && st.jdone[1]) return;
+            break;
+        case PH_PCG_Q:
+        case PH_PCG_U:
+            if (st.pdone || st.stepPhase) return;
+            break;
+        case PH_REF_ACC:
+        case PH_REF_RES:
+            if (st.stepPhase) return;
+            break;
+        default:
+            break;
+    }
+    switch (phase) {
+        case PH_JAC: {
+            double tot[2];
+            slotReduce<2, 0u>(a.partials, d.tile0, d.ntiles, tot);
+            if (threadIdx.x == 0) {
+                const double tol2 = d.momTol * d.momTol;
+                for (int c = 0; c < 2; ++c) {
+                    if (st.jdone[c]) continue;
+                    st.jk[c] += 1;
+                    const double n = double(c == 0 ? d.nU : d.nV);
+                    if (tot[c] < tol2 * n) {  // jacobi_solver.cpp:63
+                        st.jdone[c] = 1;
+                        st.jiters[c] = st.jk[c];
+                    } else if (st.jk[c] >= d.maxMomIter) {
+                        st.jdone[c] = 1;
+                        st.jiters[c] = d.maxMomIter;
+                    }
+                }
+            }
+            break;
+        }
+        case PH_MOM_FIN: {
+            double tot[3];
+            slotReduce<3, 7u>(a.partials, d.tile0, d.ntiles, tot);
+            if (threadIdx.x == 0) {
+                st.residU = tot[0];
+                st.residV = tot[1];
+                if (tot[2] != 0.0) st.health = 1;
+                st.itU = st.jiters[0];
+                st.itV = st.jiters[1];
+            }
+            break;
+        }
+        case PH_PRES_ASM: {
+            double tot[2];
+            slotReduce<2, 1u>(a.partials, d.tile0, d.ntiles, tot);
+            if (threadIdx.x == 0) {
+                st.residMass = tot[0];
+                st.rz = tot[1];
+                st.beta = 0.0;
+                st.pk = 0;
+                st.piters = 0;
+                st.pitTotal = 0;
+                st.pcycle = 0;
+                st.pdone = (d.maxPresIter <= 0) ? 1 : 0;
+                st.stepPhase = 0;  // refinement not finished
+            }
+            break;
+        }
+        case PH_PCG_Q: {
+            double tot[1];
+            slotReduce<1, 0u>(a.partials, d.tile0, d.ntiles, tot);
+            if (threadIdx.x == 0) {
+                if (fabs(tot[0]) < 1e-20) {  // pressure_solver.cpp:158
+                    st.pdone = 1;
+                    st.piters = st.pk;
+                } else {
+                    st.alpha = st.rz / tot[0];
+                }
+            }
+            break;
+        }
+        case PH_PCG_U: {
+            double tot[2];
+            slotReduce<2, 0u>(a.partials, d.tile0, d.ntiles, tot);
+            if (threadIdx.x == 0) {
+                st.rr = tot[0];
+                if (sqrt(tot[0]) < d.pTol) {
+                    st.pdone = 1;
+                    st.piters = st.pk;
+                } else {
+                    st.beta = tot[1] / st.rz;
+                    st.rz = tot[1];
+                    st.pk += 1;
+                    if (st.pk >= d.maxPresIter) {
+                        st.pdone = 1;
+                        st.piters = d.maxPresIter;
+                    }
+                }
+            }
+            break;
+        }
+        case PH_REF_ACC: {
+            if (threadIdx.x == 0) {
+                st.pitTotal += st.piters;
+                if (st.pcycle + 1 >= d.refineCycles) st.stepPhase = 1;  // refinement finished
+            }
+            break;
+        }
+        case PH_REF_RES: {
+            double tot[2];
+            slotReduce<2, 0u>(a.partials, d.tile0, d.ntiles, tot);
+            if (threadIdx.x == 0) {
+                st.pcycle += 1;
+                if (sqrt(tot[0]) < d.pTol) {
+                    st.stepPhase = 1;
+                } else {
+                    st.rz = tot[1];
+                    st.beta = 0.0;
+                    st.pk = 0;
+                    st.piters = 0;
+                    st.pdone = 0;
+                }
+            }
+            break;
+        }
+        case PH_CORR: {
+            double tot[1];
+            slotReduce<1, 1u>(a.partials, d.tile0, d.ntiles, tot);
+            if (threadIdx.x == 0) {
+                if (tot[0] != 0.0) st.health = 1;
+                st.itP = d.mixed ? st.pitTotal : st.piters;
+                if (outer) controller(d, st);
+            }
+            break;
+        }
+        default:
+            break;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -857,14 +871,33 @@ static void launchT(Kern k, int grid, hipStream_t s, const KArgs& a) {
         case K_PCG_U: hipLaunchKernelGGL(k_pcg_update<T>, g, b, 0, s, a); break;
         case K_REF_ACC: hipLaunchKernelGGL(k_refine_accum<T>, g, b, 0, s, a); break;
         case K_REF_RES: hipLaunchKernelGGL(k_refine_resid<T>, g, b, 0, s, a); break;
-        case K_CORR: hipLaunchKernelGGL(k_correct, g, b, 0, s, a, 0); break;
-        case K_CORR_OUTER: hipLaunchKernelGGL(k_correct, g, b, 0, s, a, 1); break;
+        case K_CORR:
+        case K_CORR_OUTER: hipLaunchKernelGGL(k_correct, g, b, 0, s, a); break;
     }
 }
 
-hipError_t launchKernel(Kern k, bool f32, int grid, hipStream_t s, const KArgs& a) {
+// finalisation phase of each work kernel (-1: none)
+static int phaseOf(Kern k) {
+    switch (k) {
+        case K_JAC: return PH_JAC;
+        case K_MOM_FIN: return PH_MOM_FIN;
+        case K_PRES_ASM: return PH_PRES_ASM;
+        case K_PCG_Q: return PH_PCG_Q;
+        case K_PCG_U: return PH_PCG_U;
+        case K_REF_ACC: return PH_REF_ACC;
+        case K_REF_RES: return PH_REF_RES;
+        case K_CORR:
+        case K_CORR_OUTER: return PH_CORR;
+        default: return -1;
+    }
+}
+
+hipError_t launchKernel(Kern k, bool f32, int grid, int nslots, hipStream_t s, const KArgs& a) {
     if (f32) launchT<float>(k, grid, s, a);
     else launchT<double>(k, grid, s, a);
+    const int ph = phaseOf(k);
+    if (ph >= 0)
+        hipLaunchKernelGGL(k_finalize, dim3(nslots), dim3(kBlock), 0, s, a, ph, k == K_CORR_OUTER ? 1 : 0);
     return hipGetLastError();
 }
 
