@@ -1,0 +1,298 @@
+#!/usr/bin/env python3
+"""Converged solves/s of the SIMPLE backend on one MI355X.
+
+Workload: the 1024 x 512 channel-network case (seeded networks of 6-10
+Manhattan/diagonal channels 8-16 cells wide, 2.5D model with Ht = 200 um,
+SOU convection, pseudo-CFL 0.5 -> 5 with SER, inlet 0.2 m/s, water), seeds
+0..B-1, synthetic geometry generated in-process.
+
+One step = solving the B geometries from rest to convergence in one packed
+batch (mixed precision: FP32 stencil / Jacobi / PCG with FP64 residuals,
+refinement and reductions; one persistent launch per sub-step; the outer
+iteration replayed as a hipGraph). A solve counts as converged when the
+outer stop test max(residU, residV, residMass) <= eps holds (mass residual
+over the p' columns) AND its full-system total-pressure drop is within 1e-4
+of the FP64 oracle's converged value (tests/golden/network_1024x512_dp.json).
+
+Reported beside the value:
+  * the plain FP64 single-geometry path (the drop-in: reference algorithm
+    and inner budgets, polled) on seed 0 with the same stop test;
+  * p50/p99 solve latency inside the batch;
+  * roofline of the dominant kernel (k_pcg_q, the PCG stencil kernel),
+    timed with HIP events recorded on the batch stream inside the replayed
+    graph during the timed region;
+  * the CPU baseline: the oracle in float arithmetic (the reference's
+    arithmetic) on the box's host cores for a bounded number of outer
+    iterations, scaled to solves/s by the outer-iteration count the same
+    algorithm needs on the GPU.
+
+    python bench.py [--steps K] [--warmup W] [--batch B]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "coldplate-solver_amd"))
+
+import cpbind as cb  # noqa: E402
+import workloads as wl  # noqa: E402
+
+METRIC = "converged solves/s + p50/p99 latency at Δp err≤1e-4; PCG HBM GB/s % peak"
+PEAK_HBM_GBS = 8000.0
+DP_TOL = 1e-4
+PCG_Q_BYTES_FP32 = 28  # per p' unknown: read r, aP, p_old, cE, cN; write p_new, q (4 B each)
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--eps", type=float, default=1e-7)
+    ap.add_argument("--jacobi", type=int, default=4, help="Jacobi sweep cap per momentum solve")
+    ap.add_argument("--pcg", type=int, default=20, help="PCG iterations per refinement cycle")
+    ap.add_argument("--cycles", type=int, default=2, help="FP64 refinement cycles of p'")
+    ap.add_argument("--check", type=int, default=20, help="outer iterations between host polls")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-dropin", action="store_true")
+    ap.add_argument("--no-probe", action="store_true",
+                    help="no event nodes in the replayed graph (for runs under rocprofv3)")
+    return ap.parse_args()
+
+
+def references(B: int) -> list[float]:
+    d = json.loads((REPO / "tests" / "golden" / "network_1024x512_dp.json").read_text())
+    out = []
+    for s in range(B):
+        e = d["seeds"].get(str(s))
+        if e is None:
+            raise SystemExit(f"no reference pressure drop for seed {s}; run tests/golden/make_network_refs.py")
+        out.append(e.get("dp_oracle", e["dp_fp64_gpu"]))
+    return out
+
+
+def n_pressure(M: int, N: int) -> int:
+    return (M - 1) * (N - 2)
+
+
+def solve_batch(b: cb.Batch, geoms, a, probe: bool):
+    """Admit every geometry, run to convergence, return per-slot results."""
+    slots = []
+    for p, ct in geoms:
+        q = wl.fast_mode(p, a.jacobi, a.pcg, a.cycles)
+        q.epsilon = a.eps
+        q.pTol = 1e-12
+        s = b.admit(q, ct)
+        assert s is not None
+        slots.append(s)
+    npres = {s: n_pressure(geoms[k][0].M, geoms[k][0].N) for k, s in enumerate(slots)}
+    done_at: dict[int, float] = {}
+    iters: dict[int, int] = {}
+    samples = []  # (ms per launch list, active p' unknowns)
+    t0 = time.perf_counter()
+    steps = 0
+    running = set(slots)
+    while running:
+        b.run(a.check)
+        steps += a.check
+        info = b.poll()
+        t = time.perf_counter() - t0
+        if probe:
+            ms = b.probe_read()
+            # slots that ran the last replay: still running, or converged at this step
+            act = [s for s in slots if info[s]["status"] == cb.SLOT_RUNNING or
+                   (s in running and info[s]["iter"] == steps)]
+            full = all(info[s]["itP"] == a.pcg * a.cycles for s in act)
+            if act and full and len(ms):
+                samples.append((ms, sum(npres[s] for s in act)))
+        for s in list(running):
+            if info[s]["status"] != cb.SLOT_RUNNING:
+                running.discard(s)
+                done_at[s] = t
+                iters[s] = info[s]["iter"]
+        if steps > 200000:
+            break
+    elapsed = time.perf_counter() - t0
+    dps, status = [], []
+    for s in slots:
+        dps.append(b.full_dp(s))
+        status.append(b.poll()[s]["status"])
+        b.evict(s)
+    return dict(elapsed=elapsed, latency=[done_at.get(s, elapsed) for s in slots],
+                iters=[iters.get(s, -1) for s in slots], dp=dps, status=status, samples=samples, steps=steps)
+
+
+def dropin_baseline(a, ref0: float):
+    """Plain FP64 single-geometry path (reference algorithm and inner budgets,
+    polled) on seed 0. It steps until its stop test holds at eps and, when
+    its pressure drop is not yet within DP_TOL there, keeps stepping (checking
+    every 100 outer iterations) until it is: the time to the first converged
+    solve at the required accuracy, a lower bound for this path."""
+    p, ct = wl.network_1024x512(0)
+    p.massResidualScope = 1
+    p.epsilon = 1e-13  # the loop below decides when to stop
+    g = cb.Solver(p, ct)
+    t0 = time.perf_counter()
+    it, err, reached = 0, 1.0, None
+    while it < 200000:
+        st = g.step()
+        it = st["iter"]
+        if reached is None and st["error"] <= a.eps:
+            reached = it
+        if reached is not None and (it == reached or it % 100 == 0):
+            err = abs(g.full_dp() / ref0 - 1)
+            if err <= DP_TOL:
+                break
+        if st["converged"]:
+            break
+    dt = time.perf_counter() - t0
+    g.close()
+    return dict(seconds=dt, iters=it, iters_at_eps=reached, dp_err=err)
+
+
+def cpu_baseline(a, outer_iters: int):
+    """Oracle in float arithmetic (reference arithmetic), seed 0, bounded."""
+    sys.path.insert(0, str(REPO / "oracle"))
+    import oracle_py as op  # CPU baseline leg only
+
+    cores = os.cpu_count() or 1
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    p, ct = wl.network_1024x512(0)
+    p.massResidualScope = 1
+    p.epsilon = a.eps
+    o = op.Oracle(p, ct, fp32=True)
+    n_done, t_total = 0, 0.0
+    while t_total < a.cpu_seconds and n_done < outer_iters:
+        n, st, w = o.run(10)
+        n_done += n
+        t_total += w
+        if n < 10:
+            break
+    per_iter = t_total / max(n_done, 1)
+    return dict(value=1.0 / (per_iter * outer_iters), unit="converged solves/s", cores=int(os.environ["OMP_NUM_THREADS"]),
+                kind="port", sample=f"float oracle, seed 0, first {n_done} outer iterations in {t_total:.1f} s "
+                f"({per_iter * 1e3:.1f} ms/iteration), scaled by the {outer_iters} outer iterations the "
+                f"reference algorithm needs on this geometry")
+
+
+def main() -> int:
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: every box here has one MI355X; --gpus {a.gpus} is not supported. "
+              f"Independent geometries would shard across GPUs with no communication.", file=sys.stderr)
+        return 2
+    lib = cb.load()
+    if lib.cp_device_count() < 1:
+        raise SystemExit("no HIP device")
+    B = a.batch
+    refs = references(B)
+    geoms = [wl.network_1024x512(s) for s in range(B)]
+    b = cb.Batch(cb.CP_PREC_MIXED, 24 << 30, B)
+    if not a.no_probe:
+        b.probe(cb.K_PCG_Q)
+    for w in range(a.warmup):
+        r = solve_batch(b, geoms, a, probe=False)
+        log(f"warmup {w}: {r['elapsed']:.2f}s iters {r['iters']}")
+    runs = []
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        r = solve_batch(b, geoms, a, probe=not a.no_probe)
+        runs.append(r)
+        log(f"step {k}: {r['elapsed']:.2f}s iters {r['iters']} max dp err "
+            f"{max(abs(d / x - 1) for d, x in zip(r['dp'], refs)):.2e}")
+    total = time.perf_counter() - t0
+    good = 0
+    errs, lat = [], []
+    for r in runs:
+        for d, x, s, l in zip(r["dp"], refs, r["status"], r["latency"]):
+            e = abs(d / x - 1)
+            errs.append(e)
+            lat.append(l)
+            if s == cb.SLOT_CONVERGED and e <= DP_TOL:
+                good += 1
+    value = good / total
+    # roofline of k_pcg_q from the in-graph events
+    ms_all, bytes_all = 0.0, 0.0
+    n_launch = 0
+    for r in runs:
+        for ms, npr in r["samples"]:
+            ms_all += float(np.sum(ms))
+            bytes_all += float(len(ms)) * npr * PCG_Q_BYTES_FP32
+            n_launch += len(ms)
+    avg_ms = ms_all / max(n_launch, 1)
+    achieved = bytes_all / max(ms_all, 1e-30) / 1e6  # GB/s
+    per_launch = bytes_all / max(n_launch, 1)
+    drop = None if a.no_dropin else dropin_baseline(a, refs[0])
+    if drop:
+        log(f"drop-in FP64 seed 0: {drop}")
+    cpu = None
+    if not a.no_cpu and drop:
+        cpu = cpu_baseline(a, drop["iters"])
+        log(f"cpu baseline: {cpu}")
+    ms_step = total / max(a.steps, 1) * 1e3
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "converged solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "network_1024x512",
+            "geometries_per_step": B,
+            "grid": "1024x512 cells",
+            "stop_test": f"max(residU,residV,residMass[p' cols]) <= {a.eps:g} and dp err <= {DP_TOL:g}",
+            "inner_budgets": {"jacobi_cap": a.jacobi, "pcg_per_cycle": a.pcg, "fp64_refine_cycles": a.cycles},
+            "precision": "mixed: fp32 stencil/jacobi/pcg, fp64 residuals/refinement/reductions",
+        },
+        "latency_s": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
+        "dp_rel_err": {"max": float(max(errs)), "median": float(np.median(errs))},
+        "outer_iterations": [x for r in runs for x in r["iters"]],
+        "plain_fp64_single": None if not drop else {
+            "solves_per_s": round(1.0 / drop["seconds"], 4),
+            "seconds_per_solve": round(drop["seconds"], 3),
+            "outer_iterations": drop["iters"],
+            "outer_iterations_at_eps": drop["iters_at_eps"],
+            "dp_rel_err": drop["dp_err"],
+            "speedup_of_value": round(value * drop["seconds"], 2),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_pcg_q<float>",
+            "achieved": round(achieved, 1),
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "traffic": None,
+            "avg_launch_us": round(avg_ms * 1e3, 2),
+            "bytes_per_launch": int(per_launch),
+            "launches_timed": n_launch,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

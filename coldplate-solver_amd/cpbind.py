@@ -23,6 +23,9 @@ F_U, F_V, F_P, F_USTAR, F_VSTAR, F_PSTAR, F_DE, F_DN, F_GAMMA, F_ALPHA = range(1
 
 SLOT_FREE, SLOT_RUNNING, SLOT_CONVERGED, SLOT_MAXITER, SLOT_FAILED = range(5)
 
+K_MOM_ASSEMBLE, K_JACOBI, K_MOM_FINISH, K_PRES_ASSEMBLE, K_PCG_Q, K_PCG_UPDATE = range(6)
+K_CORRECT = 8
+
 
 class Params(C.Structure):
     """cp_params (include/coldplate_hip.h)."""
@@ -98,6 +101,7 @@ class SlotInfo(C.Structure):
         ("status", C.c_int32), ("iter", C.c_int32),
         ("error", C.c_double), ("residU", C.c_double), ("residV", C.c_double),
         ("residMass", C.c_double), ("health", C.c_int32), ("M", C.c_int32), ("N", C.c_int32),
+        ("itU", C.c_int32), ("itV", C.c_int32), ("itP", C.c_int32), ("pseudoCFL", C.c_double),
     ]
 
 
@@ -150,6 +154,8 @@ EXPORTS = [
     ("cp_batch_get_fields", C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
     ("cp_batch_plane", C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.POINTER(PlaneMetrics)]),
     ("cp_batch_evict", C.c_int, [_vp, C.c_int]),
+    ("cp_batch_probe", C.c_int, [_vp, C.c_int]),
+    ("cp_batch_probe_read", C.c_int, [_vp, _dp, C.c_int]),
     ("cp_geom_straight", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
     ("cp_geom_network", C.c_int, [C.c_int, C.c_int, C.c_uint64, _dp]),
     ("cp_geom_gamma", C.c_int, [C.c_int, C.c_int, C.c_uint64, C.c_double, C.c_double, C.c_double, _dp]),
@@ -321,6 +327,16 @@ class Batch:
     def full_dp(self, slot: int, gamma_min: float = 1.0 - 1e-6) -> float:
         M, N, hx = self.shapes[slot]
         return self.plane(slot, 0.0, gamma_min)["avgTotal"] - self.plane(slot, N * hx, gamma_min)["avgTotal"]
+
+    def probe(self, kernel: int) -> None:
+        check(load().cp_batch_probe(self.h, kernel), self.h)
+
+    def probe_read(self, cap: int = 4096) -> np.ndarray:
+        out = np.zeros(cap)
+        n = load().cp_batch_probe_read(self.h, dptr(out), cap)
+        if n < 0:
+            check(n, self.h)
+        return out[:n].copy()
 
     def evict(self, slot: int) -> None:
         check(load().cp_batch_evict(self.h, slot), self.h)

@@ -112,8 +112,10 @@ enum Kern {
 
 // Host-side launchers (defined next to the kernels). launchKernel enqueues
 // the work kernel on `grid` workgroups and, when it reduces, the per-slot
-// finalisation on `nslots` workgroups.
-hipError_t launchKernel(Kern k, bool f32, int grid, int nslots, hipStream_t s, const KArgs& a);
+// finalisation on `nslots` workgroups. Non-null events are recorded right
+// before and right after the work kernel (not around the finalisation).
+hipError_t launchKernel(Kern k, bool f32, int grid, int nslots, hipStream_t s, const KArgs& a,
+                        hipEvent_t evBefore = nullptr, hipEvent_t evAfter = nullptr);
 hipError_t launchPlane(int nreq, hipStream_t s, const SlotDesc* desc, const PlaneReq* req, PlaneOut* out);
 
 }  // namespace cpdev

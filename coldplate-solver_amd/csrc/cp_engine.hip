@@ -160,6 +160,10 @@ struct cp_batch {
     // graph mode
     hipGraphExec_t graphExec = nullptr;
     int graphJ = -1, graphP = -1, graphR = -1;
+    // measurement probe: events around every launch of one work kernel
+    int probeKernel = -1;
+    std::vector<hipEvent_t> probeEv;
+    int probeUsed = 0;
 
     size_t tElem() const { return precision == CP_PREC_MIXED ? sizeof(float) : sizeof(double); }
     size_t bytesPerNode() const {
@@ -197,7 +201,13 @@ KArgs kargs(cp_batch* b) {
 }
 
 int launch(cp_batch* b, Kern k, int grid) {
-    const hipError_t e = launchKernel(k, b->precision == CP_PREC_MIXED, grid, b->maxSlots, b->stream, kargs(b));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (b->probeKernel == int(k) && b->probeUsed + 2 <= int(b->probeEv.size())) {
+        e0 = b->probeEv[b->probeUsed];
+        e1 = b->probeEv[b->probeUsed + 1];
+        b->probeUsed += 2;
+    }
+    const hipError_t e = launchKernel(k, b->precision == CP_PREC_MIXED, grid, b->maxSlots, b->stream, kargs(b), e0, e1);
     if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
     return CP_OK;
 }
@@ -438,6 +448,7 @@ void batchDestroy(cp_batch* b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->graphExec) (void)hipGraphExecDestroy(b->graphExec);
+    for (hipEvent_t ev : b->probeEv) (void)hipEventDestroy(ev);
     (void)hipFree(b->arenaBase);
     (void)hipFree(b->dDesc);
     (void)hipFree(b->dState);
@@ -633,6 +644,15 @@ int graphRun(cp_batch* b, int steps) {
         }
         const int grid = b->numCU * 8;  // fixed: independent of the admitted slots
         hipGraph_t g;
+        if (b->probeKernel >= 0) {
+            const size_t need = size_t(2) * (J + 2 * P * R + 8);
+            while (b->probeEv.size() < need) {
+                hipEvent_t ev;
+                HIP_TRY(hipEventCreate(&ev));
+                b->probeEv.push_back(ev);
+            }
+        }
+        b->probeUsed = 0;
         HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         const int rc = enqueueFixedStep(b, J, P, R, grid);
         hipGraph_t captured = nullptr;
@@ -806,6 +826,10 @@ int cp_batch_poll(cp_batch* b, cp_slot_info* info) {
         o.health = st.health ? CP_E_DIVERGED : 0;
         o.M = b->slots[s].M;
         o.N = b->slots[s].N;
+        o.itU = st.itU;
+        o.itV = st.itV;
+        o.itP = st.itP;
+        o.pseudoCFL = st.pseudoCFL;
     }
     return CP_OK;
 }
@@ -829,6 +853,27 @@ int cp_batch_plane(cp_batch* b, int slot, double x, double gamma_min, cp_plane_m
 int cp_batch_evict(cp_batch* b, int slot) {
     if (!b) return setError(nullptr, CP_E_ARG, "null batch");
     return batchEvict(b, slot);
+}
+
+int cp_batch_probe(cp_batch* b, int kernel) {
+    if (!b) return setError(nullptr, CP_E_ARG, "null batch");
+    if (kernel < -1 || kernel > int(K_CORR_OUTER)) return setError(&b->err, CP_E_ARG, "unknown kernel %d", kernel);
+    b->probeKernel = kernel == int(K_CORR) ? int(K_CORR_OUTER) : kernel;
+    b->graphJ = -1;  // recapture with (or without) the event nodes
+    return CP_OK;
+}
+
+int cp_batch_probe_read(cp_batch* b, double* ms, int cap) {
+    if (!b || !ms) return setError(nullptr, CP_E_ARG, "null argument");
+    std::string& err = b->err;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    int n = 0;
+    for (int k = 0; k + 1 < b->probeUsed && n < cap; k += 2) {
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, b->probeEv[k], b->probeEv[k + 1]));
+        ms[n++] = double(t);
+    }
+    return n;
 }
 
 // ---- single-geometry context ------------------------------------------------

@@ -892,9 +892,14 @@ static int phaseOf(Kern k) {
     }
 }
 
-hipError_t launchKernel(Kern k, bool f32, int grid, int nslots, hipStream_t s, const KArgs& a) {
+hipError_t launchKernel(Kern k, bool f32, int grid, int nslots, hipStream_t s, const KArgs& a, hipEvent_t evBefore,
+                        hipEvent_t evAfter) {
+    // external records: inside stream capture they become event-record nodes
+    // that timestamp every replay (a plain record would only add an edge)
+    if (evBefore) (void)hipEventRecordWithFlags(evBefore, s, hipEventRecordExternal);
     if (f32) launchT<float>(k, grid, s, a);
     else launchT<double>(k, grid, s, a);
+    if (evAfter) (void)hipEventRecordWithFlags(evAfter, s, hipEventRecordExternal);
     const int ph = phaseOf(k);
     if (ph >= 0)
         hipLaunchKernelGGL(k_finalize, dim3(nslots), dim3(kBlock), 0, s, a, ph, k == K_CORR_OUTER ? 1 : 0);

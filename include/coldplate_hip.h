@@ -219,6 +219,8 @@ typedef struct cp_slot_info {
     double  error, residU, residV, residMass;
     int32_t health;
     int32_t M, N;
+    int32_t itU, itV, itP;  /* inner iterations of the last outer iteration */
+    double  pseudoCFL;
 } cp_slot_info;
 
 /*
@@ -242,6 +244,24 @@ int cp_batch_get_fields(cp_batch* b, int slot, double* u, double* v, double* p);
 int cp_batch_plane(cp_batch* b, int slot, double x, double gamma_min, cp_plane_metrics* out);
 /* Release a slot's arena space. */
 int cp_batch_evict(cp_batch* b, int slot);
+
+/* Work kernels of one outer iteration, for cp_batch_probe. */
+#define CP_K_MOM_ASSEMBLE 0
+#define CP_K_JACOBI       1
+#define CP_K_MOM_FINISH   2
+#define CP_K_PRES_ASSEMBLE 3
+#define CP_K_PCG_Q        4
+#define CP_K_PCG_UPDATE   5
+#define CP_K_CORRECT      8
+
+/* Measurement hook (no counterpart in the reference, which has no device
+   timing): record HIP events on the batch stream around every launch of
+   work kernel `kernel` (CP_K_*) inside the captured outer-iteration graph;
+   -1 disables. cp_batch_probe_read returns the durations (ms) of those
+   launches in the most recently completed replay, in launch order, and
+   their count. Call it after cp_batch_poll (which synchronises). */
+int cp_batch_probe(cp_batch* b, int kernel);
+int cp_batch_probe_read(cp_batch* b, double* ms, int cap);
 
 /* ------------------------------------------------------------------ */
 /* seeded geometry generators (cellType, M x N row-major)              */

@@ -1,0 +1,101 @@
+"""Writes tests/golden/network_1024x512_dp.json: converged full-system
+pressure drops of the 1024 x 512 channel-network workload (seeds 0..7).
+
+Two sources, recorded side by side:
+  * "dp_oracle": the CPU FP64 oracle (oracle/simple_oracle.hpp) run to a
+    fixed point (mass residual over the p' columns, epsilon 1e-9, PCG cap
+    300 at tolerance 1e-10);
+  * "dp_fp64_gpu": the device library in FP64 with the same settings (its
+    per-step parity with the oracle is tested in tests/test_gpu_parity.py).
+The bench uses dp_oracle where present, else dp_fp64_gpu; the test suite
+checks that the two agree wherever both exist.
+
+    python tests/golden/make_network_refs.py --gpu          # needs an MI355X
+    python tests/golden/make_network_refs.py --oracle 0 1   # CPU, ~10 min/seed
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+REPO = HERE.parent.parent
+sys.path.insert(0, str(REPO / "coldplate-solver_amd"))
+sys.path.insert(0, str(REPO / "oracle"))
+
+OUT = HERE / "network_1024x512_dp.json"
+SEEDS = list(range(8))
+GAMMA_MIN = 1.0 - 1e-6  # reference row rule (binary geometry)
+
+
+def strong(p):
+    p.massResidualScope = 1
+    p.epsilon = 1e-9
+    p.pTol = 1e-10
+    p.maxPressureIter = 300
+    return p
+
+
+def load() -> dict:
+    if OUT.exists():
+        return json.loads(OUT.read_text())
+    return {"workload": "network_1024x512", "gamma_min": GAMMA_MIN,
+            "settings": "massResidualScope=1, epsilon=1e-9, pTol=1e-10, maxPressureIter=300", "seeds": {}}
+
+
+def run_gpu(d: dict) -> None:
+    import cpbind as cb
+    import workloads as wl
+
+    b = cb.Batch(cb.CP_PREC_FP64, 16 << 30, len(SEEDS))
+    slots = {}
+    for s in SEEDS:
+        p, ct = wl.network_1024x512(s)
+        slots[s] = b.admit(strong(p), ct)
+    while True:
+        b.run(25)
+        info = b.poll()
+        if all(info[k]["status"] != cb.SLOT_RUNNING for k in slots.values()):
+            break
+    for s, k in slots.items():
+        e = d["seeds"].setdefault(str(s), {})
+        e["dp_fp64_gpu"] = b.full_dp(k, GAMMA_MIN)
+        e["iters_fp64_gpu"] = info[k]["iter"]
+        print(s, e, flush=True)
+
+
+def run_oracle(d: dict, seeds: list[int]) -> None:
+    import oracle_py as op
+    import workloads as wl
+
+    for s in seeds:
+        p, ct = wl.network_1024x512(s)
+        o = op.Oracle(strong(p), ct)
+        t = time.time()
+        n, st, _ = o.run(10000)
+        e = d["seeds"].setdefault(str(s), {})
+        e["dp_oracle"] = o.full_dp(GAMMA_MIN)
+        e["iters_oracle"] = st["iter"]
+        print(s, e, f"{time.time() - t:.0f}s", flush=True)
+        OUT.write_text(json.dumps(d, indent=1))
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpu", action="store_true")
+    ap.add_argument("--oracle", type=int, nargs="*")
+    a = ap.parse_args()
+    d = load()
+    if a.gpu:
+        run_gpu(d)
+    if a.oracle:
+        run_oracle(d, a.oracle)
+    OUT.write_text(json.dumps(d, indent=1))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
