@@ -154,6 +154,7 @@ EXPORTS = [
     ("cp_batch_get_fields", C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
     ("cp_batch_plane", C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.POINTER(PlaneMetrics)]),
     ("cp_batch_evict", C.c_int, [_vp, C.c_int]),
+    ("cp_batch_set_tiling", C.c_int, [_vp, C.c_int, C.c_int]),
     ("cp_batch_probe", C.c_int, [_vp, C.c_int]),
     ("cp_batch_probe_read", C.c_int, [_vp, _dp, C.c_int]),
     ("cp_geom_straight", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
@@ -327,6 +328,9 @@ class Batch:
     def full_dp(self, slot: int, gamma_min: float = 1.0 - 1e-6) -> float:
         M, N, hx = self.shapes[slot]
         return self.plane(slot, 0.0, gamma_min)["avgTotal"] - self.plane(slot, N * hx, gamma_min)["avgTotal"]
+
+    def set_tiling(self, tile_rows: int, wg_per_cu: int = 0) -> None:
+        check(load().cp_batch_set_tiling(self.h, tile_rows, wg_per_cu), self.h)
 
     def probe(self, kernel: int) -> None:
         check(load().cp_batch_probe(self.h, kernel), self.h)

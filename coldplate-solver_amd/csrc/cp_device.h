@@ -8,9 +8,9 @@
 //   p(i,j), p'(i,j) [i<=M, j<=N]
 //   gam(i,j) = gamma(i-1, j-1)  (cell-centred fields moved onto the p index)
 // so the staggered stencils of the reference (SIMPLE.h:17-31) become fixed
-// offsets on one grid. Kernels cover the grid with 64 x 4 node tiles; one
+// offsets on one grid. Kernels cover the grid with 64 x 4R node tiles; one
 // workgroup of 256 threads (4 wave64) processes one tile per visit, each wave
-// one 64-node row segment.
+// a 64-column strip of R rows, marching down the rows.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -37,6 +37,7 @@ enum PresArr { P_CE = 0, P_CN, P_AP, P_R, P_PK0, P_PK1, P_Q, P_X, P_COUNT };
 struct alignas(64) SlotDesc {
     int M, N, pitch;
     int tile0, ntiles, tilesX;
+    int tileRows;                // rows per wave R: a tile is 64 x 4R nodes
     int nU, nV;                  // momentum DOF counts (Jacobi stop test)
     int convScheme, pseudo, localPseudo;
     int maxMomIter, maxPresIter, refineCycles, massScope;
@@ -112,9 +113,10 @@ enum Kern {
 
 // Host-side launchers (defined next to the kernels). launchKernel enqueues
 // the work kernel on `grid` workgroups and, when it reduces, the per-slot
-// finalisation on `nslots` workgroups. Non-null events are recorded right
+// finalisation on `nslots` workgroups; `rows` is the batch's R (1, 2, 4 or
+// 8). Non-null events are recorded right
 // before and right after the work kernel (not around the finalisation).
-hipError_t launchKernel(Kern k, bool f32, int grid, int nslots, hipStream_t s, const KArgs& a,
+hipError_t launchKernel(Kern k, bool f32, int rows, int grid, int nslots, hipStream_t s, const KArgs& a,
                         hipEvent_t evBefore = nullptr, hipEvent_t evAfter = nullptr);
 hipError_t launchPlane(int nreq, hipStream_t s, const SlotDesc* desc, const PlaneReq* req, PlaneOut* out);
 

@@ -164,6 +164,10 @@ struct cp_batch {
     int probeKernel = -1;
     std::vector<hipEvent_t> probeEv;
     int probeUsed = 0;
+    // tile height R (rows per wave) shared by every slot: 1, 2, 4 or 8
+    int tileRows = 4;
+    // workgroups per CU of the replayed graph's launches
+    int wgPerCU = 8;
 
     size_t tElem() const { return precision == CP_PREC_MIXED ? sizeof(float) : sizeof(double); }
     size_t bytesPerNode() const {
@@ -207,7 +211,8 @@ int launch(cp_batch* b, Kern k, int grid) {
         e1 = b->probeEv[b->probeUsed + 1];
         b->probeUsed += 2;
     }
-    const hipError_t e = launchKernel(k, b->precision == CP_PREC_MIXED, grid, b->maxSlots, b->stream, kargs(b), e0, e1);
+    const hipError_t e =
+        launchKernel(k, b->precision == CP_PREC_MIXED, b->tileRows, grid, b->maxSlots, b->stream, kargs(b), e0, e1);
     if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
     return CP_OK;
 }
@@ -300,15 +305,31 @@ int validate(const cp_params* p, std::string* err) {
     return CP_OK;
 }
 
-SlotDesc makeDesc(const cp_params& p, int precision) {
+// Rows per wave for a grid: taller tiles cut the stencil kernels' vertical
+// re-reads; small grids keep short tiles so the launch still has enough
+// workgroups. An explicit choice (the autotuner's) overrides this.
+int defaultTileRows(int M, int N) {
+    const long cells = long(M) * long(N);
+    if (cells >= (1L << 18)) return 4;
+    if (cells >= (1L << 15)) return 2;
+    return 1;
+}
+
+void setTiling(SlotDesc& d, int tileRows) {
+    d.tileRows = tileRows;
+    d.tilesX = (d.N + 1 + kTileW - 1) / kTileW;
+    const int rowsPerTile = kTileH * d.tileRows;
+    const int tilesY = (d.M + 1 + rowsPerTile - 1) / rowsPerTile;
+    d.ntiles = d.tilesX * tilesY;
+}
+
+SlotDesc makeDesc(const cp_params& p, int precision, int tileRows) {
     SlotDesc d;
     memset(&d, 0, sizeof(d));
     d.M = p.M;
     d.N = p.N;
     d.pitch = int(roundUp(size_t(p.N) + 1, 32));
-    d.tilesX = (p.N + 1 + kTileW - 1) / kTileW;
-    const int tilesY = (p.M + 1 + kTileH - 1) / kTileH;
-    d.ntiles = d.tilesX * tilesY;
+    setTiling(d, tileRows);
     d.nU = (p.M - 1) * (p.N - 2);
     d.nV = (p.M - 2) * (p.N - 1);
     d.convScheme = p.convectionScheme;
@@ -476,7 +497,7 @@ int batchAdmit(cp_batch* b, const cp_params* prm, const double* ct, int* slotOut
     for (int k = 0; k < b->maxSlots; ++k)
         if (!b->slots[k].used) { s = k; break; }
     if (s < 0) return setError(&err, CP_E_CAPACITY, "no free slot");
-    SlotDesc d = makeDesc(*prm, b->precision);
+    SlotDesc d = makeDesc(*prm, b->precision, b->tileRows);
     if (b->totalTiles + size_t(d.ntiles) > b->maxTiles) return setError(&err, CP_E_CAPACITY, "tile table full");
     const size_t bytes = b->footprint(prm->M, prm->N);
     size_t off = 0;
@@ -642,7 +663,7 @@ int graphRun(cp_batch* b, int steps) {
             HIP_TRY(hipGraphExecDestroy(b->graphExec));
             b->graphExec = nullptr;
         }
-        const int grid = b->numCU * 8;  // fixed: independent of the admitted slots
+        const int grid = b->numCU * b->wgPerCU;  // fixed: independent of the admitted slots
         hipGraph_t g;
         if (b->probeKernel >= 0) {
             const size_t need = size_t(2) * (J + 2 * P * R + 8);
@@ -855,6 +876,24 @@ int cp_batch_evict(cp_batch* b, int slot) {
     return batchEvict(b, slot);
 }
 
+int cp_batch_set_tiling(cp_batch* b, int tile_rows, int wg_per_cu) {
+    if (!b) return setError(nullptr, CP_E_ARG, "null batch");
+    if (!(tile_rows == 0 || tile_rows == 1 || tile_rows == 2 || tile_rows == 4 || tile_rows == 8) || wg_per_cu < 0 ||
+        wg_per_cu > 64)
+        return setError(&b->err, CP_E_ARG, "tiling out of range (rows %d, wg/CU %d)", tile_rows, wg_per_cu);
+    if (tile_rows > 0 && tile_rows != b->tileRows) {
+        // the tiling is only a decomposition of each slot's grid: retile all
+        b->tileRows = tile_rows;
+        for (int s = 0; s < b->maxSlots; ++s)
+            if (b->slots[s].used) setTiling(b->hDesc[s], tile_rows);
+        const int rc = rebuildTiles(b);
+        if (rc) return rc;
+    }
+    if (wg_per_cu > 0) b->wgPerCU = wg_per_cu;
+    b->graphJ = -1;
+    return CP_OK;
+}
+
 int cp_batch_probe(cp_batch* b, int kernel) {
     if (!b) return setError(nullptr, CP_E_ARG, "null batch");
     if (kernel < -1 || kernel > int(K_CORR_OUTER)) return setError(&b->err, CP_E_ARG, "unknown kernel %d", kernel);
@@ -891,6 +930,7 @@ int cp_create(const cp_params* prm, const double* cellType, cp_ctx** out) {
     const size_t budget = roundUp(size_t(prm->M + 1) * pitch * per + 256 * 64, 1 << 20);
     rc = batchCreate(prm->precision, budget, 1, &b, nullptr);
     if (rc) return rc;
+    b->tileRows = defaultTileRows(prm->M, prm->N);
     int slot = -1;
     rc = batchAdmit(b, prm, cellType, &slot);
     if (rc) {

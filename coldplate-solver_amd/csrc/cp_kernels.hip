@@ -66,20 +66,28 @@ __device__ __forceinline__ void blockReduceTo(double (&v)[NQ], double* dst) {
 }
 
 // Sums (or maxes) the partials of one slot's tiles in tile order; the result
-// is valid in every thread of the block.
+// is valid in every thread of the block. Each thread issues its loads in
+// groups of 8 before combining them, so the latency is paid once per group.
 template <int NQ, unsigned MaxMask>
 __device__ __forceinline__ void slotReduce(const double* partials, int tile0, int ntiles, double (&out)[NQ]) {
     __shared__ double res[4];
     double v[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) v[q] = 0.0;
-    for (int k = threadIdx.x; k < ntiles; k += kBlock) {
-        const double* src = partials + size_t(tile0 + k) * kNumPartials;
+    constexpr int G = 8;
+    for (int base = 0; base < ntiles; base += kBlock * G) {
+        double x[G][NQ];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const double x = src[q];
-            v[q] = ((MaxMask >> q) & 1u) ? nanmax(v[q], x) : v[q] + x;
+        for (int g = 0; g < G; ++g) {
+            const int k = base + g * kBlock + int(threadIdx.x);
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+                x[g][q] = (k < ntiles) ? partials[size_t(tile0 + k) * kNumPartials + q] : 0.0;
         }
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) v[q] = ((MaxMask >> q) & 1u) ? nanmax(v[q], x[g][q]) : v[q] + x[g][q];
     }
     __syncthreads();
     blockReduceTo<NQ, MaxMask>(v, res);
@@ -93,9 +101,17 @@ __device__ __forceinline__ void slotReduce(const double* partials, int tile0, in
 // tile decoding
 // ---------------------------------------------------------------------------
 struct Tile {
-    int s, i, j;
+    int s;   // slot
+    int i;   // first row of this wave's R rows
+    int j;   // column of this lane
+    int i0;  // first row of the tile
+    int j0;  // first column of the tile
 };
 
+// A tile is 64 columns x 4R rows (R = rows per wave, a compile-time launch
+// parameter shared by every slot of a batch): wave w of the workgroup owns
+// rows [i0 + w R, i0 + (w+1) R).
+template <int R>
 __device__ __forceinline__ Tile decodeTile(const KArgs& a, int t) {
     Tile tl;
     tl.s = a.tileSlot[t];
@@ -103,8 +119,10 @@ __device__ __forceinline__ Tile decodeTile(const KArgs& a, int t) {
     const int lt = t - d.tile0;
     const int ty = lt / d.tilesX;
     const int tx = lt - ty * d.tilesX;
-    tl.i = ty * kTileH + int(threadIdx.x >> 6);
-    tl.j = tx * kTileW + int(threadIdx.x & 63);
+    tl.i0 = ty * (kTileH * R);
+    tl.j0 = tx * kTileW;
+    tl.i = tl.i0 + int(threadIdx.x >> 6) * R;
+    tl.j = tl.j0 + int(threadIdx.x & 63);
     return tl;
 }
 
@@ -208,18 +226,21 @@ __device__ __forceinline__ void momRow(const SlotDesc& d, size_t k, int i, int j
     static_cast<T*>(A[M_D0])[k] = T(0);
 }
 
-template <typename T>
+template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_mom_assemble(KArgs a) {
     const int nT = *a.nTiles;
     for (int t = blockIdx.x; t < nT; t += gridDim.x) {
-        const Tile tl = decodeTile(a, t);
+        const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
         if (st.status != ST_RUNNING) continue;
-        const int M = d.M, N = d.N, i = tl.i, j = tl.j;
+        const int M = d.M, N = d.N, j = tl.j;
         const double maxVel = 3.0 * fmax(fabs(st.inletVelocity), 0.1);
         const double cfl = st.pseudoCFL;
-        if (i <= M && j <= N) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = tl.i + r;
+            if (i > M || j > N) continue;
             const size_t k = size_t(i) * d.pitch + j;
             if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2) momRow<T, true>(d, k, i, j, maxVel, cfl);
             if (i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1) momRow<T, false>(d, k, i, j, maxVel, cfl);
@@ -241,39 +262,74 @@ __global__ __launch_bounds__(kBlock) void k_mom_assemble(KArgs a) {
 // U and V are independent systems swept in the same launch, each with its
 // own stop test and sweep count.
 // ---------------------------------------------------------------------------
+// Value of array `f` at (i, j) of a (M+1) x pitch plane, 0 outside the
+// node range [0, M] x [0, jmax].
 template <typename T>
-__device__ __forceinline__ double sweepAt(void* const* A, size_t k, int P, int parity) {
-    const T* src = static_cast<const T*>(A[M_D0 + parity]);
-    T* dst = static_cast<T*>(A[M_D0 + (parity ^ 1)]);
-    const T aP = static_cast<const T*>(A[M_AP])[k];
-    const T dc = src[k];
-    const T Ax = aP * dc - static_cast<const T*>(A[M_AE])[k] * src[k + 1] -
-                 static_cast<const T*>(A[M_AW])[k] * src[k - 1] -
-                 static_cast<const T*>(A[M_AN])[k] * src[k + P] -
-                 static_cast<const T*>(A[M_AS])[k] * src[k - P];
-    const T r = static_cast<const T*>(A[M_R0])[k] - Ax;
-    dst[k] = dc + r / aP;
-    return double(r) * double(r);
+__device__ __forceinline__ T at(const T* f, int i, int j, int M, int jmax, int P) {
+    return (i >= 0 && i <= M && j >= 0 && j <= jmax) ? f[size_t(i) * P + j] : T(0);
 }
 
-template <typename T>
+// Jacobi sweep on LDS halo tiles: the workgroup stages delta of U and V for
+// its (4R+2) x 66 halo region (rows i0-1 .. i0+4R, columns j0-1 .. j0+64)
+// with coalesced loads, then each thread updates its R nodes from LDS; the
+// coefficient arrays are read once per node. delta is zero outside each
+// system's unknowns, so the product needs no masks.
+template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_jacobi(KArgs a) {
+    constexpr int TH = kTileH * R, HH = TH + 2, HW = kTileW + 2;
+    __shared__ T sU[HH][HW];
+    __shared__ T sV[HH][HW];
     const int nT = *a.nTiles;
     for (int t = blockIdx.x; t < nT; t += gridDim.x) {
-        const Tile tl = decodeTile(a, t);
+        const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
         if (st.status != ST_RUNNING) continue;
         const int doneU = st.jdone[0], doneV = st.jdone[1];
         if (doneU && doneV) continue;
-        const int M = d.M, N = d.N, i = tl.i, j = tl.j;
+        const int M = d.M, N = d.N, P = d.pitch;
+        const int pu = st.jk[0] & 1, pv = st.jk[1] & 1;
+        const T* srcU = static_cast<const T*>(d.mu[M_D0 + pu]);
+        const T* srcV = static_cast<const T*>(d.mv[M_D0 + pv]);
+        __syncthreads();  // the previous tile's LDS reads are done
+        for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
+            const int hi = idx / HW, hj = idx - hi * HW;
+            const int i = tl.i0 - 1 + hi, j = tl.j0 - 1 + hj;
+            sU[hi][hj] = doneU ? T(0) : at(srcU, i, j, M, N, P);
+            sV[hi][hj] = doneV ? T(0) : at(srcV, i, j, M, N, P);
+        }
+        __syncthreads();
+        const int lane = int(threadIdx.x & 63), w = int(threadIdx.x >> 6);
         double v[2] = {0.0, 0.0};
-        if (i <= M && j <= N) {
-            const size_t k = size_t(i) * d.pitch + j;
-            if (!doneU && i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2)
-                v[0] = sweepAt<T>(d.mu, k, d.pitch, st.jk[0] & 1);
-            if (!doneV && i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1)
-                v[1] = sweepAt<T>(d.mv, k, d.pitch, st.jk[1] & 1);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int row = w * R + r, i = tl.i0 + row, j = tl.j0 + lane;
+            const size_t k = size_t(i) * P + j;
+            // U unknowns: i in [1, M-1], j in [1, N-2]; V: i in [1, M-2], j in [1, N-1]
+            if (!doneU && i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2) {
+                void* const* A = d.mu;
+                const T ap = static_cast<const T*>(A[M_AP])[k];
+                const T dc = sU[row + 1][lane + 1];
+                const T Ax = ap * dc - static_cast<const T*>(A[M_AE])[k] * sU[row + 1][lane + 2] -
+                             static_cast<const T*>(A[M_AW])[k] * sU[row + 1][lane] -
+                             static_cast<const T*>(A[M_AN])[k] * sU[row + 2][lane + 1] -
+                             static_cast<const T*>(A[M_AS])[k] * sU[row][lane + 1];
+                const T res = static_cast<const T*>(A[M_R0])[k] - Ax;
+                static_cast<T*>(A[M_D0 + (pu ^ 1)])[k] = dc + res / ap;
+                v[0] += double(res) * double(res);
+            }
+            if (!doneV && i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1) {
+                void* const* A = d.mv;
+                const T ap = static_cast<const T*>(A[M_AP])[k];
+                const T dc = sV[row + 1][lane + 1];
+                const T Ax = ap * dc - static_cast<const T*>(A[M_AE])[k] * sV[row + 1][lane + 2] -
+                             static_cast<const T*>(A[M_AW])[k] * sV[row + 1][lane] -
+                             static_cast<const T*>(A[M_AN])[k] * sV[row + 2][lane + 1] -
+                             static_cast<const T*>(A[M_AS])[k] * sV[row][lane + 1];
+                const T res = static_cast<const T*>(A[M_R0])[k] - Ax;
+                static_cast<T*>(A[M_D0 + (pv ^ 1)])[k] = dc + res / ap;
+                v[1] += double(res) * double(res);
+            }
         }
         blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials);
     }
@@ -283,21 +339,24 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(KArgs a) {
 // scatter with clamp + max residual (momentum_solver.cpp:318-335) fused with
 // the velocity boundary conditions on (u*, v*) (boundaries.cpp:48-106).
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_mom_finish(KArgs a) {
     const int nT = *a.nTiles;
     for (int t = blockIdx.x; t < nT; t += gridDim.x) {
-        const Tile tl = decodeTile(a, t);
+        const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
         if (st.status != ST_RUNNING) continue;
-        const int M = d.M, N = d.N, P = d.pitch, i = tl.i, j = tl.j;
+        const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
         const double maxVel = 3.0 * fmax(fabs(st.inletVelocity), 0.1);
         const double Uin = st.inletVelocity;
         const T* dU = static_cast<const T*>(d.mu[M_D0 + (st.jiters[0] & 1)]);
         const T* dV = static_cast<const T*>(d.mv[M_D0 + (st.jiters[1] & 1)]);
         double v[3] = {0.0, 0.0, 0.0};
-        if (i <= M && j <= N) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = tl.i + r;
+            if (i > M || j > N) continue;
             const size_t k = size_t(i) * P + j;
             // u* : (M+1) x N
             if (j <= N - 1) {
@@ -310,7 +369,7 @@ __global__ __launch_bounds__(kBlock) void k_mom_finish(KArgs a) {
                     val = clampv(d.u[k - 1] + double(dU[k - 1]), maxVel);  // outlet: u(i,N-1) = u(i,N-2)
                 } else {
                     val = clampv(d.u[k] + double(dU[k]), maxVel);
-                    v[0] = fabs(val - d.u[k]);
+                    v[0] = nanmax(v[0], fabs(val - d.u[k]));
                 }
                 d.us[k] = val;
                 if (!(val == val)) v[2] = 1.0;
@@ -324,7 +383,7 @@ __global__ __launch_bounds__(kBlock) void k_mom_finish(KArgs a) {
                     val = clampv(d.v[k - 1] + double(dV[k - 1]), maxVel);  // outlet: v(i,N) = v(i,N-1)
                 } else {
                     val = clampv(d.v[k] + double(dV[k]), maxVel);
-                    v[1] = fabs(val - d.v[k]);
+                    v[1] = nanmax(v[1], fabs(val - d.v[k]));
                 }
                 d.vs[k] = val;
                 if (!(val == val)) v[2] = 1.0;
@@ -338,24 +397,27 @@ __global__ __launch_bounds__(kBlock) void k_mom_finish(KArgs a) {
 // pressure correction: mass residual, coefficients, PCG start
 // (pressure_solver.cpp:463-486, 648-653, 721-732)
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_pres_assemble(KArgs a) {
     const int nT = *a.nTiles;
     for (int t = blockIdx.x; t < nT; t += gridDim.x) {
-        const Tile tl = decodeTile(a, t);
+        const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
         if (st.status != ST_RUNNING) continue;
-        const int M = d.M, N = d.N, P = d.pitch, i = tl.i, j = tl.j;
+        const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
         const int jMass = d.massScope ? N - 2 : N - 1;
         double v[2] = {0.0, 0.0};
-        if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 1) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = tl.i + r;
+            if (!(i >= 1 && i <= M - 1 && j >= 1 && j <= N - 1)) continue;
             const size_t k = size_t(i) * P + j;
             const double rho = d.rho, hx = d.hx, hy = d.hy;
             const double mE = rho * d.us[k] * hy, mW = rho * d.us[k - 1] * hy;
             const double mN = rho * d.vs[k] * hx, mS = rho * d.vs[k - P] * hx;
             const double b = mW - mE + mS - mN;
-            if (j <= jMass) v[0] = fabs(b);
+            if (j <= jMass) v[0] = nanmax(v[0], fabs(b));
             if (j <= N - 2) {
                 const double aE = rho * d.dE[k] * hy;
                 const double aW = (j > 1) ? rho * d.dE[k - 1] * hy : 0.0;
@@ -366,13 +428,13 @@ __global__ __launch_bounds__(kBlock) void k_pres_assemble(KArgs a) {
                 static_cast<T*>(d.pr[P_CE])[k] = T(aE);
                 static_cast<T*>(d.pr[P_CN])[k] = T(aN);
                 static_cast<T*>(d.pr[P_AP])[k] = T(aP);
-                const T r = T(b);
-                static_cast<T*>(d.pr[P_R])[k] = r;
+                const T rr = T(b);
+                static_cast<T*>(d.pr[P_R])[k] = rr;
                 static_cast<T*>(d.pr[P_X])[k] = T(0);
                 d.bm[k] = b;
                 d.pc[k] = 0.0;
-                const T z = r / T(aP);
-                v[1] = double(r) * double(z);
+                const T z = rr / T(aP);
+                v[1] += double(rr) * double(z);
             }
         }
         blockReduceTo<2, 1u>(v, a.partials + size_t(t) * kNumPartials);
@@ -389,39 +451,71 @@ __device__ __forceinline__ bool isPDof(int i, int j, int M, int N) {
 // fly for the node and its four neighbours), q = A p_k, partial p.q
 // (pressure_solver.cpp:153-159, 173-181)
 // ---------------------------------------------------------------------------
+// p_k at (i, j): z + beta p_{k-1} on p' unknowns, 0 elsewhere.
 template <typename T>
+__device__ __forceinline__ T pAt(const T* r, const T* aP, const T* pold, int i, int j, int M, int N, int P, int kk,
+                                 T beta) {
+    if (!isPDof(i, j, M, N)) return T(0);
+    const size_t k = size_t(i) * P + j;
+    const T z = r[k] / aP[k];
+    return kk == 0 ? z : z + beta * pold[k];
+}
+
+// q = A p_k on LDS halo tiles: the workgroup forms p_k = z + beta p_{k-1}
+// (z = r / aP) once per node of its (4R+2) x 66 halo region, stages the east
+// couplings with their west neighbour and the north couplings with their
+// south neighbour, then each thread computes its R nodes from LDS.
+template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_pcg_q(KArgs a) {
+    constexpr int TH = kTileH * R, HH = TH + 2, HW = kTileW + 2;
+    __shared__ T sp[HH][HW];
+    __shared__ T scE[TH][kTileW + 1];
+    __shared__ T scN[TH + 1][kTileW];
     const int nT = *a.nTiles;
     for (int t = blockIdx.x; t < nT; t += gridDim.x) {
-        const Tile tl = decodeTile(a, t);
+        const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
         if (st.status != ST_RUNNING || st.pdone || st.stepPhase) continue;
-        const int M = d.M, N = d.N, P = d.pitch, i = tl.i, j = tl.j;
+        const int M = d.M, N = d.N, P = d.pitch;
         const int kk = st.pk;
         const T beta = T(st.beta);
+        const T* r = static_cast<const T*>(d.pr[P_R]);
+        const T* aP = static_cast<const T*>(d.pr[P_AP]);
+        const T* cE = static_cast<const T*>(d.pr[P_CE]);
+        const T* cN = static_cast<const T*>(d.pr[P_CN]);
+        const T* pold = static_cast<const T*>(d.pr[P_PK0 + (kk & 1)]);
+        T* pnew = static_cast<T*>(d.pr[P_PK0 + ((kk & 1) ^ 1)]);
+        T* Q = static_cast<T*>(d.pr[P_Q]);
+        __syncthreads();  // the previous tile's LDS reads are done
+        for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
+            const int hi = idx / HW, hj = idx - hi * HW;
+            sp[hi][hj] = pAt(r, aP, pold, tl.i0 - 1 + hi, tl.j0 - 1 + hj, M, N, P, kk, beta);
+        }
+        for (int idx = int(threadIdx.x); idx < TH * (kTileW + 1); idx += kBlock) {
+            const int ci = idx / (kTileW + 1), cj = idx - ci * (kTileW + 1);
+            scE[ci][cj] = at(cE, tl.i0 + ci, tl.j0 - 1 + cj, M, N, P);
+        }
+        for (int idx = int(threadIdx.x); idx < (TH + 1) * kTileW; idx += kBlock) {
+            const int ci = idx / kTileW, cj = idx - ci * kTileW;
+            scN[ci][cj] = at(cN, tl.i0 - 1 + ci, tl.j0 + cj, M, N, P);
+        }
+        __syncthreads();
+        const int lane = int(threadIdx.x & 63), w = int(threadIdx.x >> 6);
         double v[1] = {0.0};
-        if (isPDof(i, j, M, N)) {
-            const size_t k = size_t(i) * P + j;
-            const T* r = static_cast<const T*>(d.pr[P_R]);
-            const T* aP = static_cast<const T*>(d.pr[P_AP]);
-            const T* cE = static_cast<const T*>(d.pr[P_CE]);
-            const T* cN = static_cast<const T*>(d.pr[P_CN]);
-            const T* pold = static_cast<const T*>(d.pr[P_PK0 + (kk & 1)]);
-            T* pnew = static_cast<T*>(d.pr[P_PK0 + ((kk & 1) ^ 1)]);
-            auto pAt = [&](int ii, int jj, size_t kn) -> T {
-                if (!isPDof(ii, jj, M, N)) return T(0);
-                const T z = r[kn] / aP[kn];
-                return kk == 0 ? z : z + beta * pold[kn];
-            };
-            const T pC = pAt(i, j, k);
-            const T pS = pAt(i - 1, j, k - P), pW = pAt(i, j - 1, k - 1);
-            const T pE = pAt(i, j + 1, k + 1), pN = pAt(i + 1, j, k + P);
-            // CSR column order S, W, P, E, N (pressure_solver.cpp:108-118)
-            const T q = -cN[k - P] * pS - cE[k - 1] * pW + aP[k] * pC - cE[k] * pE - cN[k] * pN;
-            pnew[k] = pC;
-            static_cast<T*>(d.pr[P_Q])[k] = q;
-            v[0] = double(pC) * double(q);
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+            const int row = w * R + rr, i = tl.i0 + row, j = tl.j0 + lane;
+            if (isPDof(i, j, M, N)) {
+                const size_t k = size_t(i) * P + j;
+                const T pC = sp[row + 1][lane + 1];
+                // CSR column order S, W, P, E, N (pressure_solver.cpp:108-118)
+                const T q = -scN[row][lane] * sp[row][lane + 1] - scE[row][lane] * sp[row + 1][lane] + aP[k] * pC -
+                            scE[row][lane + 1] * sp[row + 1][lane + 2] - scN[row + 1][lane] * sp[row + 2][lane + 1];
+                pnew[k] = pC;
+                Q[k] = q;
+                v[0] += double(pC) * double(q);
+            }
         }
         blockReduceTo<1, 0u>(v, a.partials + size_t(t) * kNumPartials);
     }
@@ -431,30 +525,33 @@ __global__ __launch_bounds__(kBlock) void k_pcg_q(KArgs a) {
 // PCG iteration, part 2: x += alpha p, r -= alpha q, partial r.r and r.z;
 // stop on ||r||_2 < pTol or the iteration cap (pressure_solver.cpp:159-182)
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
     const int nT = *a.nTiles;
     for (int t = blockIdx.x; t < nT; t += gridDim.x) {
-        const Tile tl = decodeTile(a, t);
+        const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
         if (st.status != ST_RUNNING || st.pdone || st.stepPhase) continue;
-        const int M = d.M, N = d.N, P = d.pitch, i = tl.i, j = tl.j;
+        const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
         const int kk = st.pk;
         const T alpha = T(st.alpha);
+        const T* pk = static_cast<const T*>(d.pr[P_PK0 + ((kk & 1) ^ 1)]);
+        T* x = static_cast<T*>(d.pr[P_X]);
+        T* r = static_cast<T*>(d.pr[P_R]);
+        const T* Q = static_cast<const T*>(d.pr[P_Q]);
+        const T* aP = static_cast<const T*>(d.pr[P_AP]);
         double v[2] = {0.0, 0.0};
-        if (isPDof(i, j, M, N)) {
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+            const int i = tl.i + rr;
+            if (!isPDof(i, j, M, N)) continue;
             const size_t k = size_t(i) * P + j;
-            const T p = static_cast<const T*>(d.pr[P_PK0 + ((kk & 1) ^ 1)])[k];
-            T* x = static_cast<T*>(d.pr[P_X]);
-            T* r = static_cast<T*>(d.pr[P_R]);
-            const T q = static_cast<const T*>(d.pr[P_Q])[k];
-            const T aP = static_cast<const T*>(d.pr[P_AP])[k];
-            x[k] += alpha * p;
-            const T rn = r[k] - alpha * q;
+            x[k] += alpha * pk[k];
+            const T rn = r[k] - alpha * Q[k];
             r[k] = rn;
-            v[0] = double(rn) * double(rn);
-            v[1] = double(rn) * double(rn / aP);
+            v[0] += double(rn) * double(rn);
+            v[1] += double(rn) * double(rn / aP[k]);
         }
         blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials);
     }
@@ -464,48 +561,55 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
 // Mixed mode, FP64 iterative refinement of p': p' += x (FP32 correction),
 // then (if another cycle follows) r64 = b - A p' in FP64, r = r64, x = 0.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_refine_accum(KArgs a) {
     const int nT = *a.nTiles;
     for (int t = blockIdx.x; t < nT; t += gridDim.x) {
-        const Tile tl = decodeTile(a, t);
+        const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
         if (st.status != ST_RUNNING || st.stepPhase) continue;
-        const int M = d.M, N = d.N, P = d.pitch, i = tl.i, j = tl.j;
-        if (isPDof(i, j, M, N)) {
+        const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
+        const T* x = static_cast<const T*>(d.pr[P_X]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = tl.i + r;
+            if (!isPDof(i, j, M, N)) continue;
             const size_t k = size_t(i) * P + j;
-            d.pc[k] += double(static_cast<const T*>(d.pr[P_X])[k]);
+            d.pc[k] += double(x[k]);
         }
     }
 }
 
-template <typename T>
+template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_refine_resid(KArgs a) {
     const int nT = *a.nTiles;
     for (int t = blockIdx.x; t < nT; t += gridDim.x) {
-        const Tile tl = decodeTile(a, t);
+        const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
         if (st.status != ST_RUNNING || st.stepPhase) continue;
-        const int M = d.M, N = d.N, P = d.pitch, i = tl.i, j = tl.j;
+        const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
+        const T* aP = static_cast<const T*>(d.pr[P_AP]);
+        const T* cE = static_cast<const T*>(d.pr[P_CE]);
+        const T* cN = static_cast<const T*>(d.pr[P_CN]);
+        const double* pc = d.pc;
         double v[2] = {0.0, 0.0};
-        if (isPDof(i, j, M, N)) {
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+            const int i = tl.i + rr;
+            if (!isPDof(i, j, M, N)) continue;
             const size_t k = size_t(i) * P + j;
-            const T* aP = static_cast<const T*>(d.pr[P_AP]);
-            const T* cE = static_cast<const T*>(d.pr[P_CE]);
-            const T* cN = static_cast<const T*>(d.pr[P_CN]);
-            const double* pc = d.pc;
-            auto pAt = [&](int ii, int jj, size_t kn) -> double { return isPDof(ii, jj, M, N) ? pc[kn] : 0.0; };
-            const double Ap = -double(cN[k - P]) * pAt(i - 1, j, k - P) - double(cE[k - 1]) * pAt(i, j - 1, k - 1) +
-                              double(aP[k]) * pc[k] - double(cE[k]) * pAt(i, j + 1, k + 1) -
-                              double(cN[k]) * pAt(i + 1, j, k + P);
+            auto pAt64 = [&](int ii, int jj, size_t kn) -> double { return isPDof(ii, jj, M, N) ? pc[kn] : 0.0; };
+            const double Ap = -double(cN[k - P]) * pAt64(i - 1, j, k - P) - double(cE[k - 1]) * pAt64(i, j - 1, k - 1) +
+                              double(aP[k]) * pc[k] - double(cE[k]) * pAt64(i, j + 1, k + 1) -
+                              double(cN[k]) * pAt64(i + 1, j, k + P);
             const double r64 = d.bm[k] - Ap;
             const T r = T(r64);
             static_cast<T*>(d.pr[P_R])[k] = r;
             static_cast<T*>(d.pr[P_X])[k] = T(0);
-            v[0] = r64 * r64;
-            v[1] = double(r) * double(r / aP[k]);
+            v[0] += r64 * r64;
+            v[1] += double(r) * double(r / aP[k]);
         }
         blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials);
     }
@@ -586,18 +690,22 @@ __device__ void controller(const SlotDesc& d, SlotState& st) {
 // (iterations.cpp:456-500, boundaries.cpp:48-134). Its finalisation runs the
 // slot's outer-loop scalar logic when the step is an outer iteration.
 // ---------------------------------------------------------------------------
+template <int R>
 __global__ __launch_bounds__(kBlock) void k_correct(KArgs a) {
     const int nT = *a.nTiles;
     for (int t = blockIdx.x; t < nT; t += gridDim.x) {
-        const Tile tl = decodeTile(a, t);
+        const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
         if (st.status != ST_RUNNING) continue;
-        const int M = d.M, N = d.N, P = d.pitch, i = tl.i, j = tl.j;
+        const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
         const double maxVel = 3.0 * fmax(fabs(st.inletVelocity), 0.1);
         const double Uin = st.inletVelocity;
         double bad = 0.0;
-        if (i <= M && j <= N) {
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+            const int i = tl.i + rr;
+            if (i > M || j > N) continue;
             const size_t k = size_t(i) * P + j;
             const double* pc = d.pc;
             // pressure update + ghosts; each ghost has exactly one writer
@@ -859,20 +967,30 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
 // ---------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------
-template <typename T>
-static void launchT(Kern k, int grid, hipStream_t s, const KArgs& a) {
+template <typename T, int R>
+static void launchTR(Kern k, int grid, hipStream_t s, const KArgs& a) {
     const dim3 g(grid), b(kBlock);
     switch (k) {
-        case K_MOM_ASM: hipLaunchKernelGGL(k_mom_assemble<T>, g, b, 0, s, a); break;
-        case K_JAC: hipLaunchKernelGGL(k_jacobi<T>, g, b, 0, s, a); break;
-        case K_MOM_FIN: hipLaunchKernelGGL(k_mom_finish<T>, g, b, 0, s, a); break;
-        case K_PRES_ASM: hipLaunchKernelGGL(k_pres_assemble<T>, g, b, 0, s, a); break;
-        case K_PCG_Q: hipLaunchKernelGGL(k_pcg_q<T>, g, b, 0, s, a); break;
-        case K_PCG_U: hipLaunchKernelGGL(k_pcg_update<T>, g, b, 0, s, a); break;
-        case K_REF_ACC: hipLaunchKernelGGL(k_refine_accum<T>, g, b, 0, s, a); break;
-        case K_REF_RES: hipLaunchKernelGGL(k_refine_resid<T>, g, b, 0, s, a); break;
+        case K_MOM_ASM: hipLaunchKernelGGL((k_mom_assemble<T, R>), g, b, 0, s, a); break;
+        case K_JAC: hipLaunchKernelGGL((k_jacobi<T, R>), g, b, 0, s, a); break;
+        case K_MOM_FIN: hipLaunchKernelGGL((k_mom_finish<T, R>), g, b, 0, s, a); break;
+        case K_PRES_ASM: hipLaunchKernelGGL((k_pres_assemble<T, R>), g, b, 0, s, a); break;
+        case K_PCG_Q: hipLaunchKernelGGL((k_pcg_q<T, R>), g, b, 0, s, a); break;
+        case K_PCG_U: hipLaunchKernelGGL((k_pcg_update<T, R>), g, b, 0, s, a); break;
+        case K_REF_ACC: hipLaunchKernelGGL((k_refine_accum<T, R>), g, b, 0, s, a); break;
+        case K_REF_RES: hipLaunchKernelGGL((k_refine_resid<T, R>), g, b, 0, s, a); break;
         case K_CORR:
-        case K_CORR_OUTER: hipLaunchKernelGGL(k_correct, g, b, 0, s, a); break;
+        case K_CORR_OUTER: hipLaunchKernelGGL((k_correct<R>), g, b, 0, s, a); break;
+    }
+}
+
+template <typename T>
+static void launchT(Kern k, int rows, int grid, hipStream_t s, const KArgs& a) {
+    switch (rows) {
+        case 1: launchTR<T, 1>(k, grid, s, a); break;
+        case 2: launchTR<T, 2>(k, grid, s, a); break;
+        case 8: launchTR<T, 8>(k, grid, s, a); break;
+        default: launchTR<T, 4>(k, grid, s, a); break;
     }
 }
 
@@ -892,13 +1010,13 @@ static int phaseOf(Kern k) {
     }
 }
 
-hipError_t launchKernel(Kern k, bool f32, int grid, int nslots, hipStream_t s, const KArgs& a, hipEvent_t evBefore,
-                        hipEvent_t evAfter) {
+hipError_t launchKernel(Kern k, bool f32, int rows, int grid, int nslots, hipStream_t s, const KArgs& a,
+                        hipEvent_t evBefore, hipEvent_t evAfter) {
     // external records: inside stream capture they become event-record nodes
     // that timestamp every replay (a plain record would only add an edge)
     if (evBefore) (void)hipEventRecordWithFlags(evBefore, s, hipEventRecordExternal);
-    if (f32) launchT<float>(k, grid, s, a);
-    else launchT<double>(k, grid, s, a);
+    if (f32) launchT<float>(k, rows, grid, s, a);
+    else launchT<double>(k, rows, grid, s, a);
     if (evAfter) (void)hipEventRecordWithFlags(evAfter, s, hipEventRecordExternal);
     const int ph = phaseOf(k);
     if (ph >= 0)

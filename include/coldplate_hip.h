@@ -245,6 +245,14 @@ int cp_batch_plane(cp_batch* b, int slot, double x, double gamma_min, cp_plane_m
 /* Release a slot's arena space. */
 int cp_batch_evict(cp_batch* b, int slot);
 
+/* Launch shape of the batch (no counterpart in the reference): tile_rows R
+   in {1, 2, 4, 8} gives 64 x 4R-node tiles for every slot (0 = keep; the
+   default is 4), wg_per_cu the workgroups per CU of the replayed launches
+   (0 = keep; default 8). Changing R retiles the admitted slots; results of a
+   slot depend on R only through the order of its partial sums. Chosen by
+   the autotuner (coldplate-solver_amd/autotune.py). */
+int cp_batch_set_tiling(cp_batch* b, int tile_rows, int wg_per_cu);
+
 /* Work kernels of one outer iteration, for cp_batch_probe. */
 #define CP_K_MOM_ASSEMBLE 0
 #define CP_K_JACOBI       1
