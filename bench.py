@@ -71,6 +71,13 @@ def parse():
     ap.add_argument("--no-dropin", action="store_true")
     ap.add_argument("--no-probe", action="store_true",
                     help="no event nodes in the replayed graph (for runs under rocprofv3)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run 3 device-to-device copies of MIB MiB (byte-counter calibration)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the step's kernels directly instead of replaying the captured graph")
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="stop each batch after this many outer iterations (profiling runs only; "
+                         "such solves do not count as converged)")
     return ap.parse_args()
 
 
@@ -112,19 +119,18 @@ def solve_batch(b: cb.Batch, geoms, a, probe: bool):
         info = b.poll()
         t = time.perf_counter() - t0
         if probe:
+            # sample the last replay when every slot of the batch ran all of
+            # its PCG iterations: each probed launch then covered all unknowns
             ms = b.probe_read()
-            # slots that ran the last replay: still running, or converged at this step
-            act = [s for s in slots if info[s]["status"] == cb.SLOT_RUNNING or
-                   (s in running and info[s]["iter"] == steps)]
-            full = all(info[s]["itP"] == a.pcg * a.cycles for s in act)
-            if act and full and len(ms):
-                samples.append((ms, sum(npres[s] for s in act)))
+            full = all(info[s]["status"] == cb.SLOT_RUNNING and info[s]["itP"] == a.pcg * a.cycles for s in slots)
+            if full and len(ms):
+                samples.append((ms, sum(npres[s] for s in slots)))
         for s in list(running):
             if info[s]["status"] != cb.SLOT_RUNNING:
                 running.discard(s)
                 done_at[s] = t
                 iters[s] = info[s]["iter"]
-        if steps > 200000:
+        if steps > 200000 or (a.profile_steps and steps >= a.profile_steps):
             break
     elapsed = time.perf_counter() - t0
     dps, status = [], []
@@ -189,6 +195,25 @@ def cpu_baseline(a, outer_iters: int):
                 f"reference algorithm needs on this geometry")
 
 
+def copy_calibration(mib: int) -> None:
+    """Device-to-device copy of a known size (hipMemcpy), so that HBM byte
+    counters read under rocprofv3 --pmc can be scaled against it."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(n)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(dst, src, C.c_size_t(n), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"copy calibration: 3 x {mib} MiB device-to-device")
+
+
 def main() -> int:
     a = parse()
     if a.gpus != 1:
@@ -198,10 +223,18 @@ def main() -> int:
     lib = cb.load()
     if lib.cp_device_count() < 1:
         raise SystemExit("no HIP device")
+    if a.calibrate_copy:
+        copy_calibration(a.calibrate_copy)
     B = a.batch
     refs = references(B)
     geoms = [wl.network_1024x512(s) for s in range(B)]
-    b = cb.Batch(cb.CP_PREC_MIXED, 24 << 30, B)
+    # arena: the packed geometries' footprint plus a margin
+    probe_b = cb.Batch(cb.CP_PREC_MIXED, 1 << 20, 1)
+    need = sum(probe_b.footprint(p.M, p.N) for p, _ in geoms)
+    probe_b.close()
+    b = cb.Batch(cb.CP_PREC_MIXED, need + (64 << 20), B)
+    if a.no_graph:
+        b.set_graph(False)
     if not a.no_probe:
         b.probe(cb.K_PCG_Q)
     for w in range(a.warmup):

@@ -155,6 +155,7 @@ EXPORTS = [
     ("cp_batch_plane", C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.POINTER(PlaneMetrics)]),
     ("cp_batch_evict", C.c_int, [_vp, C.c_int]),
     ("cp_batch_set_tiling", C.c_int, [_vp, C.c_int, C.c_int]),
+    ("cp_batch_set_graph", C.c_int, [_vp, C.c_int]),
     ("cp_batch_probe", C.c_int, [_vp, C.c_int]),
     ("cp_batch_probe_read", C.c_int, [_vp, _dp, C.c_int]),
     ("cp_geom_straight", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
@@ -331,6 +332,9 @@ class Batch:
 
     def set_tiling(self, tile_rows: int, wg_per_cu: int = 0) -> None:
         check(load().cp_batch_set_tiling(self.h, tile_rows, wg_per_cu), self.h)
+
+    def set_graph(self, use_graph: bool) -> None:
+        check(load().cp_batch_set_graph(self.h, 1 if use_graph else 0), self.h)
 
     def probe(self, kernel: int) -> None:
         check(load().cp_batch_probe(self.h, kernel), self.h)

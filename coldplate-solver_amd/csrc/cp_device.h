@@ -22,6 +22,7 @@ constexpr int kTileW = 64;
 constexpr int kTileH = 4;
 constexpr int kBlock = kTileW * kTileH;
 constexpr int kNumPartials = 4;  // per-tile reduction partials (doubles)
+constexpr int kNumXCD = 8;       // accelerator complex dies of an MI355X
 
 // slot status
 constexpr int ST_RUNNING = 1;

@@ -168,6 +168,8 @@ struct cp_batch {
     int tileRows = 4;
     // workgroups per CU of the replayed graph's launches
     int wgPerCU = 8;
+    // replay the captured step graph (1) or launch its kernels directly (0)
+    int useGraph = 1;
 
     size_t tElem() const { return precision == CP_PREC_MIXED ? sizeof(float) : sizeof(double); }
     size_t bytesPerNode() const {
@@ -189,9 +191,12 @@ struct cp_ctx {
 
 namespace {
 
+// Workgroups of a launch: enough for the tiles, at most wgPerCU per CU, and
+// a multiple of the XCD count (the kernels split the tile list per XCD).
 int gridFor(const cp_batch* b, int tiles) {
-    const int cap = b->numCU * 8;
-    return std::max(1, std::min(tiles, cap));
+    const int cap = b->numCU * b->wgPerCU;
+    const int g = std::max(1, std::min(tiles, cap));
+    return (g + kNumXCD - 1) / kNumXCD * kNumXCD;
 }
 
 KArgs kargs(cp_batch* b) {
@@ -658,21 +663,32 @@ int graphRun(cp_batch* b, int steps) {
     const int J = maxOverSlots(b, &SlotDesc::maxMomIter);
     const int P = maxOverSlots(b, &SlotDesc::maxPresIter);
     const int R = maxOverSlots(b, &SlotDesc::refineCycles);
+    if (b->probeKernel >= 0) {
+        const size_t need = size_t(2) * (J + 2 * P * R + 8);
+        while (b->probeEv.size() < need) {
+            hipEvent_t ev;
+            HIP_TRY(hipEventCreate(&ev));
+            b->probeEv.push_back(ev);
+        }
+    }
+    if (!b->useGraph) {
+        // the same fixed-budget sequence, launched kernel by kernel; the
+        // probe keeps the events of the last step
+        const int grid = gridFor(b, 1 << 30);
+        for (int k = 0; k < steps; ++k) {
+            b->probeUsed = 0;
+            const int rc = enqueueFixedStep(b, J, P, R, grid);
+            if (rc) return rc;
+        }
+        return CP_OK;
+    }
     if (!b->graphExec || J != b->graphJ || P != b->graphP || R != b->graphR) {
         if (b->graphExec) {
             HIP_TRY(hipGraphExecDestroy(b->graphExec));
             b->graphExec = nullptr;
         }
-        const int grid = b->numCU * b->wgPerCU;  // fixed: independent of the admitted slots
+        const int grid = gridFor(b, 1 << 30);  // fixed: independent of the admitted slots
         hipGraph_t g;
-        if (b->probeKernel >= 0) {
-            const size_t need = size_t(2) * (J + 2 * P * R + 8);
-            while (b->probeEv.size() < need) {
-                hipEvent_t ev;
-                HIP_TRY(hipEventCreate(&ev));
-                b->probeEv.push_back(ev);
-            }
-        }
         b->probeUsed = 0;
         HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         const int rc = enqueueFixedStep(b, J, P, R, grid);
@@ -891,6 +907,12 @@ int cp_batch_set_tiling(cp_batch* b, int tile_rows, int wg_per_cu) {
     }
     if (wg_per_cu > 0) b->wgPerCU = wg_per_cu;
     b->graphJ = -1;
+    return CP_OK;
+}
+
+int cp_batch_set_graph(cp_batch* b, int use_graph) {
+    if (!b) return setError(nullptr, CP_E_ARG, "null batch");
+    b->useGraph = use_graph ? 1 : 0;
     return CP_OK;
 }
 

@@ -126,6 +126,18 @@ __device__ __forceinline__ Tile decodeTile(const KArgs& a, int t) {
     return tl;
 }
 
+// XCD-aware tile order. The hardware dispatches workgroup b to XCD b mod 8
+// (each XCD has its own L2), and launches use a multiple of 8 workgroups. Each
+// XCD walks one contiguous range of the tile list, so tiles that share halo
+// cache lines (east/west and north/south neighbours) meet in the same L2
+// instead of fetching those lines once per XCD.
+#define XCD_TILE_LOOP(t, nT)                                                     \
+    const int xcd_ = int(blockIdx.x) % kNumXCD, lb_ = int(blockIdx.x) / kNumXCD; \
+    const int per_ = ((nT) + kNumXCD - 1) / kNumXCD;                              \
+    const int tEnd_ = min((nT), (xcd_ + 1) * per_);                               \
+    const int gx_ = int(gridDim.x) / kNumXCD;                                     \
+    for (int t = xcd_ * per_ + lb_; t < tEnd_; t += gx_)
+
 __device__ __forceinline__ double clampv(double x, double m) { return fmax(-m, fmin(m, x)); }
 
 // ---------------------------------------------------------------------------
@@ -229,7 +241,7 @@ __device__ __forceinline__ void momRow(const SlotDesc& d, size_t k, int i, int j
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_mom_assemble(KArgs a) {
     const int nT = *a.nTiles;
-    for (int t = blockIdx.x; t < nT; t += gridDim.x) {
+    XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
@@ -280,7 +292,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(KArgs a) {
     __shared__ T sU[HH][HW];
     __shared__ T sV[HH][HW];
     const int nT = *a.nTiles;
-    for (int t = blockIdx.x; t < nT; t += gridDim.x) {
+    XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
@@ -342,7 +354,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(KArgs a) {
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_mom_finish(KArgs a) {
     const int nT = *a.nTiles;
-    for (int t = blockIdx.x; t < nT; t += gridDim.x) {
+    XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
@@ -400,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_mom_finish(KArgs a) {
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_pres_assemble(KArgs a) {
     const int nT = *a.nTiles;
-    for (int t = blockIdx.x; t < nT; t += gridDim.x) {
+    XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
@@ -472,7 +484,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_q(KArgs a) {
     __shared__ T scE[TH][kTileW + 1];
     __shared__ T scN[TH + 1][kTileW];
     const int nT = *a.nTiles;
-    for (int t = blockIdx.x; t < nT; t += gridDim.x) {
+    XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
@@ -528,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_q(KArgs a) {
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
     const int nT = *a.nTiles;
-    for (int t = blockIdx.x; t < nT; t += gridDim.x) {
+    XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
@@ -564,7 +576,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_refine_accum(KArgs a) {
     const int nT = *a.nTiles;
-    for (int t = blockIdx.x; t < nT; t += gridDim.x) {
+    XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
@@ -584,7 +596,7 @@ __global__ __launch_bounds__(kBlock) void k_refine_accum(KArgs a) {
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_refine_resid(KArgs a) {
     const int nT = *a.nTiles;
-    for (int t = blockIdx.x; t < nT; t += gridDim.x) {
+    XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
@@ -693,7 +705,7 @@ __device__ void controller(const SlotDesc& d, SlotState& st) {
 template <int R>
 __global__ __launch_bounds__(kBlock) void k_correct(KArgs a) {
     const int nT = *a.nTiles;
-    for (int t = blockIdx.x; t < nT; t += gridDim.x) {
+    XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];

@@ -253,6 +253,12 @@ int cp_batch_evict(cp_batch* b, int slot);
    the autotuner (coldplate-solver_amd/autotune.py). */
 int cp_batch_set_tiling(cp_batch* b, int tile_rows, int wg_per_cu);
 
+/* 1 (default): cp_batch_run replays the outer iteration as a captured
+   hipGraph; 0: it launches the same kernel sequence directly (no
+   counterpart in the reference; used for tracing tools that do not follow
+   graph replays). */
+int cp_batch_set_graph(cp_batch* b, int use_graph);
+
 /* Work kernels of one outer iteration, for cp_batch_probe. */
 #define CP_K_MOM_ASSEMBLE 0
 #define CP_K_JACOBI       1
