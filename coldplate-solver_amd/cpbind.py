@@ -158,6 +158,7 @@ EXPORTS = [
     ("cp_batch_set_graph", C.c_int, [_vp, C.c_int]),
     ("cp_batch_probe", C.c_int, [_vp, C.c_int]),
     ("cp_batch_probe_read", C.c_int, [_vp, _dp, C.c_int]),
+    ("cp_batch_probe_burst", C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     ("cp_geom_straight", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
     ("cp_geom_network", C.c_int, [C.c_int, C.c_int, C.c_uint64, _dp]),
     ("cp_geom_gamma", C.c_int, [C.c_int, C.c_int, C.c_uint64, C.c_double, C.c_double, C.c_double, _dp]),
@@ -345,6 +346,12 @@ class Batch:
         if n < 0:
             check(n, self.h)
         return out[:n].copy()
+
+    def probe_burst(self, launches: int, kernel: int = K_PCG_Q) -> float:
+        """Mean ms per launch of `launches` back-to-back k_pcg_q launches."""
+        ms = C.c_double(0.0)
+        check(load().cp_batch_probe_burst(self.h, kernel, launches, C.byref(ms)), self.h)
+        return ms.value
 
     def evict(self, slot: int) -> None:
         check(load().cp_batch_evict(self.h, slot), self.h)

@@ -93,6 +93,7 @@ struct KArgs {
     const int* tileSlot;
     const int* nTiles;
     double* partials;
+    int force;  // k_pcg_q only: also run slots whose PCG has stopped (timing bursts)
 };
 
 // Finalisation phases (k_finalize), one per reducing work kernel.
@@ -119,6 +120,9 @@ enum Kern {
 // before and right after the work kernel (not around the finalisation).
 hipError_t launchKernel(Kern k, bool f32, int rows, int grid, int nslots, hipStream_t s, const KArgs& a,
                         hipEvent_t evBefore = nullptr, hipEvent_t evAfter = nullptr);
+// The work kernel alone, without its finalisation (timing bursts of an
+// idempotent kernel).
+hipError_t launchWorkOnly(Kern k, bool f32, int rows, int grid, hipStream_t s, const KArgs& a);
 hipError_t launchPlane(int nreq, hipStream_t s, const SlotDesc* desc, const PlaneReq* req, PlaneOut* out);
 
 }  // namespace cpdev

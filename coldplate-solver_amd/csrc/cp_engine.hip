@@ -163,6 +163,7 @@ struct cp_batch {
     // measurement probe: events around every launch of one work kernel
     int probeKernel = -1;
     std::vector<hipEvent_t> probeEv;
+    hipEvent_t burstEv[2] = {nullptr, nullptr};
     int probeUsed = 0;
     // tile height R (rows per wave) shared by every slot: 1, 2, 4 or 8
     int tileRows = 4;
@@ -206,6 +207,7 @@ KArgs kargs(cp_batch* b) {
     a.tileSlot = b->dTileSlot;
     a.nTiles = b->dNTiles;
     a.partials = b->dPartials;
+    a.force = 0;
     return a;
 }
 
@@ -475,6 +477,8 @@ void batchDestroy(cp_batch* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->graphExec) (void)hipGraphExecDestroy(b->graphExec);
     for (hipEvent_t ev : b->probeEv) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : b->burstEv)
+        if (ev) (void)hipEventDestroy(ev);
     (void)hipFree(b->arenaBase);
     (void)hipFree(b->dDesc);
     (void)hipFree(b->dState);
@@ -935,6 +939,31 @@ int cp_batch_probe_read(cp_batch* b, double* ms, int cap) {
         ms[n++] = double(t);
     }
     return n;
+}
+
+int cp_batch_probe_burst(cp_batch* b, int kernel, int launches, double* ms_per_launch) {
+    if (!b || !ms_per_launch) return setError(nullptr, CP_E_ARG, "null argument");
+    std::string& err = b->err;
+    if (kernel != int(K_PCG_Q))
+        return setError(&err, CP_E_ARG, "timing bursts are defined for CP_K_PCG_Q only (an idempotent kernel)");
+    if (launches < 1 || launches > 4096) return setError(&err, CP_E_ARG, "launches must be in [1, 4096]");
+    for (hipEvent_t& ev : b->burstEv)
+        if (!ev) HIP_TRY(hipEventCreate(&ev));
+    KArgs a = kargs(b);
+    a.force = 1;
+    const int grid = gridFor(b, 1 << 30);
+    const bool f32 = b->precision == CP_PREC_MIXED;
+    HIP_TRY(hipEventRecord(b->burstEv[0], b->stream));
+    for (int k = 0; k < launches; ++k) {
+        const hipError_t e = launchWorkOnly(K_PCG_Q, f32, b->tileRows, grid, b->stream, a);
+        if (e != hipSuccess) return setError(&err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+    }
+    HIP_TRY(hipEventRecord(b->burstEv[1], b->stream));
+    HIP_TRY(hipEventSynchronize(b->burstEv[1]));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, b->burstEv[0], b->burstEv[1]));
+    *ms_per_launch = double(t) / launches;
+    return CP_OK;
 }
 
 // ---- single-geometry context ------------------------------------------------

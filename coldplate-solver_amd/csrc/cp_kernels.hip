@@ -488,7 +488,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_q(KArgs a) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
-        if (st.status != ST_RUNNING || st.pdone || st.stepPhase) continue;
+        if (st.status != ST_RUNNING || ((st.pdone || st.stepPhase) && !a.force)) continue;
         const int M = d.M, N = d.N, P = d.pitch;
         const int kk = st.pk;
         const T beta = T(st.beta);
@@ -1033,6 +1033,12 @@ hipError_t launchKernel(Kern k, bool f32, int rows, int grid, int nslots, hipStr
     const int ph = phaseOf(k);
     if (ph >= 0)
         hipLaunchKernelGGL(k_finalize, dim3(nslots), dim3(kBlock), 0, s, a, ph, k == K_CORR_OUTER ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launchWorkOnly(Kern k, bool f32, int rows, int grid, hipStream_t s, const KArgs& a) {
+    if (f32) launchT<float>(k, rows, grid, s, a);
+    else launchT<double>(k, rows, grid, s, a);
     return hipGetLastError();
 }
 

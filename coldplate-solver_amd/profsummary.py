@@ -1,0 +1,172 @@
+"""Summarise rocprofv3 output of bench.py into a short text profile.
+
+Reads three rocprofv3 runs of the same bench workload, each a directory
+holding either the rocpd database (*_results.db, the default output) or the
+CSV files (--output-format csv):
+  * --kernel-trace --stats   -> <stats>
+  * --pmc FETCH_SIZE         -> <fetch>
+  * --pmc WRITE_SIZE         -> <write>
+and the bench's own JSON line from the kernel-trace run's log (its live,
+event-timed average launch duration of the same kernel).
+
+The byte counters are scaled against a device-to-device copy of known size
+run in the same process (bench.py --calibrate-copy MIB): the copy reads and
+writes MIB MiB, so factor = MIB * 1024 / counter KiB for each counter.
+
+Writes <out>.md (human summary with the commands) and <out>.json (the
+traffic figure bench.py reports as roofline.traffic).
+
+    python coldplate-solver_amd/profsummary.py --stats D1 --fetch D2 --write D3 \
+        --log D1.log --copy-mib 256 --kernel k_pcg_q --out profiles/k_pcg_q_network_1024x512_b8
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import glob
+import json
+import os
+import sqlite3
+import sys
+from collections import defaultdict
+
+
+def _one(pattern: str) -> str:
+    files = sorted(glob.glob(pattern))
+    if not files:
+        raise SystemExit(f"no file matches {pattern}")
+    return files[0]
+
+
+def _db(d: str) -> str | None:
+    files = sorted(glob.glob(os.path.join(d, "*_results.db")))
+    return files[0] if files else None
+
+
+def kernel_stats(d: str) -> list[dict]:
+    db = _db(d)
+    if db:
+        c = sqlite3.connect(db)
+        rows = c.execute("select name, count(*), sum(duration), avg(duration), min(duration), max(duration) "
+                         "from kernels group by name").fetchall()
+        total = sum(r[2] for r in rows) or 1
+        return [{"Name": r[0], "Calls": str(r[1]), "TotalDurationNs": str(r[2]), "AverageNs": str(r[3]),
+                 "MinNs": str(r[4]), "MaxNs": str(r[5]), "Percentage": str(100.0 * r[2] / total)} for r in rows]
+    with open(_one(os.path.join(d, "*kernel_stats.csv"))) as f:
+        return list(csv.DictReader(f))
+
+
+def counter_per_dispatch(d: str, counter: str) -> dict[str, list[float]]:
+    """Kernel name -> list of per-dispatch counter values (summed over the
+    dimensions rocprofv3 reports separately)."""
+    per = defaultdict(float)
+    names = {}
+    db = _db(d)
+    if db:
+        c = sqlite3.connect(db)
+        for disp, name, val in c.execute("select dispatch_id, kernel_name, value from counters_collection "
+                                         "where counter_name = ?", (counter,)):
+            per[disp] += float(val)
+            names[disp] = name
+    else:
+        with open(_one(os.path.join(d, "*counter_collection.csv"))) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] != counter:
+                    continue
+                key = r["Dispatch_Id"]
+                per[key] += float(r["Counter_Value"])
+                names[key] = r["Kernel_Name"]
+    out = defaultdict(list)
+    for key, v in per.items():
+        out[names[key]].append(v)
+    return out
+
+
+def pick(d: dict[str, list[float]], sub: str) -> tuple[str, list[float]]:
+    for k, v in d.items():
+        if sub in k:
+            return k, v
+    raise SystemExit(f"no kernel matching {sub}")
+
+
+def bench_line(log: str) -> dict | None:
+    if not log or not os.path.exists(log):
+        return None
+    for line in reversed(open(log, errors="replace").read().splitlines()):
+        line = line.strip()
+        if line.startswith("{") and '"metric"' in line:
+            return json.loads(line)
+    return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--stats", required=True)
+    ap.add_argument("--fetch", required=True)
+    ap.add_argument("--write", required=True)
+    ap.add_argument("--log", default="")
+    ap.add_argument("--copy-mib", type=int, required=True)
+    ap.add_argument("--kernel", default="k_pcg_q")
+    ap.add_argument("--workload", default="network_1024x512, 8 geometries, mixed precision")
+    ap.add_argument("--commands", default="", help="text file holding the exact commands that were run")
+    ap.add_argument("--out", required=True)
+    a = ap.parse_args()
+
+    stats = kernel_stats(a.stats)
+    fetch = counter_per_dispatch(a.fetch, "FETCH_SIZE")
+    write = counter_per_dispatch(a.write, "WRITE_SIZE")
+    copy_kib = a.copy_mib * 1024.0
+    # the calibration copy is the largest copyBuffer dispatch
+    _, cf = pick(fetch, "copyBuffer")
+    _, cw = pick(write, "copyBuffer")
+    f_copy, w_copy = max(cf), max(cw)
+    f_factor, w_factor = copy_kib / f_copy, copy_kib / w_copy
+    kname, kf = pick(fetch, a.kernel)
+    _, kw = pick(write, a.kernel)
+    f_avg = sum(kf) / len(kf)
+    w_avg = sum(kw) / len(kw)
+    traffic = (f_avg * f_factor + w_avg * w_factor) * 1024.0
+
+    b = bench_line(a.log)
+    krow = next(r for r in stats if a.kernel in r["Name"])
+    rocprof_avg_us = float(krow["AverageNs"]) / 1e3
+    live_us = b["roofline"]["avg_launch_us"] if b else None
+    alg = b["roofline"]["bytes_per_launch"] if b else None
+
+    lines = [f"# {a.kernel} — {a.workload}", ""]
+    if a.commands and os.path.exists(a.commands):
+        lines += ["Commands (repository root, one rocprofv3 run each):", "", "```"]
+        lines += open(a.commands).read().strip().splitlines()
+        lines += ["```", ""]
+    lines += ["## Kernel time (rocprofv3 --kernel-trace --stats)", "",
+              "| kernel | calls | avg µs | min µs | max µs | % time |", "|---|---|---|---|---|---|"]
+    for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:14]:
+        lines.append(f"| `{r['Name']}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+                     f"{float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
+    lines += ["", "## Live timing vs rocprof", ""]
+    lines.append(f"* rocprof average of `{krow['Name']}`: {rocprof_avg_us:.2f} µs over {krow['Calls']} launches")
+    if b:
+        lines.append(f"* bench.py HIP-event average of the same kernel in the same run: {live_us:.2f} µs "
+                     f"over {b['roofline']['launches_timed']} sampled launches")
+        lines.append(f"* algorithmic bytes per launch: {alg} (28 B per p' unknown)")
+        lines.append(f"* achieved (bench, events): {b['roofline']['achieved']:.1f} GB/s = "
+                     f"{100 * b['roofline']['frac']:.1f} % of 8000 GB/s; "
+                     f"with the rocprof average: {alg / (rocprof_avg_us * 1e3):.1f} GB/s")
+    lines += ["", "## HBM traffic (rocprofv3 --pmc, one counter per run)", ""]
+    lines.append(f"* calibration: {a.copy_mib} MiB device-to-device copy reads FETCH_SIZE {f_copy:.0f} KiB "
+                 f"and WRITE_SIZE {w_copy:.0f} KiB -> factors FETCH x{f_factor:.3f}, WRITE x{w_factor:.3f}")
+    lines.append(f"* `{kname}`: FETCH_SIZE {f_avg:.0f} KiB/dispatch ({len(kf)} dispatches), "
+                 f"WRITE_SIZE {w_avg:.0f} KiB/dispatch ({len(kw)} dispatches)")
+    lines.append(f"* calibrated traffic per launch: {traffic / 1e6:.1f} MB"
+                 + (f" = {traffic / alg:.2f} x the algorithmic bytes" if alg else ""))
+    open(a.out + ".md", "w").write("\n".join(lines) + "\n")
+    json.dump({"kernel": kname, "workload": a.workload, "fetch_kib": f_avg, "write_kib": w_avg,
+               "fetch_factor": f_factor, "write_factor": w_factor, "traffic_bytes": traffic,
+               "rocprof_avg_us": rocprof_avg_us, "bench_avg_us": live_us, "bytes_per_launch": alg},
+              open(a.out + ".json", "w"), indent=1)
+    print("\n".join(lines))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

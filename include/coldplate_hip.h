@@ -276,6 +276,13 @@ int cp_batch_set_graph(cp_batch* b, int use_graph);
    their count. Call it after cp_batch_poll (which synchronises). */
 int cp_batch_probe(cp_batch* b, int kernel);
 int cp_batch_probe_read(cp_batch* b, double* ms, int cap);
+/* Timing burst (no counterpart in the reference): launch the PCG stencil
+   kernel (CP_K_PCG_Q, the only kernel accepted) `launches` times back to
+   back on the batch stream between two HIP events, over every running
+   slot, and return the mean time per launch in ms. The kernel only
+   recomputes the search direction and q = A p from unchanged inputs, so
+   the slots' state and results are unaffected. Synchronises the stream. */
+int cp_batch_probe_burst(cp_batch* b, int kernel, int launches, double* ms_per_launch);
 
 /* ------------------------------------------------------------------ */
 /* seeded geometry generators (cellType, M x N row-major)              */

@@ -146,3 +146,23 @@ def test_tile_heights_agree_with_oracle(rows):
     for a, f in ((u, cb.F_U), (v, cb.F_V), (pp, cb.F_P)):
         ref = o.field(f)
         assert np.abs(a - ref).max() <= 1e-8 * np.abs(ref).max(), f
+
+
+def test_timing_burst_leaves_results_unchanged():
+    """k_pcg_q timing bursts between replays (as bench.py runs them) do not
+    change any slot's results."""
+    p, ct = wl.network(64, 128, 17, cb.CP_PREC_MIXED)
+    q = wl.fast_mode(p)
+    say("[batch] timing burst")
+    runs = []
+    for burst in (False, True):
+        b = cb.Batch(cb.CP_PREC_MIXED, 64 << 20, 2)
+        s = b.admit(q, ct)
+        for _ in range(4):
+            b.run(5)
+            if burst:
+                assert b.probe_burst(4) > 0.0
+        runs.append(b.fields(s))
+        b.close()
+    for x, y in zip(*runs):
+        np.testing.assert_array_equal(x, y)
