@@ -18,6 +18,8 @@ LIB_PATH = PKG_DIR / "lib" / "libcoldplate_hip.so"
 CP_OK = 0
 CP_PREC_FP64 = 0
 CP_PREC_MIXED = 1
+CP_PRECOND_JACOBI = 0
+CP_PRECOND_AMG = 1
 
 F_U, F_V, F_P, F_USTAR, F_VSTAR, F_PSTAR, F_DE, F_DN, F_GAMMA, F_ALPHA = range(10)
 
@@ -60,6 +62,9 @@ class Params(C.Structure):
         ("refineCycles", C.c_int32),
         ("fixedMomentumSweeps", C.c_int32),
         ("massResidualScope", C.c_int32),
+        ("pressurePrecond", C.c_int32),
+        ("mgOmega", C.c_double),
+        ("mgScale", C.c_double),
     ]
 
     def copy(self) -> "Params":

@@ -23,6 +23,8 @@ constexpr int kTileH = 4;
 constexpr int kBlock = kTileW * kTileH;
 constexpr int kNumPartials = 4;  // per-tile reduction partials (doubles)
 constexpr int kNumXCD = 8;       // accelerator complex dies of an MI355X
+constexpr int kMaxLevels = 12;   // aggregation-multigrid levels (incl. the finest)
+constexpr int kCoarseMax = 64;   // unknowns of the coarsest level (dense inverse)
 
 // slot status
 constexpr int ST_RUNNING = 1;
@@ -32,7 +34,25 @@ constexpr int ST_FAILED = 4;
 
 // Solver-typed arrays (T = double in FP64 mode, float in mixed mode).
 enum MomArr { M_AP = 0, M_AE, M_AW, M_AN, M_AS, M_R0, M_D0, M_D1, M_COUNT };
-enum PresArr { P_CE = 0, P_CN, P_AP, P_R, P_PK0, P_PK1, P_Q, P_X, P_COUNT };
+// P_Z: preconditioned residual z = M^-1 r (multigrid); P_MGX: the finest
+// level's pre-smoothed iterate. P_X stays last: FP64 aliases it to p'.
+enum PresArr { P_CE = 0, P_CN, P_AP, P_R, P_PK0, P_PK1, P_Q, P_Z, P_MGX, P_X, P_COUNT };
+
+// One level of the aggregation-multigrid hierarchy of a slot. Unknown (I, J),
+// 0 <= I < H, 0 <= J < W, is stored at row I+1, column J+1 of an array with
+// `pitch` columns; the finest level is the p' grid itself (p'(i,j) at node
+// (i,j), i = I+1, j = J+1, on the slot's node grid). Level l+1 aggregates
+// 2x2 blocks of level l. Its tiles are 64 x 4R unknowns from (0,0).
+struct MgLevel {
+    int H, W, pitch;
+    int tile0, ntiles, tilesX;
+    void* aP;
+    void* cE;
+    void* cN;
+    void* b;  // right-hand side (finest: the PCG residual r)
+    void* x;  // iterate after pre-smoothing
+    void* z;  // V-cycle result (finest: P_Z)
+};
 
 // Read-only description of one slot; uploaded when a slot is admitted.
 struct alignas(64) SlotDesc {
@@ -45,6 +65,8 @@ struct alignas(64) SlotDesc {
     int maxIterations, inletRamp, rampSteps;
     int enableSER, serMinIter, serUseMax, enableLS, lsMaxTries, enableRamp;
     int mixed;
+    int precond, nLevels;        // CP_PRECOND_*; multigrid levels (>= 2 when precond == 1)
+    double mgOmega, mgScale;
     double hx, hy, rho, vol, De, Dn, convScale, sinkDiag, uvAlpha, pAlpha, hChar;
     double timeStep, minSpeed, momTol, pTol, epsilon, targetVel;
     double serCFLMin, serCFLMax, serSmooth, serMaxInc, serMaxDec;
@@ -56,6 +78,9 @@ struct alignas(64) SlotDesc {
     void* mu[M_COUNT];
     void* mv[M_COUNT];
     void* pr[P_COUNT];
+    // multigrid hierarchy and the dense inverse of its coarsest level
+    double* coarseInv;
+    MgLevel lv[kMaxLevels];
 };
 
 // Mutable per-slot scalars; written only by the finalising tile of a slot
@@ -94,10 +119,11 @@ struct KArgs {
     const int* nTiles;
     double* partials;
     int force;  // k_pcg_q only: also run slots whose PCG has stopped (timing bursts)
+    int level;  // multigrid kernels: hierarchy level of the tile list
 };
 
 // Finalisation phases (k_finalize), one per reducing work kernel.
-enum Phase { PH_JAC = 0, PH_MOM_FIN, PH_PRES_ASM, PH_PCG_Q, PH_PCG_U, PH_REF_ACC, PH_REF_RES, PH_CORR };
+enum Phase { PH_JAC = 0, PH_MOM_FIN, PH_PRES_ASM, PH_PCG_Q, PH_PCG_U, PH_REF_ACC, PH_REF_RES, PH_CORR, PH_MG_Z };
 
 // Kernels of one SIMPLE step, in launch order.
 enum Kern {
@@ -111,6 +137,11 @@ enum Kern {
     K_REF_RES,      // mixed: FP64 residual of p'
     K_CORR,         // p update, velocity correction, BCs
     K_CORR_OUTER,   // K_CORR + outer-loop scalar logic
+    K_MG_COARSEN,   // multigrid: Galerkin coarse operator of level a.level
+    K_MG_FACTOR,    // multigrid: dense inverse of the coarsest level (one block per slot)
+    K_MG_DOWN,      // multigrid: pre-smoothing + restriction at level a.level
+    K_MG_COARSEST,  // multigrid: coarsest-level solve (one block per slot)
+    K_MG_UP,        // multigrid: prolongation + post-smoothing at level a.level
 };
 
 // Host-side launchers (defined next to the kernels). launchKernel enqueues
@@ -123,6 +154,9 @@ hipError_t launchKernel(Kern k, bool f32, int rows, int grid, int nslots, hipStr
 // The work kernel alone, without its finalisation (timing bursts of an
 // idempotent kernel).
 hipError_t launchWorkOnly(Kern k, bool f32, int rows, int grid, hipStream_t s, const KArgs& a);
+// Rows per wave of the finest multigrid level's tiles for a batch tile
+// height (coarser levels use 1); must match what the kernels were built for.
+inline int mgRows(bool f32, int rows) { return f32 ? (rows < 4 ? rows : 4) : (rows < 2 ? rows : 2); }
 hipError_t launchPlane(int nreq, hipStream_t s, const SlotDesc* desc, const PlaneReq* req, PlaneOut* out);
 
 }  // namespace cpdev

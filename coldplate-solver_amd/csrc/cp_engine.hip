@@ -126,6 +126,45 @@ struct SlotHost {
 };
 
 constexpr int kStateArrays = 12;  // u v p us vs pc dE dN aU aV gam bm
+constexpr int kMgArrays = 6;      // aP cE cN b x z of each coarse multigrid level
+
+// Shapes of the aggregation-multigrid hierarchy of an M x N cell grid: level
+// 0 is the (M-1) x (N-2) p' grid, each next level aggregates 2x2 blocks, and
+// coarsening stops once a level has at most kCoarseMax unknowns (at least one
+// coarse level). Returns the number of levels.
+int mgShapes(int M, int N, int* H, int* W) {
+    H[0] = M - 1;
+    W[0] = N - 2;
+    int n = 1;
+    while ((n == 1 || long(H[n - 1]) * W[n - 1] > kCoarseMax) && n < kMaxLevels) {
+        H[n] = (H[n - 1] + 1) / 2;
+        W[n] = (W[n - 1] + 1) / 2;
+        ++n;
+    }
+    return n;
+}
+
+size_t mgPitch(int W) { return roundUp(size_t(W) + 2, 32); }
+
+// Device bytes of one slot: the node-grid arrays, then the coarse multigrid
+// levels and the coarsest dense inverse (allocated for every slot, so that a
+// footprint depends on the grid and the arithmetic mode alone).
+size_t slotMainBytes(int precision, int M, int N) {
+    const size_t tE = precision == CP_PREC_MIXED ? sizeof(float) : sizeof(double);
+    const int presArrays = precision == CP_PREC_MIXED ? P_COUNT : P_COUNT - 1;  // FP64: x aliases p'
+    const size_t perNode = kStateArrays * sizeof(double) + (2 * M_COUNT + presArrays) * tE;
+    const size_t pitch = roundUp(size_t(N) + 1, 32);
+    return roundUp(size_t(M + 1) * pitch * perNode + 256 * 32, 256);
+}
+
+size_t slotMgBytes(int precision, int M, int N) {
+    const size_t tE = precision == CP_PREC_MIXED ? sizeof(float) : sizeof(double);
+    int H[kMaxLevels], W[kMaxLevels];
+    const int n = mgShapes(M, N, H, W);
+    size_t bytes = roundUp(sizeof(double) * kCoarseMax * kCoarseMax, 256);
+    for (int l = 1; l < n; ++l) bytes += kMgArrays * roundUp(size_t(H[l] + 2) * mgPitch(W[l]) * tE, 256);
+    return bytes;
+}
 
 }  // namespace
 
@@ -147,6 +186,10 @@ struct cp_batch {
     SlotState* dState = nullptr;
     int* dTileSlot = nullptr;
     int* dNTiles = nullptr;
+    // multigrid tile lists: level l's list at dMgTileSlot + l * maxTiles
+    int* dMgTileSlot = nullptr;
+    int* dMgNTiles = nullptr;
+    int mgLevels = 0;  // deepest hierarchy over the admitted multigrid slots
     double* dPartials = nullptr;
     PlaneReq* dPlaneReq = nullptr;
     PlaneOut* dPlaneOut = nullptr;
@@ -159,7 +202,7 @@ struct cp_batch {
     int totalTiles = 0;
     // graph mode
     hipGraphExec_t graphExec = nullptr;
-    int graphJ = -1, graphP = -1, graphR = -1;
+    int graphJ = -1, graphP = -1, graphR = -1, graphL = -1;
     // measurement probe: events around every launch of one work kernel
     int probeKernel = -1;
     std::vector<hipEvent_t> probeEv;
@@ -178,10 +221,8 @@ struct cp_batch {
         const int presArrays = precision == CP_PREC_MIXED ? P_COUNT : P_COUNT - 1;  // FP64: x aliases p'
         return kStateArrays * sizeof(double) + (2 * M_COUNT + presArrays) * tElem();
     }
-    size_t footprint(int M, int N) const {
-        const size_t pitch = roundUp(size_t(N) + 1, 32);
-        return roundUp(size_t(M + 1) * pitch * bytesPerNode() + 256 * 32, 256);
-    }
+    size_t mainBytes(int M, int N) const { return slotMainBytes(precision, M, N); }
+    size_t footprint(int M, int N) const { return slotMainBytes(precision, M, N) + slotMgBytes(precision, M, N); }
 };
 
 struct cp_ctx {
@@ -208,6 +249,7 @@ KArgs kargs(cp_batch* b) {
     a.nTiles = b->dNTiles;
     a.partials = b->dPartials;
     a.force = 0;
+    a.level = 0;
     return a;
 }
 
@@ -221,6 +263,51 @@ int launch(cp_batch* b, Kern k, int grid) {
     const hipError_t e =
         launchKernel(k, b->precision == CP_PREC_MIXED, b->tileRows, grid, b->maxSlots, b->stream, kargs(b), e0, e1);
     if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+    return CP_OK;
+}
+
+// ---- multigrid launches ---------------------------------------------------
+// Workgroups of a level's launch: the full grid on the finest level, a
+// quarter per coarser level (the tile loops are grid-stride, so any grid is
+// correct; the shape stays independent of the admitted slots).
+int mgGrid(const cp_batch* b, int level) {
+    int g = gridFor(b, 1 << 30) >> (2 * level);
+    g = std::max(g, 64);
+    return (g + kNumXCD - 1) / kNumXCD * kNumXCD;
+}
+
+int launchMg(cp_batch* b, Kern k, int level) {
+    const bool f32 = b->precision == CP_PREC_MIXED;
+    const bool perSlot = k == K_MG_FACTOR || k == K_MG_COARSEST;
+    KArgs a = kargs(b);
+    a.tileSlot = b->dMgTileSlot + size_t(level) * b->maxTiles;
+    a.nTiles = b->dMgNTiles + level;
+    a.level = level;
+    const int rows = level == 0 ? mgRows(f32, b->tileRows) : 1;
+    const int grid = perSlot ? b->maxSlots : mgGrid(b, level);
+    const hipError_t e = launchKernel(k, f32, rows, grid, b->maxSlots, b->stream, a);
+    if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+    return CP_OK;
+}
+
+// Coarse operators of every level and the coarsest inverse (after assembly).
+int mgSetup(cp_batch* b) {
+    int rc;
+    for (int l = 1; l < b->mgLevels; ++l)
+        if ((rc = launchMg(b, K_MG_COARSEN, l))) return rc;
+    return b->mgLevels > 0 ? launchMg(b, K_MG_FACTOR, 0) : CP_OK;
+}
+
+// z = V-cycle(r) for every multigrid slot whose CG is running, then r.z.
+int mgVcycle(cp_batch* b) {
+    const int L = b->mgLevels;
+    if (L == 0) return CP_OK;
+    int rc;
+    for (int l = 0; l + 1 < L; ++l)
+        if ((rc = launchMg(b, K_MG_DOWN, l))) return rc;
+    if ((rc = launchMg(b, K_MG_COARSEST, 0))) return rc;
+    for (int l = L - 2; l >= 0; --l)
+        if ((rc = launchMg(b, K_MG_UP, l))) return rc;
     return CP_OK;
 }
 
@@ -249,6 +336,26 @@ int rebuildTiles(cp_batch* b) {
     }
     if (ts.size() > b->maxTiles) return setError(&err, CP_E_CAPACITY, "tile table full");
     b->totalTiles = int(ts.size());
+    int counts[kMaxLevels];
+    std::vector<std::vector<int>> lists(kMaxLevels);  // alive until the stream is synchronised
+    b->mgLevels = 0;
+    for (int l = 0; l < kMaxLevels; ++l) {
+        std::vector<int>& ml = lists[l];
+        for (int s = 0; s < b->maxSlots; ++s) {
+            if (!b->slots[s].used) continue;
+            SlotDesc& d = b->hDesc[s];
+            if (l >= d.nLevels) continue;
+            d.lv[l].tile0 = int(ml.size());
+            for (int k = 0; k < d.lv[l].ntiles; ++k) ml.push_back(s);
+            b->mgLevels = std::max(b->mgLevels, d.nLevels);
+        }
+        if (ml.size() > b->maxTiles) return setError(&err, CP_E_CAPACITY, "multigrid tile table full");
+        counts[l] = int(ml.size());
+        if (!ml.empty())
+            HIP_TRY(hipMemcpyAsync(b->dMgTileSlot + size_t(l) * b->maxTiles, ml.data(), ml.size() * sizeof(int),
+                                   hipMemcpyHostToDevice, b->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(b->dMgNTiles, counts, sizeof(counts), hipMemcpyHostToDevice, b->stream));
     if (!ts.empty())
         HIP_TRY(hipMemcpyAsync(b->dTileSlot, ts.data(), ts.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
     HIP_TRY(hipMemcpyAsync(b->dNTiles, &b->totalTiles, sizeof(int), hipMemcpyHostToDevice, b->stream));
@@ -309,6 +416,17 @@ int validate(const cp_params* p, std::string* err) {
     if (!(p->uvAlpha > 0) || !(p->uvAlpha <= 1)) return setError(err, CP_E_ARG, "uvAlpha must be in (0, 1]");
     if (p->precision != CP_PREC_FP64 && p->precision != CP_PREC_MIXED)
         return setError(err, CP_E_ARG, "unknown precision %d", p->precision);
+    if (p->pressurePrecond != CP_PRECOND_JACOBI && p->pressurePrecond != CP_PRECOND_AMG)
+        return setError(err, CP_E_ARG, "unknown pressurePrecond %d", p->pressurePrecond);
+    if (p->pressurePrecond == CP_PRECOND_AMG &&
+        (!(p->mgOmega > 0) || !(p->mgOmega <= 1) || !(p->mgScale > 0) || !(p->mgScale <= 4)))
+        return setError(err, CP_E_ARG, "mgOmega must be in (0, 1] and mgScale in (0, 4]");
+    if (p->pressurePrecond == CP_PRECOND_AMG) {
+        int H[kMaxLevels], W[kMaxLevels];
+        const int n = mgShapes(p->M, p->N, H, W);
+        if (long(H[n - 1]) * W[n - 1] > kCoarseMax)
+            return setError(err, CP_E_ARG, "grid too elongated for %d multigrid levels", kMaxLevels);
+    }
     return CP_OK;
 }
 
@@ -328,6 +446,12 @@ void setTiling(SlotDesc& d, int tileRows) {
     const int rowsPerTile = kTileH * d.tileRows;
     const int tilesY = (d.M + 1 + rowsPerTile - 1) / rowsPerTile;
     d.ntiles = d.tilesX * tilesY;
+    for (int l = 0; l < d.nLevels; ++l) {
+        MgLevel& L = d.lv[l];
+        const int rows = kTileH * (l == 0 ? mgRows(d.mixed != 0, tileRows) : 1);
+        L.tilesX = (L.W + kTileW - 1) / kTileW;
+        L.ntiles = L.tilesX * ((L.H + rows - 1) / rows);
+    }
 }
 
 SlotDesc makeDesc(const cp_params& p, int precision, int tileRows) {
@@ -336,6 +460,19 @@ SlotDesc makeDesc(const cp_params& p, int precision, int tileRows) {
     d.M = p.M;
     d.N = p.N;
     d.pitch = int(roundUp(size_t(p.N) + 1, 32));
+    d.mixed = precision == CP_PREC_MIXED ? 1 : 0;
+    d.precond = p.pressurePrecond;
+    d.mgOmega = p.mgOmega;
+    d.mgScale = p.mgScale;
+    if (d.precond == CP_PRECOND_AMG) {
+        int H[kMaxLevels], W[kMaxLevels];
+        d.nLevels = mgShapes(p.M, p.N, H, W);
+        for (int l = 0; l < d.nLevels; ++l) {
+            d.lv[l].H = H[l];
+            d.lv[l].W = W[l];
+            d.lv[l].pitch = l == 0 ? d.pitch : int(mgPitch(W[l]));
+        }
+    }
     setTiling(d, tileRows);
     d.nU = (p.M - 1) * (p.N - 2);
     d.nV = (p.M - 2) * (p.N - 1);
@@ -409,6 +546,29 @@ void assignPointers(cp_batch* b, SlotDesc& d, char* base) {
     const int presArrays = b->precision == CP_PREC_MIXED ? P_COUNT : P_COUNT - 1;
     for (int k = 0; k < presArrays; ++k) d.pr[k] = tp + (2 * M_COUNT + k) * tb;
     if (b->precision != CP_PREC_MIXED) d.pr[P_X] = d.pc;  // FP64: PCG writes p' directly
+    // multigrid: the finest level is the p' system itself; coarse levels and
+    // the dense inverse follow the slot's main arrays
+    char* mp = base + b->mainBytes(d.M, d.N);
+    d.coarseInv = reinterpret_cast<double*>(mp);
+    mp += roundUp(sizeof(double) * kCoarseMax * kCoarseMax, 256);
+    if (d.nLevels > 0) {
+        MgLevel& L0 = d.lv[0];
+        L0.aP = d.pr[P_AP];
+        L0.cE = d.pr[P_CE];
+        L0.cN = d.pr[P_CN];
+        L0.b = d.pr[P_R];
+        L0.x = d.pr[P_MGX];
+        L0.z = d.pr[P_Z];
+        for (int l = 1; l < d.nLevels; ++l) {
+            MgLevel& L = d.lv[l];
+            const size_t arr = roundUp(size_t(L.H + 2) * L.pitch * b->tElem(), 256);
+            void** ptrs[kMgArrays] = {&L.aP, &L.cE, &L.cN, &L.b, &L.x, &L.z};
+            for (int k = 0; k < kMgArrays; ++k) {
+                *ptrs[k] = mp;
+                mp += arr;
+            }
+        }
+    }
 }
 
 // Copy a pitched device plane to a dense rows x cols host array.
@@ -454,6 +614,8 @@ int batchCreate(int precision, size_t budget, int maxSlots, cp_batch** out, std:
     HIP_TRY(hipMalloc(&b->dState, sizeof(SlotState) * maxSlots));
     HIP_TRY(hipMalloc(&b->dTileSlot, sizeof(int) * b->maxTiles));
     HIP_TRY(hipMalloc(&b->dNTiles, sizeof(int)));
+    HIP_TRY(hipMalloc(&b->dMgTileSlot, sizeof(int) * b->maxTiles * kMaxLevels));
+    HIP_TRY(hipMalloc(&b->dMgNTiles, sizeof(int) * kMaxLevels));
     HIP_TRY(hipMalloc(&b->dPartials, sizeof(double) * kNumPartials * b->maxTiles));
     HIP_TRY(hipMalloc(&b->dPlaneReq, sizeof(PlaneReq) * 2 * maxSlots));
     HIP_TRY(hipMalloc(&b->dPlaneOut, sizeof(PlaneOut) * 2 * maxSlots));
@@ -467,6 +629,7 @@ int batchCreate(int precision, size_t budget, int maxSlots, cp_batch** out, std:
     HIP_TRY(hipMemcpyAsync(b->dState, b->hState, sizeof(SlotState) * maxSlots, hipMemcpyHostToDevice, b->stream));
     const int zero = 0;
     HIP_TRY(hipMemcpyAsync(b->dNTiles, &zero, sizeof(int), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemsetAsync(b->dMgNTiles, 0, sizeof(int) * kMaxLevels, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
     *out = b;
     return CP_OK;
@@ -484,6 +647,8 @@ void batchDestroy(cp_batch* b) {
     (void)hipFree(b->dState);
     (void)hipFree(b->dTileSlot);
     (void)hipFree(b->dNTiles);
+    (void)hipFree(b->dMgTileSlot);
+    (void)hipFree(b->dMgNTiles);
     (void)hipFree(b->dPartials);
     (void)hipFree(b->dPlaneReq);
     (void)hipFree(b->dPlaneOut);
@@ -599,6 +764,7 @@ int polledPressure(cp_batch* b) {
     const int grid = gridFor(b, b->totalTiles);
     int rc = launch(b, K_PRES_ASM, grid);
     if (rc) return rc;
+    if ((rc = mgSetup(b)) || (rc = mgVcycle(b))) return rc;
     const int cap = maxOverSlots(b, &SlotDesc::maxPresIter);
     const int cycles = maxOverSlots(b, &SlotDesc::refineCycles);
     for (int c = 0; c < cycles; ++c) {
@@ -608,6 +774,7 @@ int polledPressure(cp_batch* b) {
             for (int k = 0; k < n; ++k) {
                 if ((rc = launch(b, K_PCG_Q, grid))) return rc;
                 if ((rc = launch(b, K_PCG_U, grid))) return rc;
+                if ((rc = mgVcycle(b))) return rc;
             }
             launched += n;
             if (launched >= cap) break;
@@ -624,6 +791,7 @@ int polledPressure(cp_batch* b) {
         if ((rc = launch(b, K_REF_ACC, grid))) return rc;
         if (c + 1 < cycles) {
             if ((rc = launch(b, K_REF_RES, grid))) return rc;
+            if ((rc = mgVcycle(b))) return rc;
             if ((rc = pollState(b))) return rc;
             bool pending = false;
             for (int s = 0; s < b->maxSlots; ++s)
@@ -649,15 +817,21 @@ int enqueueFixedStep(cp_batch* b, int J, int P, int R, int grid) {
         if ((rc = launch(b, K_JAC, grid))) return rc;
     if ((rc = launch(b, K_MOM_FIN, grid))) return rc;
     if ((rc = launch(b, K_PRES_ASM, grid))) return rc;
+    if ((rc = mgSetup(b)) || (rc = mgVcycle(b))) return rc;
     for (int c = 0; c < R; ++c) {
         for (int k = 0; k < P; ++k) {
             if ((rc = launch(b, K_PCG_Q, grid))) return rc;
             if ((rc = launch(b, K_PCG_U, grid))) return rc;
+            // the V-cycle after the last iteration would only feed a search
+            // direction that is never used
+            if (k + 1 < P && (rc = mgVcycle(b))) return rc;
         }
         if (b->precision != CP_PREC_MIXED) break;
         if ((rc = launch(b, K_REF_ACC, grid))) return rc;
-        if (c + 1 < R)
+        if (c + 1 < R) {
             if ((rc = launch(b, K_REF_RES, grid))) return rc;
+            if ((rc = mgVcycle(b))) return rc;
+        }
     }
     return launch(b, K_CORR_OUTER, grid);
 }
@@ -686,7 +860,7 @@ int graphRun(cp_batch* b, int steps) {
         }
         return CP_OK;
     }
-    if (!b->graphExec || J != b->graphJ || P != b->graphP || R != b->graphR) {
+    if (!b->graphExec || J != b->graphJ || P != b->graphP || R != b->graphR || b->mgLevels != b->graphL) {
         if (b->graphExec) {
             HIP_TRY(hipGraphExecDestroy(b->graphExec));
             b->graphExec = nullptr;
@@ -706,6 +880,7 @@ int graphRun(cp_batch* b, int steps) {
         b->graphJ = J;
         b->graphP = P;
         b->graphR = R;
+        b->graphL = b->mgLevels;
     }
     for (int k = 0; k < steps; ++k) HIP_TRY(hipGraphLaunch(b->graphExec, b->stream));
     return CP_OK;
@@ -974,11 +1149,9 @@ int cp_create(const cp_params* prm, const double* cellType, cp_ctx** out) {
     if (!out || !cellType) return setError(nullptr, CP_E_ARG, "null argument");
     cp_batch* b = nullptr;
     // arena sized for exactly this geometry
-    const size_t pitch = roundUp(size_t(prm->N) + 1, 32);
-    const size_t tE = prm->precision == CP_PREC_MIXED ? 4 : 8;
-    const int presArrays = prm->precision == CP_PREC_MIXED ? P_COUNT : P_COUNT - 1;
-    const size_t per = kStateArrays * 8 + (2 * M_COUNT + presArrays) * tE;
-    const size_t budget = roundUp(size_t(prm->M + 1) * pitch * per + 256 * 64, 1 << 20);
+    const size_t budget = roundUp(slotMainBytes(prm->precision, prm->M, prm->N) +
+                                      slotMgBytes(prm->precision, prm->M, prm->N) + 256 * 64,
+                                  1 << 20);
     rc = batchCreate(prm->precision, budget, 1, &b, nullptr);
     if (rc) return rc;
     b->tileRows = defaultTileRows(prm->M, prm->N);

@@ -463,13 +463,14 @@ __device__ __forceinline__ bool isPDof(int i, int j, int M, int N) {
 // fly for the node and its four neighbours), q = A p_k, partial p.q
 // (pressure_solver.cpp:153-159, 173-181)
 // ---------------------------------------------------------------------------
-// p_k at (i, j): z + beta p_{k-1} on p' unknowns, 0 elsewhere.
+// p_k at (i, j): z + beta p_{k-1} on p' unknowns, 0 elsewhere. z is the
+// multigrid result when Z is given, else the Jacobi z = r / aP.
 template <typename T>
-__device__ __forceinline__ T pAt(const T* r, const T* aP, const T* pold, int i, int j, int M, int N, int P, int kk,
-                                 T beta) {
+__device__ __forceinline__ T pAt(const T* r, const T* aP, const T* Z, const T* pold, int i, int j, int M, int N,
+                                 int P, int kk, T beta) {
     if (!isPDof(i, j, M, N)) return T(0);
     const size_t k = size_t(i) * P + j;
-    const T z = r[k] / aP[k];
+    const T z = Z ? Z[k] : r[k] / aP[k];
     return kk == 0 ? z : z + beta * pold[k];
 }
 
@@ -497,12 +498,13 @@ __global__ __launch_bounds__(kBlock) void k_pcg_q(KArgs a) {
         const T* cE = static_cast<const T*>(d.pr[P_CE]);
         const T* cN = static_cast<const T*>(d.pr[P_CN]);
         const T* pold = static_cast<const T*>(d.pr[P_PK0 + (kk & 1)]);
+        const T* Z = d.precond ? static_cast<const T*>(d.pr[P_Z]) : nullptr;
         T* pnew = static_cast<T*>(d.pr[P_PK0 + ((kk & 1) ^ 1)]);
         T* Q = static_cast<T*>(d.pr[P_Q]);
         __syncthreads();  // the previous tile's LDS reads are done
         for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
             const int hi = idx / HW, hj = idx - hi * HW;
-            sp[hi][hj] = pAt(r, aP, pold, tl.i0 - 1 + hi, tl.j0 - 1 + hj, M, N, P, kk, beta);
+            sp[hi][hj] = pAt(r, aP, Z, pold, tl.i0 - 1 + hi, tl.j0 - 1 + hj, M, N, P, kk, beta);
         }
         for (int idx = int(threadIdx.x); idx < TH * (kTileW + 1); idx += kBlock) {
             const int ci = idx / (kTileW + 1), cj = idx - ci * (kTileW + 1);
@@ -624,6 +626,307 @@ __global__ __launch_bounds__(kBlock) void k_refine_resid(KArgs a) {
             v[1] += double(r) * double(r / aP[k]);
         }
         blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Aggregation-multigrid preconditioner (pressurePrecond = 1). One V-cycle:
+// at each level two damped-Jacobi sweeps from zero, the residual summed over
+// 2x2 aggregates into the next level's right-hand side; the coarsest level
+// solved with a dense inverse; then, going up, the coarse correction (scaled)
+// added to the pre-smoothed iterate and two more damped-Jacobi sweeps. Each
+// level is one launch down and one up: a tile stages its 64 x 4R unknowns
+// plus a two-deep halo in LDS and recomputes the halo's sweeps, so the two
+// sweeps and the residual need no grid-wide synchronisation.
+// ---------------------------------------------------------------------------
+struct MgTile {
+    int s;       // slot
+    int I0, J0;  // first unknown of the tile
+};
+
+template <int R>
+__device__ __forceinline__ MgTile decodeMgTile(const KArgs& a, int t) {
+    MgTile tl;
+    tl.s = a.tileSlot[t];
+    const MgLevel& L = a.desc[tl.s].lv[a.level];
+    const int lt = t - L.tile0;
+    const int ty = lt / L.tilesX;
+    const int tx = lt - ty * L.tilesX;
+    tl.I0 = ty * (kTileH * R);
+    tl.J0 = tx * kTileW;
+    return tl;
+}
+
+// multigrid work of a slot in the current pressure solve
+__device__ __forceinline__ bool mgActive(const SlotDesc& d, const SlotState& st) {
+    return d.precond && st.status == ST_RUNNING && !st.pdone && !st.stepPhase;
+}
+
+__device__ __forceinline__ size_t mgAt(const MgLevel& L, int I, int J) {
+    return size_t(I + 1) * L.pitch + (J + 1);
+}
+
+// A x at LDS position (h, w) of a halo tile, terms in the CSR column order
+// S, W, P, E, N (absent neighbours hold zero coefficients and values).
+template <typename T, int HH, int HW>
+__device__ __forceinline__ T mgAx(const T (&aP)[HH][HW], const T (&cE)[HH][HW], const T (&cN)[HH][HW],
+                                  const T (&x)[HH][HW], int h, int w) {
+    T s = T(0);
+    s += -cN[h - 1][w] * x[h - 1][w];
+    s += -cE[h][w - 1] * x[h][w - 1];
+    s += aP[h][w] * x[h][w];
+    s += -cE[h][w] * x[h][w + 1];
+    s += -cN[h][w] * x[h + 1][w];
+    return s;
+}
+
+// Galerkin coarse operator of level a.level (>= 1) from level a.level - 1:
+// aP_c = sum of the aggregate's aP minus twice its internal couplings,
+// cE_c / cN_c = sums of the couplings crossing to the east / north aggregate
+// (only couplings between unknowns count).
+template <typename T, int R>
+__global__ __launch_bounds__(kBlock) void k_mg_coarsen(KArgs a) {
+    const int nT = *a.nTiles;
+    XCD_TILE_LOOP(t, nT) {
+        const MgTile tl = decodeMgTile<R>(a, t);
+        const SlotDesc& d = a.desc[tl.s];
+        const SlotState& st = a.state[tl.s];
+        if (!mgActive(d, st) || a.level >= d.nLevels) continue;
+        const MgLevel& C = d.lv[a.level];
+        const MgLevel& F = d.lv[a.level - 1];
+        const T* faP = static_cast<const T*>(F.aP);
+        const T* fcE = static_cast<const T*>(F.cE);
+        const T* fcN = static_cast<const T*>(F.cN);
+        auto in = [&](int I, int J) { return I >= 0 && I < F.H && J >= 0 && J < F.W; };
+        auto g = [&](int I, int J) { return in(I, J) ? double(faP[mgAt(F, I, J)]) : 0.0; };
+        auto eE = [&](int I, int J) { return (in(I, J) && J + 1 < F.W) ? double(fcE[mgAt(F, I, J)]) : 0.0; };
+        auto eN = [&](int I, int J) { return (in(I, J) && I + 1 < F.H) ? double(fcN[mgAt(F, I, J)]) : 0.0; };
+        const int lane = int(threadIdx.x & 63), w = int(threadIdx.x >> 6);
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+            const int I = tl.I0 + w * R + rr, J = tl.J0 + lane;
+            if (I >= C.H || J >= C.W) continue;
+            const int i0 = 2 * I, j0 = 2 * J;
+            double acc = 0.0;
+            acc += g(i0, j0);
+            acc += g(i0, j0 + 1);
+            acc += g(i0 + 1, j0);
+            acc += g(i0 + 1, j0 + 1);
+            double internal = 0.0;
+            internal += eE(i0, j0);
+            internal += eE(i0 + 1, j0);
+            internal += eN(i0, j0);
+            internal += eN(i0, j0 + 1);
+            const size_t k = mgAt(C, I, J);
+            static_cast<T*>(C.aP)[k] = T(acc - 2.0 * internal);
+            static_cast<T*>(C.cE)[k] = T(eE(i0, j0 + 1) + eE(i0 + 1, j0 + 1));
+            static_cast<T*>(C.cN)[k] = T(eN(i0 + 1, j0) + eN(i0 + 1, j0 + 1));
+        }
+    }
+}
+
+// Dense inverse of the coarsest operator (n = H W <= kCoarseMax unknowns) by
+// Gauss-Jordan elimination in double, one workgroup per slot.
+__global__ __launch_bounds__(kBlock) void k_mg_factor(KArgs a) {
+    const int s = blockIdx.x;
+    const SlotDesc& d = a.desc[s];
+    const SlotState& st = a.state[s];
+    if (d.ntiles == 0 || !mgActive(d, st)) return;
+    __shared__ double A[kCoarseMax][kCoarseMax + 1];
+    __shared__ double f[kCoarseMax];
+    __shared__ double ipS;
+    const MgLevel& L = d.lv[d.nLevels - 1];
+    const int W = L.W, n = L.H * L.W;
+    const bool f32 = d.mixed != 0;
+    auto ld = [&](void* p, int r) {
+        const size_t k = mgAt(L, r / W, r % W);
+        return f32 ? double(static_cast<const float*>(p)[k]) : static_cast<const double*>(p)[k];
+    };
+    for (int idx = int(threadIdx.x); idx < n * n; idx += kBlock) {
+        const int r = idx / n, c = idx - r * n;
+        double v = 0.0;
+        if (c == r) v = ld(L.aP, r);
+        else if (c == r + 1 && (r % W) + 1 < W) v = -ld(L.cE, r);
+        else if (c + 1 == r && (c % W) + 1 < W) v = -ld(L.cE, c);
+        else if (c == r + W) v = -ld(L.cN, r);
+        else if (c + W == r) v = -ld(L.cN, c);
+        A[r][c] = v;
+    }
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        if (threadIdx.x == 0) {
+            double piv = A[k][k];
+            if (!(piv > 0.0)) piv = 1.0;
+            ipS = 1.0 / piv;
+        }
+        __syncthreads();
+        const double ip = ipS;
+        const int t = int(threadIdx.x);
+        if (t < n) {
+            f[t] = (t == k) ? 0.0 : A[t][k];
+            A[k][t] = ((t == k) ? 1.0 : A[k][t]) * ip;
+        }
+        __syncthreads();
+        for (int idx = t; idx < n * n; idx += kBlock) {
+            const int i = idx / n, j = idx - i * n;
+            if (i != k) A[i][j] = ((j == k) ? 0.0 : A[i][j]) - f[i] * A[k][j];
+        }
+        __syncthreads();
+    }
+    for (int idx = int(threadIdx.x); idx < n * n; idx += kBlock) d.coarseInv[idx] = A[idx / n][idx % n];
+}
+
+// Down-sweep at level a.level (not the slot's coarsest): x1 = w b/aP,
+// x2 = x1 + w (b - A x1)/aP, r = b - A x2; stores x2 and the aggregate sums
+// of r as the next level's right-hand side.
+template <typename T, int R>
+__global__ __launch_bounds__(kBlock) void k_mg_down(KArgs a) {
+    constexpr int TH = kTileH * R, HH = TH + 4, HW = kTileW + 4;
+    __shared__ T sb[HH][HW], saP[HH][HW], scE[HH][HW], scN[HH][HW], sx1[HH][HW], sx2[HH][HW];
+    const int nT = *a.nTiles;
+    XCD_TILE_LOOP(t, nT) {
+        const MgTile tl = decodeMgTile<R>(a, t);
+        const SlotDesc& d = a.desc[tl.s];
+        const SlotState& st = a.state[tl.s];
+        if (!mgActive(d, st) || a.level >= d.nLevels - 1) continue;
+        const MgLevel& L = d.lv[a.level];
+        const MgLevel& C = d.lv[a.level + 1];
+        const T om = T(d.mgOmega);
+        const T* b = static_cast<const T*>(L.b);
+        const T* aP = static_cast<const T*>(L.aP);
+        const T* cE = static_cast<const T*>(L.cE);
+        const T* cN = static_cast<const T*>(L.cN);
+        __syncthreads();  // the previous tile's LDS reads are done
+        for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
+            const int h = idx / HW, w = idx - h * HW;
+            const int I = tl.I0 - 2 + h, J = tl.J0 - 2 + w;
+            const bool in = I >= 0 && I < L.H && J >= 0 && J < L.W;
+            const size_t k = in ? mgAt(L, I, J) : 0;
+            const T bv = in ? b[k] : T(0), av = in ? aP[k] : T(0);
+            sb[h][w] = bv;
+            saP[h][w] = av;
+            scE[h][w] = in ? cE[k] : T(0);
+            scN[h][w] = in ? cN[k] : T(0);
+            sx1[h][w] = in ? om * (bv / av) : T(0);
+        }
+        __syncthreads();
+        for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
+            const int h = idx / HW, w = idx - h * HW;
+            const int I = tl.I0 - 2 + h, J = tl.J0 - 2 + w;
+            T v = T(0);
+            if (h >= 1 && h < HH - 1 && w >= 1 && w < HW - 1 && I >= 0 && I < L.H && J >= 0 && J < L.W)
+                v = sx1[h][w] + om * ((sb[h][w] - mgAx(saP, scE, scN, sx1, h, w)) / saP[h][w]);
+            sx2[h][w] = v;
+        }
+        __syncthreads();
+        const int lane = int(threadIdx.x & 63), wv = int(threadIdx.x >> 6);
+        T* xo = static_cast<T*>(L.x);
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+            const int row = wv * R + rr, h = row + 2, w = lane + 2;
+            const int I = tl.I0 + row, J = tl.J0 + lane;
+            T r = T(0);
+            if (I < L.H && J < L.W) {
+                r = sb[h][w] - mgAx(saP, scE, scN, sx2, h, w);
+                xo[mgAt(L, I, J)] = sx2[h][w];
+            }
+            sx1[h][w] = r;  // x1 is no longer read: reuse for the residual
+        }
+        __syncthreads();
+        T* cb = static_cast<T*>(C.b);
+        for (int idx = int(threadIdx.x); idx < (TH / 2) * (kTileW / 2); idx += kBlock) {
+            const int ci = idx / (kTileW / 2), cj = idx - ci * (kTileW / 2);
+            const int Ic = tl.I0 / 2 + ci, Jc = tl.J0 / 2 + cj;
+            if (Ic >= C.H || Jc >= C.W) continue;
+            const int h = 2 + 2 * ci, w = 2 + 2 * cj;
+            T acc = T(0);
+            acc += sx1[h][w];
+            acc += sx1[h][w + 1];
+            acc += sx1[h + 1][w];
+            acc += sx1[h + 1][w + 1];
+            cb[mgAt(C, Ic, Jc)] = acc;
+        }
+    }
+}
+
+// Coarsest level: z = A^-1 b with the dense inverse, one workgroup per slot.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_coarsest(KArgs a) {
+    const int s = blockIdx.x;
+    const SlotDesc& d = a.desc[s];
+    const SlotState& st = a.state[s];
+    if (d.ntiles == 0 || !mgActive(d, st)) return;
+    __shared__ double sb[kCoarseMax];
+    const MgLevel& L = d.lv[d.nLevels - 1];
+    const int W = L.W, n = L.H * L.W;
+    const int t = int(threadIdx.x);
+    if (t < n) sb[t] = double(static_cast<const T*>(L.b)[mgAt(L, t / W, t % W)]);
+    __syncthreads();
+    if (t < n) {
+        double acc = 0.0;
+        for (int k = 0; k < n; ++k) acc += d.coarseInv[size_t(t) * n + k] * sb[k];
+        static_cast<T*>(L.z)[mgAt(L, t / W, t % W)] = T(acc);
+    }
+}
+
+// Up-sweep at level a.level (not the slot's coarsest): x3 = x2 + s z_c,
+// x4 = x3 + w (b - A x3)/aP, z = x4 + w (b - A x4)/aP. On the finest level
+// the tile also reduces r.z (b is the PCG residual there).
+template <typename T, int R>
+__global__ __launch_bounds__(kBlock) void k_mg_up(KArgs a) {
+    constexpr int TH = kTileH * R, HH = TH + 4, HW = kTileW + 4;
+    __shared__ T sb[HH][HW], saP[HH][HW], scE[HH][HW], scN[HH][HW], sx3[HH][HW], sx4[HH][HW];
+    const int nT = *a.nTiles;
+    XCD_TILE_LOOP(t, nT) {
+        const MgTile tl = decodeMgTile<R>(a, t);
+        const SlotDesc& d = a.desc[tl.s];
+        const SlotState& st = a.state[tl.s];
+        if (!mgActive(d, st) || a.level >= d.nLevels - 1) continue;
+        const MgLevel& L = d.lv[a.level];
+        const MgLevel& C = d.lv[a.level + 1];
+        const T om = T(d.mgOmega), sc = T(d.mgScale);
+        const T* b = static_cast<const T*>(L.b);
+        const T* aP = static_cast<const T*>(L.aP);
+        const T* cE = static_cast<const T*>(L.cE);
+        const T* cN = static_cast<const T*>(L.cN);
+        const T* x2 = static_cast<const T*>(L.x);
+        const T* zc = static_cast<const T*>(C.z);
+        __syncthreads();
+        for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
+            const int h = idx / HW, w = idx - h * HW;
+            const int I = tl.I0 - 2 + h, J = tl.J0 - 2 + w;
+            const bool in = I >= 0 && I < L.H && J >= 0 && J < L.W;
+            const size_t k = in ? mgAt(L, I, J) : 0;
+            sb[h][w] = in ? b[k] : T(0);
+            saP[h][w] = in ? aP[k] : T(0);
+            scE[h][w] = in ? cE[k] : T(0);
+            scN[h][w] = in ? cN[k] : T(0);
+            sx3[h][w] = in ? x2[k] + sc * zc[mgAt(C, I >> 1, J >> 1)] : T(0);
+        }
+        __syncthreads();
+        for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
+            const int h = idx / HW, w = idx - h * HW;
+            const int I = tl.I0 - 2 + h, J = tl.J0 - 2 + w;
+            T v = T(0);
+            if (h >= 1 && h < HH - 1 && w >= 1 && w < HW - 1 && I >= 0 && I < L.H && J >= 0 && J < L.W)
+                v = sx3[h][w] + om * ((sb[h][w] - mgAx(saP, scE, scN, sx3, h, w)) / saP[h][w]);
+            sx4[h][w] = v;
+        }
+        __syncthreads();
+        const int lane = int(threadIdx.x & 63), wv = int(threadIdx.x >> 6);
+        T* zo = static_cast<T*>(L.z);
+        double v[1] = {0.0};
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+            const int row = wv * R + rr, h = row + 2, w = lane + 2;
+            const int I = tl.I0 + row, J = tl.J0 + lane;
+            if (I < L.H && J < L.W) {
+                const T z = sx4[h][w] + om * ((sb[h][w] - mgAx(saP, scE, scN, sx4, h, w)) / saP[h][w]);
+                zo[mgAt(L, I, J)] = z;
+                v[0] += double(sb[h][w]) * double(z);
+            }
+        }
+        if (a.level == 0) blockReduceTo<1, 0u>(v, a.partials + size_t(t) * kNumPartials);
     }
 }
 
@@ -844,6 +1147,9 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
         case PH_JAC:
             if (st.jdone[0] && st.jdone[1]) return;
             break;
+        case PH_MG_Z:
+            if (!d.precond) return;
+            [[fallthrough]];
         case PH_PCG_Q:
         case PH_PCG_U:
             if (st.pdone || st.stepPhase) return;
@@ -926,8 +1232,10 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
                     st.pdone = 1;
                     st.piters = st.pk;
                 } else {
-                    st.beta = tot[1] / st.rz;
-                    st.rz = tot[1];
+                    if (!d.precond) {  // multigrid: beta follows the V-cycle (PH_MG_Z)
+                        st.beta = tot[1] / st.rz;
+                        st.rz = tot[1];
+                    }
                     st.pk += 1;
                     if (st.pk >= d.maxPresIter) {
                         st.pdone = 1;
@@ -961,6 +1269,21 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
             }
             break;
         }
+        case PH_MG_Z: {
+            // r.z of the V-cycle just applied: the start of a (re)started CG
+            // (pk == 0) or the next search direction's beta
+            double tot[1];
+            slotReduce<1, 0u>(a.partials, d.lv[0].tile0, d.lv[0].ntiles, tot);
+            if (threadIdx.x == 0) {
+                if (st.pk == 0) {
+                    st.beta = 0.0;
+                } else {
+                    st.beta = tot[0] / st.rz;
+                }
+                st.rz = tot[0];
+            }
+            break;
+        }
         case PH_CORR: {
             double tot[1];
             slotReduce<1, 1u>(a.partials, d.tile0, d.ntiles, tot);
@@ -979,6 +1302,13 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
 // ---------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------
+// Multigrid halo tiles hold six (4R+4) x 68 arrays in LDS; the host uses
+// R <= 4 (float) or R <= 2 (double) for them, which keeps them under 64 KiB.
+template <typename T, int R>
+constexpr bool mgTileFits() {
+    return size_t(6) * (kTileH * R + 4) * (kTileW + 4) * sizeof(T) <= 65536 && R <= (sizeof(T) == 4 ? 4 : 2);
+}
+
 template <typename T, int R>
 static void launchTR(Kern k, int grid, hipStream_t s, const KArgs& a) {
     const dim3 g(grid), b(kBlock);
@@ -993,6 +1323,15 @@ static void launchTR(Kern k, int grid, hipStream_t s, const KArgs& a) {
         case K_REF_RES: hipLaunchKernelGGL((k_refine_resid<T, R>), g, b, 0, s, a); break;
         case K_CORR:
         case K_CORR_OUTER: hipLaunchKernelGGL((k_correct<R>), g, b, 0, s, a); break;
+        case K_MG_COARSEN: hipLaunchKernelGGL((k_mg_coarsen<T, R>), g, b, 0, s, a); break;
+        case K_MG_DOWN:
+            if constexpr (mgTileFits<T, R>()) hipLaunchKernelGGL((k_mg_down<T, R>), g, b, 0, s, a);
+            break;
+        case K_MG_UP:
+            if constexpr (mgTileFits<T, R>()) hipLaunchKernelGGL((k_mg_up<T, R>), g, b, 0, s, a);
+            break;
+        case K_MG_FACTOR: hipLaunchKernelGGL(k_mg_factor, g, b, 0, s, a); break;
+        case K_MG_COARSEST: hipLaunchKernelGGL((k_mg_coarsest<T>), g, b, 0, s, a); break;
     }
 }
 
@@ -1018,6 +1357,7 @@ static int phaseOf(Kern k) {
         case K_REF_RES: return PH_REF_RES;
         case K_CORR:
         case K_CORR_OUTER: return PH_CORR;
+        case K_MG_UP: return PH_MG_Z;  // the finest level's up-sweep only
         default: return -1;
     }
 }
@@ -1030,7 +1370,7 @@ hipError_t launchKernel(Kern k, bool f32, int rows, int grid, int nslots, hipStr
     if (f32) launchT<float>(k, rows, grid, s, a);
     else launchT<double>(k, rows, grid, s, a);
     if (evAfter) (void)hipEventRecordWithFlags(evAfter, s, hipEventRecordExternal);
-    const int ph = phaseOf(k);
+    const int ph = (k == K_MG_UP && a.level != 0) ? -1 : phaseOf(k);
     if (ph >= 0)
         hipLaunchKernelGGL(k_finalize, dim3(nslots), dim3(kBlock), 0, s, a, ph, k == K_CORR_OUTER ? 1 : 0);
     return hipGetLastError();

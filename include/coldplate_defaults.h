@@ -36,6 +36,9 @@ static inline void cp_fill_reference_defaults(cp_params* p) {
     p->refineCycles = 2;
     p->fixedMomentumSweeps = 0;
     p->massResidualScope = 0;
+    p->pressurePrecond = 0;
+    p->mgOmega = 0.8;
+    p->mgScale = 1.8;
 }
 
 #endif /* COLDPLATE_DEFAULTS_H */

@@ -47,6 +47,10 @@ extern "C" {
 #define CP_PREC_MIXED  1   /* FP32 stencil/Jacobi/PCG + FP64 refinement,
                               FP64 reductions and convergence checks */
 
+/* Pressure-correction preconditioners (cp_params.pressurePrecond). */
+#define CP_PRECOND_JACOBI 0  /* point Jacobi, the reference */
+#define CP_PRECOND_AMG    1  /* aggregation-multigrid V-cycle */
+
 /* Fields for cp_get_field / cp_set_field. */
 #define CP_F_U      0
 #define CP_F_V      1
@@ -112,6 +116,17 @@ typedef struct cp_params {
        never corrected and floors residMass (1.655e-6 kg/s/m on the 200x50
        straight channel); scope 1 lets the stop test see convergence. */
     int32_t massResidualScope;
+    /* Preconditioner of the pressure-correction CG: 0 = point Jacobi, the
+       reference (pressure_solver.cpp:121-184); 1 = one V-cycle of
+       aggregation multigrid: 2x2 piecewise-constant aggregates with Galerkin
+       coarse operators (again 5-point), two damped-Jacobi sweeps (weight
+       mgOmega) before and after the coarse correction, the correction
+       scaled by mgScale (unsmoothed aggregation under-corrects), and a dense
+       solve on the coarsest level (<= 64 unknowns). The V-cycle is a
+       symmetric positive definite operator, so CG and its stop tests are
+       unchanged. */
+    int32_t pressurePrecond;
+    double  mgOmega, mgScale;
 } cp_params;
 
 /* Statistics of one outer iteration (one pass of SIMPLE.cpp:299-439). */

@@ -237,4 +237,40 @@ int or_pcg_f64(int n, const int* row_ptr, const int* col_idx, const double* valu
     return pcgWrap<double>(n, row_ptr, col_idx, values, b, x, tol, maxIter);
 }
 
+// Aggregation multigrid on a caller grid operator (H x W unknowns, row-major
+// aP / cE / cN as AggMG): z = V-cycle(r). Returns the number of levels.
+int or_mg_apply(int H, int W, const double* aP, const double* cE, const double* cN, double omega, double scale,
+                const double* r, double* z) {
+    oracle::AggMG<double> mg;
+    mg.omega = omega;
+    mg.scale = scale;
+    mg.buildGrid(H, W, aP, cE, cN);
+    mg.apply(r, z);
+    return int(mg.lv.size());
+}
+int or_mg_apply_f32(int H, int W, const float* aP, const float* cE, const float* cN, float omega, float scale,
+                    const float* r, float* z) {
+    oracle::AggMG<float> mg;
+    mg.omega = omega;
+    mg.scale = scale;
+    mg.buildGrid(H, W, aP, cE, cN);
+    mg.apply(r, z);
+    return int(mg.lv.size());
+}
+// Level `level` of the hierarchy: its shape (always) and, when the output
+// arrays are non-null, its aP / cE / cN.
+int or_mg_level(int H, int W, const double* aP, const double* cE, const double* cN, int level, int* Hl, int* Wl,
+                double* aPl, double* cEl, double* cNl) {
+    oracle::AggMG<double> mg;
+    mg.buildGrid(H, W, aP, cE, cN);
+    if (level < 0 || level >= int(mg.lv.size())) return -1;
+    const auto& L = mg.lv[level];
+    *Hl = L.H;
+    *Wl = L.W;
+    if (aPl) std::copy(L.aP.begin(), L.aP.end(), aPl);
+    if (cEl) std::copy(L.cE.begin(), L.cE.end(), cEl);
+    if (cNl) std::copy(L.cN.begin(), L.cN.end(), cNl);
+    return int(mg.lv.size());
+}
+
 }  // extern "C"

@@ -57,6 +57,9 @@ def load() -> C.CDLL:
             "or_jacobi_f64": (C.c_int, [C.c_int, _ip, _ip, _dp, _dp, _dp, _ip, C.c_double, C.c_int]),
             "or_pcg_f32": (C.c_int, [C.c_int, _ip, _ip, _fp, _fp, _fp, C.c_float, C.c_int]),
             "or_pcg_f64": (C.c_int, [C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_double, C.c_int]),
+            "or_mg_apply": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp]),
+            "or_mg_apply_f32": (C.c_int, [C.c_int, C.c_int, _fp, _fp, _fp, C.c_float, C.c_float, _fp, _fp]),
+            "or_mg_level": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, _ip, _ip, _dp, _dp, _dp]),
         }
         for k, (r, a) in sig.items():
             f = getattr(lib, k)
@@ -222,3 +225,29 @@ def pcg(row_ptr, col_idx, values, b, x0, tol, max_iter, fp32=True, impl="oracle"
         it = load().or_pcg_f64(n, _ptr(rp, _ip), _ptr(ci, _ip), _ptr(va, _dp), _ptr(bb, _dp), _ptr(x, _dp),
                                float(tol), int(max_iter))
     return it, x
+
+
+def mg_apply(aP, cE, cN, r, omega=0.8, scale=1.8, fp32=False):
+    """One aggregation-multigrid V-cycle of the oracle on an H x W grid
+    operator; returns (z, number of levels)."""
+    H, W = aP.shape
+    ty, pt = (np.float32, _fp) if fp32 else (np.float64, _dp)
+    a = [np.ascontiguousarray(x, ty) for x in (aP, cE, cN, r)]
+    z = np.zeros((H, W), ty)
+    fn = load().or_mg_apply_f32 if fp32 else load().or_mg_apply
+    nl = fn(H, W, *(_ptr(x, pt) for x in a[:3]), omega, scale, _ptr(a[3], pt), _ptr(z, pt))
+    return z, nl
+
+
+def mg_level(aP, cE, cN, level):
+    """Coarse operator (aP, cE, cN) of hierarchy level `level`."""
+    H, W = aP.shape
+    a = [np.ascontiguousarray(x, np.float64) for x in (aP, cE, cN)]
+    hl, wl = C.c_int(0), C.c_int(0)
+    n = load().or_mg_level(H, W, *(_ptr(x, _dp) for x in a), level, C.byref(hl), C.byref(wl), None, None, None)
+    if n < 0:
+        raise ValueError("no such level")
+    out = [np.zeros((hl.value, wl.value)) for _ in range(3)]
+    load().or_mg_level(H, W, *(_ptr(x, _dp) for x in a), level, C.byref(hl), C.byref(wl),
+                       *(_ptr(x, _dp) for x in out))
+    return out[0], out[1], out[2], n

@@ -163,6 +163,223 @@ int pcg(const CSR<T>& A, const T* b, T* x, T tol, int maxIter) {
     return iter;
 }
 
+// ---------------------------------------------------------------------------
+// Aggregation-multigrid preconditioner of the pressure-correction system
+// (cp_params.pressurePrecond = 1; no counterpart in the reference, whose CPU
+// PCG is Jacobi-preconditioned and whose AMGCL backends are not restated).
+// The unknowns form the H x W grid of p' DOFs (I = i-1, J = j-1); the
+// operator is 5-point: A x = aP x - cE x_E - cE_W x_W - cN x_N - cN_S x_S,
+// cE(I,J) coupling (I,J)-(I,J+1) and cN(I,J) coupling (I,J)-(I+1,J).
+// Levels: 2x2 piecewise-constant aggregates, Galerkin coarse operator
+// A_c = P^T A P (5-point again); coarsening stops once a level has at most
+// kCoarseMax unknowns (at least one coarse level). V-cycle: two
+// damped-Jacobi sweeps from zero, restriction of the residual (sum over the
+// aggregate), coarse correction scaled by `scale`, two damped-Jacobi sweeps;
+// the coarsest level is solved with its dense inverse (Gauss-Jordan in
+// double, no pivoting: the coarse matrices are symmetric positive definite).
+// The device kernels follow the same operation order.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct AggMG {
+    static constexpr int kMaxLevels = 12;
+    static constexpr int kCoarseMax = 64;
+    struct Level {
+        int H = 0, W = 0;
+        std::vector<T> aP, cE, cN, b, x, z;
+        size_t at(int I, int J) const { return size_t(I) * W + J; }
+    };
+    std::vector<Level> lv;
+    std::vector<double> inv;  // coarsest level, n x n row-major
+    T omega = T(0.8), scale = T(1.8);
+
+    // A x at (I,J), terms in the CSR column order S, W, P, E, N.
+    static T ax(const Level& L, const T* x, int I, int J) {
+        const size_t k = L.at(I, J);
+        T s = T(0);
+        if (I > 0) s += -L.cN[k - L.W] * x[k - L.W];
+        if (J > 0) s += -L.cE[k - 1] * x[k - 1];
+        s += L.aP[k] * x[k];
+        if (J + 1 < L.W) s += -L.cE[k] * x[k + 1];
+        if (I + 1 < L.H) s += -L.cN[k] * x[k + L.W];
+        return s;
+    }
+
+    static void coarsen(const Level& f, Level& c) {
+        c.H = (f.H + 1) / 2;
+        c.W = (f.W + 1) / 2;
+        const size_t n = size_t(c.H) * c.W;
+        c.aP.assign(n, T(0)); c.cE.assign(n, T(0)); c.cN.assign(n, T(0));
+        c.b.assign(n, T(0)); c.x.assign(n, T(0)); c.z.assign(n, T(0));
+        auto in = [&](int I, int J) { return I < f.H && J < f.W; };
+        auto g = [&](const std::vector<T>& a, int I, int J) { return in(I, J) ? double(a[f.at(I, J)]) : 0.0; };
+        // coupling arrays restricted to edges whose both ends are unknowns
+        auto eE = [&](int I, int J) { return (in(I, J) && J + 1 < f.W) ? double(f.cE[f.at(I, J)]) : 0.0; };
+        auto eN = [&](int I, int J) { return (in(I, J) && I + 1 < f.H) ? double(f.cN[f.at(I, J)]) : 0.0; };
+        for (int I = 0; I < c.H; ++I)
+            for (int J = 0; J < c.W; ++J) {
+                const int i0 = 2 * I, j0 = 2 * J;
+                double acc = 0.0;
+                acc += g(f.aP, i0, j0);
+                acc += g(f.aP, i0, j0 + 1);
+                acc += g(f.aP, i0 + 1, j0);
+                acc += g(f.aP, i0 + 1, j0 + 1);
+                double internal = 0.0;
+                internal += eE(i0, j0);
+                internal += eE(i0 + 1, j0);
+                internal += eN(i0, j0);
+                internal += eN(i0, j0 + 1);
+                const size_t k = c.at(I, J);
+                c.aP[k] = T(acc - 2.0 * internal);
+                c.cE[k] = T(eE(i0, j0 + 1) + eE(i0 + 1, j0 + 1));
+                c.cN[k] = T(eN(i0 + 1, j0) + eN(i0 + 1, j0 + 1));
+            }
+    }
+
+    // Dense inverse of the coarsest operator by Gauss-Jordan elimination.
+    void factor() {
+        const Level& L = lv.back();
+        const int n = L.H * L.W;
+        std::vector<double> a(size_t(n) * n, 0.0);
+        for (int I = 0; I < L.H; ++I)
+            for (int J = 0; J < L.W; ++J) {
+                const int r = I * L.W + J;
+                a[size_t(r) * n + r] = double(L.aP[r]);
+                if (J + 1 < L.W) { a[size_t(r) * n + r + 1] = -double(L.cE[r]); a[size_t(r + 1) * n + r] = -double(L.cE[r]); }
+                if (I + 1 < L.H) { a[size_t(r) * n + r + L.W] = -double(L.cN[r]); a[size_t(r + L.W) * n + r] = -double(L.cN[r]); }
+            }
+        gaussJordan(a.data(), n);
+        inv.swap(a);
+    }
+
+    // In-place inverse; row k is scaled by 1/pivot, then eliminated from every
+    // other row (the device kernel performs the same per-element updates).
+    static void gaussJordan(double* a, int n) {
+        std::vector<double> f(n);
+        for (int k = 0; k < n; ++k) {
+            double piv = a[size_t(k) * n + k];
+            if (!(piv > 0.0)) piv = 1.0;
+            const double ip = 1.0 / piv;
+            a[size_t(k) * n + k] = 1.0;
+            for (int j = 0; j < n; ++j) a[size_t(k) * n + j] *= ip;
+            for (int i = 0; i < n; ++i) f[i] = (i == k) ? 0.0 : a[size_t(i) * n + k];
+            for (int i = 0; i < n; ++i) {
+                if (i == k) continue;
+                a[size_t(i) * n + k] = 0.0;
+                for (int j = 0; j < n; ++j) a[size_t(i) * n + j] -= f[i] * a[size_t(k) * n + j];
+            }
+        }
+    }
+
+    // Level 0 from the assembled CSR (DOF order is row-major over (I, J)).
+    void build(const CSR<T>& A, int H, int W) {
+        const size_t n = size_t(H) * W;
+        std::vector<T> aP(n, T(0)), cE(n, T(0)), cN(n, T(0));
+        for (int r = 0; r < A.n; ++r) {
+            aP[r] = A.values[A.diagOff[r]];
+            if (A.eOff[r] >= 0) cE[r] = -A.values[A.eOff[r]];
+            if (A.nOff[r] >= 0) cN[r] = -A.values[A.nOff[r]];
+        }
+        buildGrid(H, W, aP.data(), cE.data(), cN.data());
+    }
+
+    void buildGrid(int H, int W, const T* aP, const T* cE, const T* cN) {
+        lv.assign(1, Level());
+        Level& L = lv[0];
+        L.H = H;
+        L.W = W;
+        const size_t n = size_t(H) * W;
+        L.aP.assign(aP, aP + n); L.cE.assign(cE, cE + n); L.cN.assign(cN, cN + n);
+        L.b.assign(n, T(0)); L.x.assign(n, T(0)); L.z.assign(n, T(0));
+        while ((lv.size() == 1 || size_t(lv.back().H) * lv.back().W > size_t(kCoarseMax)) &&
+               int(lv.size()) < kMaxLevels) {
+            Level c;
+            coarsen(lv.back(), c);
+            lv.push_back(std::move(c));
+        }
+        factor();
+    }
+
+    // x <- x + omega (b - A x) / aP over every unknown (Jacobi: reads old x)
+    void sweep(Level& L, const std::vector<T>& xin, std::vector<T>& xout) {
+        for (int I = 0; I < L.H; ++I)
+            for (int J = 0; J < L.W; ++J) {
+                const size_t k = L.at(I, J);
+                xout[k] = xin[k] + omega * ((L.b[k] - ax(L, xin.data(), I, J)) / L.aP[k]);
+            }
+    }
+
+    void vcycle(int l) {
+        Level& L = lv[l];
+        const size_t n = size_t(L.H) * L.W;
+        if (l + 1 == int(lv.size())) {
+            for (size_t r = 0; r < n; ++r) {
+                double s = 0.0;
+                for (size_t k = 0; k < n; ++k) s += inv[r * n + k] * double(L.b[k]);
+                L.z[r] = T(s);
+            }
+            return;
+        }
+        std::vector<T> x1(n), x2(n);
+        for (size_t k = 0; k < n; ++k) x1[k] = omega * (L.b[k] / L.aP[k]);
+        sweep(L, x1, x2);
+        Level& C = lv[l + 1];
+        for (int I = 0; I < C.H; ++I)
+            for (int J = 0; J < C.W; ++J) {
+                T acc = T(0);
+                for (int q = 0; q < 4; ++q) {
+                    const int fi = 2 * I + (q >> 1), fj = 2 * J + (q & 1);
+                    if (fi < L.H && fj < L.W) acc += L.b[L.at(fi, fj)] - ax(L, x2.data(), fi, fj);
+                }
+                C.b[C.at(I, J)] = acc;
+            }
+        vcycle(l + 1);
+        for (int I = 0; I < L.H; ++I)
+            for (int J = 0; J < L.W; ++J) x1[L.at(I, J)] = x2[L.at(I, J)] + scale * C.z[C.at(I / 2, J / 2)];
+        sweep(L, x1, x2);
+        sweep(L, x2, L.z);
+    }
+
+    void apply(const T* r, T* z) {
+        Level& L = lv[0];
+        std::copy(r, r + size_t(L.H) * L.W, L.b.begin());
+        vcycle(0);
+        std::copy(L.z.begin(), L.z.end(), z);
+    }
+};
+
+// CG preconditioned by one aggregation-multigrid V-cycle; iteration, stop
+// tests and return value as pcg() above.
+template <typename T>
+int pcgMG(const CSR<T>& A, AggMG<T>& mg, const T* b, T* x, T tol, int maxIter) {
+    const int n = A.n;
+    if (n == 0) return 0;
+    std::vector<T> r(n), z(n), p(n), Ap(n);
+    spmv(A, x, Ap.data());
+    for (int i = 0; i < n; ++i) r[i] = b[i] - Ap[i];
+    mg.apply(r.data(), z.data());
+    p = z;
+    T rz = blockedSum(n, r.data(), z.data());
+    int iter = 0;
+    for (; iter < maxIter; ++iter) {
+        spmv(A, p.data(), Ap.data());
+        const T pAp = blockedSum(n, p.data(), Ap.data());
+        if (std::abs(pAp) < T(1e-20)) break;
+        const T alpha = rz / pAp;
+        for (int i = 0; i < n; ++i) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * Ap[i];
+        }
+        const T err = blockedSum(n, r.data(), r.data());
+        if (std::sqrt(err) < tol) break;
+        mg.apply(r.data(), z.data());
+        const T rzPrev = rz;
+        rz = blockedSum(n, r.data(), z.data());
+        const T beta = rz / rzPrev;
+        for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    }
+    return iter;
+}
+
 template <typename T>
 struct PlaneT {
     T flowArea = 0, massFlux = 0, avgStatic = 0, avgDynamic = 0, avgTotal = 0;
@@ -186,6 +403,7 @@ struct Simple {
     std::vector<T> u, v, p, uS, vS, pS, uOld, vOld, dE, dN, cellType, gamma, alpha;
     // persistent systems
     CSR<T> cU, cV, cP;
+    AggMG<T> mg;
     std::vector<int> linU, linV, linP;
 
     // index helpers (reference shapes, row-major)
@@ -501,7 +719,15 @@ struct Simple {
             c.rhs[r] = massImbalance(i, j);
             c.sol[r] = P(pS, i, j);
         }
-        const int it = pcg<T>(c, c.rhs.data(), c.sol.data(), pTol, prm.maxPressureIter);
+        int it;
+        if (prm.pressurePrecond == 1) {
+            mg.omega = T(prm.mgOmega);
+            mg.scale = T(prm.mgScale);
+            mg.build(c, M - 1, N - 2);
+            it = pcgMG<T>(c, mg, c.rhs.data(), c.sol.data(), pTol, prm.maxPressureIter);
+        } else {
+            it = pcg<T>(c, c.rhs.data(), c.sol.data(), pTol, prm.maxPressureIter);
+        }
         for (int r = 0; r < c.n; ++r) P(pS, c.ri[r], c.rj[r]) = c.sol[r];
         for (int i = 1; i < M; ++i) P(pS, i, N - 1) = T(0);
         return it;
