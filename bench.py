@@ -3,33 +3,36 @@
 
 Workload: the 1024 x 512 channel-network case (seeded networks of 6-10
 Manhattan/diagonal channels 8-16 cells wide, 2.5D model with Ht = 200 um,
-SOU convection, pseudo-CFL 0.5 -> 5 with SER, inlet 0.2 m/s, water), seeds
-0..B-1, synthetic geometry generated in-process.
+SOU convection, pseudo-CFL 0.5 -> 5 with SER, inlet 0.2 m/s, water),
+synthetic geometry generated in-process.
 
-One step = solving the B geometries from rest to convergence in one packed
-batch (mixed precision: FP32 stencil / Jacobi / PCG with FP64 residuals,
-refinement and reductions; one persistent launch per sub-step; the outer
-iteration replayed as a hipGraph). A solve counts as converged when the
-outer stop test max(residU, residV, residMass) <= eps holds (mass residual
-over the p' columns) AND its full-system total-pressure drop is within 1e-4
-of the FP64 oracle's converged value (tests/golden/network_1024x512_dp.json).
+One step = serving a stream of R solve requests (seeds 0..7 in turn, all
+arrived at the start) through the library's request loop (cp_serve) with
+continuous batching on B slots: mixed precision (FP32 stencil / Jacobi /
+multigrid-preconditioned CG with FP64 residuals, refinement and
+reductions), one persistent launch per sub-step, the outer iteration
+replayed as a hipGraph, finished solves evicted and replaced at once. A
+solve counts when the outer stop test max(residU, residV, residMass) <= eps
+holds (mass residual over the p' columns) AND its full-system total-pressure
+drop is within 1e-4 of the FP64 oracle's converged value
+(tests/golden/network_1024x512_dp.json).
 
 Reported beside the value:
   * the plain FP64 single-geometry path (the drop-in: reference algorithm
-    and inner budgets, polled) on seed 0 with the same stop test;
-  * p50/p99 solve latency inside the batch;
-  * roofline of the dominant kernel (k_pcg_q, the PCG stencil kernel):
-    bursts of back-to-back launches timed with HIP events on the batch
-    stream inside the timed region (the kernel recomputes its outputs from
-    unchanged inputs, so a burst leaves every solve as it was); the HBM
-    traffic per launch comes from the committed rocprofv3 byte-counter
-    profile of this workload;
+    and inner budgets, Jacobi-preconditioned CG, polled) on seed 0 with the
+    same stop test;
+  * p50/p99 latency (admission to completion, and arrival to completion);
+  * roofline of the PCG stencil kernel k_pcg_q: bursts of back-to-back
+    launches timed with HIP events on the batch stream inside the timed
+    region (the kernel recomputes its outputs from unchanged inputs, so a
+    burst leaves every solve as it was); the HBM traffic per launch comes
+    from the committed rocprofv3 byte-counter profile of this workload;
   * the CPU baseline: the oracle in float arithmetic (the reference's
-    arithmetic) on the box's host cores for a bounded number of outer
-    iterations, scaled to solves/s by the outer-iteration count the same
-    algorithm needs on the GPU.
+    arithmetic and algorithm) on the box's host cores for a bounded number
+    of outer iterations, scaled to solves/s by the outer-iteration count the
+    reference algorithm needs on the GPU.
 
-    python bench.py [--steps K] [--warmup W] [--batch B]
+    python bench.py [--steps K] [--warmup W] [--batch B] [--requests R]
 """
 from __future__ import annotations
 
@@ -52,8 +55,9 @@ METRIC = "converged solves/s + p50/p99 latency at Δp err≤1e-4; PCG HBM GB/s %
 PEAK_HBM_GBS = 8000.0
 DP_TOL = 1e-4
 PCG_Q_BYTES_FP32 = 28  # per p' unknown: read r, aP, p_old, cE, cN; write p_new, q (4 B each)
-PROBE_EVERY = 10       # polls between timing bursts
-PROBE_LAUNCHES = 16    # k_pcg_q launches per burst
+PROBE_LAUNCHES = 16    # k_pcg_q launches per burst (cp_serve)
+GAMMA_MIN = 1.0 - 1e-6  # reference plane row rule (binary geometry)
+GEOMS: list = []
 TRAFFIC = REPO / "profiles" / "k_pcg_q_network_1024x512_b8.json"
 
 
@@ -66,12 +70,15 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=8, help="slots of the request loop")
+    ap.add_argument("--requests", type=int, default=24, help="solve requests per step")
     ap.add_argument("--eps", type=float, default=1e-7)
-    ap.add_argument("--jacobi", type=int, default=4, help="Jacobi sweep cap per momentum solve")
-    ap.add_argument("--pcg", type=int, default=20, help="PCG iterations per refinement cycle")
-    ap.add_argument("--cycles", type=int, default=2, help="FP64 refinement cycles of p'")
-    ap.add_argument("--check", type=int, default=20, help="outer iterations between host polls")
+    ap.add_argument("--jacobi", type=int, default=4, help="Jacobi sweeps per momentum solve")
+    ap.add_argument("--pcg", type=int, default=4, help="CG iterations per refinement cycle")
+    ap.add_argument("--cycles", type=int, default=1, help="FP64 refinement cycles of p'")
+    ap.add_argument("--precond", choices=["amg", "jacobi"], default="amg",
+                    help="pressure-CG preconditioner of the served form")
+    ap.add_argument("--check", type=int, default=10, help="outer iterations per graph launch between polls")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dropin", action="store_true")
@@ -102,50 +109,25 @@ def n_pressure(M: int, N: int) -> int:
     return (M - 1) * (N - 2)
 
 
-def solve_batch(b: cb.Batch, geoms, a, probe: bool):
-    """Admit every geometry, run to convergence, return per-slot results."""
-    slots = []
-    for p, ct in geoms:
+def solve_stream(b: cb.Batch, a, probe: bool):
+    """Serve one stream of requests; returns per-request results."""
+    reqs = []
+    for k in range(a.requests):
+        p, ct = GEOMS[k % len(GEOMS)]
         q = wl.fast_mode(p, a.jacobi, a.pcg, a.cycles)
         q.epsilon = a.eps
         q.pTol = 1e-12
-        s = b.admit(q, ct)
-        assert s is not None
-        slots.append(s)
-    npres = {s: n_pressure(geoms[k][0].M, geoms[k][0].N) for k, s in enumerate(slots)}
-    done_at: dict[int, float] = {}
-    iters: dict[int, int] = {}
-    samples = []  # (ms per launch, launches, p' unknowns of the running slots)
+        q.pressurePrecond = cb.CP_PRECOND_AMG if a.precond == "amg" else cb.CP_PRECOND_JACOBI
+        if a.profile_steps:
+            q.maxIterations = a.profile_steps
+        reqs.append((q, ct, 0.0, GAMMA_MIN))
+    b.probe(cb.K_PCG_Q if probe else -1)
     t0 = time.perf_counter()
-    steps = 0
-    polls = 0
-    running = set(slots)
-    while running:
-        b.run(a.check)
-        steps += a.check
-        info = b.poll()
-        polls += 1
-        t = time.perf_counter() - t0
-        for s in list(running):
-            if info[s]["status"] != cb.SLOT_RUNNING:
-                running.discard(s)
-                done_at[s] = t
-                iters[s] = info[s]["iter"]
-        if probe and running and polls % PROBE_EVERY == 1:
-            # HIP-event timing burst of k_pcg_q on the batch stream, inside
-            # the timed region, over the slots still running
-            ms = b.probe_burst(PROBE_LAUNCHES)
-            samples.append((ms, PROBE_LAUNCHES, sum(npres[s] for s in running)))
-        if steps > 200000 or (a.profile_steps and steps >= a.profile_steps):
-            break
+    res = b.serve(reqs, a.check)
     elapsed = time.perf_counter() - t0
-    dps, status = [], []
-    for s in slots:
-        dps.append(b.full_dp(s))
-        status.append(b.poll()[s]["status"])
-        b.evict(s)
-    return dict(elapsed=elapsed, latency=[done_at.get(s, elapsed) for s in slots],
-                iters=[iters.get(s, -1) for s in slots], dp=dps, status=status, samples=samples, steps=steps)
+    ms, unknowns = b.probe_read()
+    return dict(elapsed=elapsed, res=res, seeds=[k % len(GEOMS) for k in range(a.requests)], ms=ms,
+                unknowns=unknowns)
 
 
 def dropin_baseline(a, ref0: float):
@@ -232,44 +214,46 @@ def main() -> int:
     if a.calibrate_copy:
         copy_calibration(a.calibrate_copy)
     B = a.batch
-    refs = references(B)
-    geoms = [wl.network_1024x512(s) for s in range(B)]
-    # arena: the packed geometries' footprint plus a margin
+    refs = references(8)
+    GEOMS[:] = [wl.network_1024x512(s) for s in range(8)]
+    # arena: B of the largest geometry's footprint plus a margin
     probe_b = cb.Batch(cb.CP_PREC_MIXED, 1 << 20, 1)
-    need = sum(probe_b.footprint(p.M, p.N) for p, _ in geoms)
+    need = B * max(probe_b.footprint(p.M, p.N) for p, _ in GEOMS)
     probe_b.close()
     b = cb.Batch(cb.CP_PREC_MIXED, need + (64 << 20), B)
     if a.no_graph:
         b.set_graph(False)
     for w in range(a.warmup):
-        r = solve_batch(b, geoms, a, probe=False)
-        log(f"warmup {w}: {r['elapsed']:.2f}s iters {r['iters']}")
+        r = solve_stream(b, a, probe=False)
+        log(f"warmup {w}: {r['elapsed']:.2f}s iters {[x['iter'] for x in r['res']]}")
     runs = []
     t0 = time.perf_counter()
     for k in range(a.steps):
-        r = solve_batch(b, geoms, a, probe=not a.no_probe)
+        r = solve_stream(b, a, probe=not a.no_probe)
         runs.append(r)
-        log(f"step {k}: {r['elapsed']:.2f}s iters {r['iters']} max dp err "
-            f"{max(abs(d / x - 1) for d, x in zip(r['dp'], refs)):.2e}")
+        log(f"step {k}: {r['elapsed']:.2f}s iters {[x['iter'] for x in r['res']]} max dp err "
+            f"{max(abs(x['dp'] / refs[sd] - 1) for x, sd in zip(r['res'], r['seeds'])):.2e}")
     total = time.perf_counter() - t0
     good = 0
-    errs, lat = [], []
+    errs, lat_service, lat_total, iters = [], [], [], []
     for r in runs:
-        for d, x, s, l in zip(r["dp"], refs, r["status"], r["latency"]):
-            e = abs(d / x - 1)
+        for x, sd in zip(r["res"], r["seeds"]):
+            e = abs(x["dp"] / refs[sd] - 1)
             errs.append(e)
-            lat.append(l)
-            if s == cb.SLOT_CONVERGED and e <= DP_TOL:
+            lat_service.append(x["tDone"] - x["tAdmit"])
+            lat_total.append(x["tDone"])
+            iters.append(x["iter"])
+            if x["status"] == cb.SLOT_CONVERGED and e <= DP_TOL:
                 good += 1
     value = good / total
     # roofline of k_pcg_q from the timing bursts
     ms_all, bytes_all = 0.0, 0.0
     n_launch = 0
     for r in runs:
-        for ms, n, npr in r["samples"]:
-            ms_all += ms * n
-            bytes_all += n * npr * PCG_Q_BYTES_FP32
-            n_launch += n
+        for ms, un in zip(r["ms"], r["unknowns"]):
+            ms_all += ms * PROBE_LAUNCHES
+            bytes_all += PROBE_LAUNCHES * un * PCG_Q_BYTES_FP32
+            n_launch += PROBE_LAUNCHES
     avg_ms = ms_all / max(n_launch, 1)
     achieved = bytes_all / max(ms_all, 1e-30) / 1e6  # GB/s
     per_launch = bytes_all / max(n_launch, 1)
@@ -301,15 +285,22 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": "network_1024x512",
-            "geometries_per_step": B,
+            "requests_per_step": a.requests,
+            "slots": B,
             "grid": "1024x512 cells",
             "stop_test": f"max(residU,residV,residMass[p' cols]) <= {a.eps:g} and dp err <= {DP_TOL:g}",
-            "inner_budgets": {"jacobi_cap": a.jacobi, "pcg_per_cycle": a.pcg, "fp64_refine_cycles": a.cycles},
-            "precision": "mixed: fp32 stencil/jacobi/pcg, fp64 residuals/refinement/reductions",
+            "inner_budgets": {"jacobi_sweeps": a.jacobi, "cg_per_cycle": a.pcg, "fp64_refine_cycles": a.cycles},
+            "pressure_preconditioner": "aggregation-multigrid V-cycle" if a.precond == "amg" else "Jacobi",
+            "serving": "cp_serve continuous batching, all requests arrived at t=0",
+            "precision": "mixed: fp32 stencil/jacobi/cg/v-cycle, fp64 residuals/refinement/reductions",
         },
-        "latency_s": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
+        "latency_s": {"p50": round(float(np.percentile(lat_service, 50)), 3),
+                      "p99": round(float(np.percentile(lat_service, 99)), 3),
+                      "basis": "admission to completion",
+                      "from_arrival_p50": round(float(np.percentile(lat_total, 50)), 3),
+                      "from_arrival_p99": round(float(np.percentile(lat_total, 99)), 3)},
         "dp_rel_err": {"max": float(max(errs)), "median": float(np.median(errs))},
-        "outer_iterations": [x for r in runs for x in r["iters"]],
+        "outer_iterations": {"mean": round(float(np.mean(iters)), 1), "max": int(max(iters))},
         "plain_fp64_single": None if not drop else {
             "solves_per_s": round(1.0 / drop["seconds"], 4),
             "seconds_per_solve": round(drop["seconds"], 3),

@@ -16,6 +16,7 @@ PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = PKG_DIR / "lib" / "libcoldplate_hip.so"
 
 CP_OK = 0
+CP_E_ARG, CP_E_HIP, CP_E_NOMEM, CP_E_STATE, CP_E_CAPACITY, CP_E_NODEV, CP_E_DIVERGED = range(-1, -8, -1)
 CP_PREC_FP64 = 0
 CP_PREC_MIXED = 1
 CP_PRECOND_JACOBI = 0
@@ -25,8 +26,7 @@ F_U, F_V, F_P, F_USTAR, F_VSTAR, F_PSTAR, F_DE, F_DN, F_GAMMA, F_ALPHA = range(1
 
 SLOT_FREE, SLOT_RUNNING, SLOT_CONVERGED, SLOT_MAXITER, SLOT_FAILED = range(5)
 
-K_MOM_ASSEMBLE, K_JACOBI, K_MOM_FINISH, K_PRES_ASSEMBLE, K_PCG_Q, K_PCG_UPDATE = range(6)
-K_CORRECT = 8
+K_PCG_Q = 4
 
 
 class Params(C.Structure):
@@ -99,6 +99,16 @@ class PlaneMetrics(C.Structure):
     ]
 
 
+class Request(C.Structure):
+    _fields_ = [("prm", C.POINTER(Params)), ("cellType", C.POINTER(C.c_double)), ("arrival", C.c_double),
+                ("gammaMin", C.c_double)]
+
+
+class Result(C.Structure):
+    _fields_ = [("status", C.c_int32), ("iter", C.c_int32), ("slot", C.c_int32), ("rc", C.c_int32),
+                ("error", C.c_double), ("dp", C.c_double), ("tAdmit", C.c_double), ("tDone", C.c_double)]
+
+
 class SlotInfo(C.Structure):
     """cp_slot_info."""
 
@@ -161,8 +171,10 @@ EXPORTS = [
     ("cp_batch_evict", C.c_int, [_vp, C.c_int]),
     ("cp_batch_set_tiling", C.c_int, [_vp, C.c_int, C.c_int]),
     ("cp_batch_set_graph", C.c_int, [_vp, C.c_int]),
+    ("cp_batch_set_mg_tiling", C.c_int, [_vp, C.c_int]),
+    ("cp_serve", C.c_int, [_vp, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     ("cp_batch_probe", C.c_int, [_vp, C.c_int]),
-    ("cp_batch_probe_read", C.c_int, [_vp, _dp, C.c_int]),
+    ("cp_batch_probe_read", C.c_int, [_vp, _dp, _dp, C.c_int]),
     ("cp_batch_probe_burst", C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     ("cp_geom_straight", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
     ("cp_geom_network", C.c_int, [C.c_int, C.c_int, C.c_uint64, _dp]),
@@ -342,15 +354,38 @@ class Batch:
     def set_graph(self, use_graph: bool) -> None:
         check(load().cp_batch_set_graph(self.h, 1 if use_graph else 0), self.h)
 
+    def serve(self, requests, steps_per_poll: int = 20) -> list[dict]:
+        """Continuous batching over `requests` = [(params, cellType, arrival_s,
+        gamma_min), ...]; returns one result dict per request, in order."""
+        n = len(requests)
+        keep = []
+        reqs = (Request * max(n, 1))()
+        for k, (prm, ct, arrival, gmin) in enumerate(requests):
+            pr = prm.copy()
+            ca = np.ascontiguousarray(ct, dtype=np.float64)
+            keep.append((pr, ca))
+            reqs[k].prm = C.pointer(pr)
+            reqs[k].cellType = ca.ctypes.data_as(C.POINTER(C.c_double))
+            reqs[k].arrival = float(arrival)
+            reqs[k].gammaMin = float(gmin)
+        res = (Result * max(n, 1))()
+        check(load().cp_serve(self.h, C.cast(reqs, C.c_void_p), n, steps_per_poll, C.cast(res, C.c_void_p)), self.h)
+        return [{f: getattr(res[k], f) for f, _ in Result._fields_} for k in range(n)]
+
+    def set_mg_tiling(self, rows: int) -> None:
+        check(load().cp_batch_set_mg_tiling(self.h, rows), self.h)
+
     def probe(self, kernel: int) -> None:
         check(load().cp_batch_probe(self.h, kernel), self.h)
 
-    def probe_read(self, cap: int = 4096) -> np.ndarray:
-        out = np.zeros(cap)
-        n = load().cp_batch_probe_read(self.h, dptr(out), cap)
+    def probe_read(self, cap: int = 65536) -> tuple[np.ndarray, np.ndarray]:
+        """Timing bursts of the last serve: (ms per launch, p' unknowns covered)."""
+        ms = np.zeros(cap)
+        un = np.zeros(cap)
+        n = load().cp_batch_probe_read(self.h, dptr(ms), dptr(un), cap)
         if n < 0:
             check(n, self.h)
-        return out[:n].copy()
+        return ms[:n].copy(), un[:n].copy()
 
     def probe_burst(self, launches: int, kernel: int = K_PCG_Q) -> float:
         """Mean ms per launch of `launches` back-to-back k_pcg_q launches."""

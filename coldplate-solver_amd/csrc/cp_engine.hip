@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -24,6 +25,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/coldplate_defaults.h"
@@ -189,7 +191,7 @@ struct cp_batch {
     // multigrid tile lists: level l's list at dMgTileSlot + l * maxTiles
     int* dMgTileSlot = nullptr;
     int* dMgNTiles = nullptr;
-    int mgLevels = 0;  // deepest hierarchy over the admitted multigrid slots
+    int mgLevels = 0;  // deepest hierarchy of any multigrid slot admitted so far
     double* dPartials = nullptr;
     PlaneReq* dPlaneReq = nullptr;
     PlaneOut* dPlaneOut = nullptr;
@@ -203,15 +205,20 @@ struct cp_batch {
     // graph mode
     hipGraphExec_t graphExec = nullptr;
     int graphJ = -1, graphP = -1, graphR = -1, graphL = -1;
-    // measurement probe: events around every launch of one work kernel
+    // measurement: k_pcg_q timing bursts inside cp_serve (every kProbeEvery
+    // polls), as (ms per launch, p' unknowns of the running slots)
     int probeKernel = -1;
-    std::vector<hipEvent_t> probeEv;
     hipEvent_t burstEv[2] = {nullptr, nullptr};
-    int probeUsed = 0;
+    std::vector<double> probeMs, probeUnknowns;
     // tile height R (rows per wave) shared by every slot: 1, 2, 4 or 8
     int tileRows = 4;
     // workgroups per CU of the replayed graph's launches
     int wgPerCU = 8;
+    // rows per wave of the finest multigrid level's tiles (0: from tileRows)
+    int mgTileRows = 0;
+    int mgRowsEff() const {
+        return mgTileRows > 0 ? mgTileRows : mgRows(precision == CP_PREC_MIXED, tileRows);
+    }
     // replay the captured step graph (1) or launch its kernels directly (0)
     int useGraph = 1;
 
@@ -254,26 +261,19 @@ KArgs kargs(cp_batch* b) {
 }
 
 int launch(cp_batch* b, Kern k, int grid) {
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (b->probeKernel == int(k) && b->probeUsed + 2 <= int(b->probeEv.size())) {
-        e0 = b->probeEv[b->probeUsed];
-        e1 = b->probeEv[b->probeUsed + 1];
-        b->probeUsed += 2;
-    }
     const hipError_t e =
-        launchKernel(k, b->precision == CP_PREC_MIXED, b->tileRows, grid, b->maxSlots, b->stream, kargs(b), e0, e1);
+        launchKernel(k, b->precision == CP_PREC_MIXED, b->tileRows, grid, b->maxSlots, b->stream, kargs(b));
     if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
     return CP_OK;
 }
 
 // ---- multigrid launches ---------------------------------------------------
-// Workgroups of a level's launch: the full grid on the finest level, a
-// quarter per coarser level (the tile loops are grid-stride, so any grid is
-// correct; the shape stays independent of the admitted slots).
+// Workgroups of a level's launch: the full grid on every level. Coarse-level
+// tiles are short dependency chains (load, two sweeps, restriction), so
+// they need many workgroups in flight; surplus workgroups exit at once.
 int mgGrid(const cp_batch* b, int level) {
-    int g = gridFor(b, 1 << 30) >> (2 * level);
-    g = std::max(g, 64);
-    return (g + kNumXCD - 1) / kNumXCD * kNumXCD;
+    (void)level;
+    return gridFor(b, 1 << 30);
 }
 
 int launchMg(cp_batch* b, Kern k, int level) {
@@ -283,7 +283,7 @@ int launchMg(cp_batch* b, Kern k, int level) {
     a.tileSlot = b->dMgTileSlot + size_t(level) * b->maxTiles;
     a.nTiles = b->dMgNTiles + level;
     a.level = level;
-    const int rows = level == 0 ? mgRows(f32, b->tileRows) : 1;
+    const int rows = level == 0 ? b->mgRowsEff() : 1;
     const int grid = perSlot ? b->maxSlots : mgGrid(b, level);
     const hipError_t e = launchKernel(k, f32, rows, grid, b->maxSlots, b->stream, a);
     if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
@@ -338,7 +338,9 @@ int rebuildTiles(cp_batch* b) {
     b->totalTiles = int(ts.size());
     int counts[kMaxLevels];
     std::vector<std::vector<int>> lists(kMaxLevels);  // alive until the stream is synchronised
-    b->mgLevels = 0;
+    // mgLevels never decreases: the captured step graph keeps its shape as
+    // slots come and go (levels deeper than every admitted slot's hierarchy
+    // have empty tile lists, and their launches exit at once)
     for (int l = 0; l < kMaxLevels; ++l) {
         std::vector<int>& ml = lists[l];
         for (int s = 0; s < b->maxSlots; ++s) {
@@ -440,7 +442,7 @@ int defaultTileRows(int M, int N) {
     return 1;
 }
 
-void setTiling(SlotDesc& d, int tileRows) {
+void setTiling(SlotDesc& d, int tileRows, int mgRows0) {
     d.tileRows = tileRows;
     d.tilesX = (d.N + 1 + kTileW - 1) / kTileW;
     const int rowsPerTile = kTileH * d.tileRows;
@@ -448,13 +450,13 @@ void setTiling(SlotDesc& d, int tileRows) {
     d.ntiles = d.tilesX * tilesY;
     for (int l = 0; l < d.nLevels; ++l) {
         MgLevel& L = d.lv[l];
-        const int rows = kTileH * (l == 0 ? mgRows(d.mixed != 0, tileRows) : 1);
+        const int rows = kTileH * (l == 0 ? mgRows0 : 1);
         L.tilesX = (L.W + kTileW - 1) / kTileW;
         L.ntiles = L.tilesX * ((L.H + rows - 1) / rows);
     }
 }
 
-SlotDesc makeDesc(const cp_params& p, int precision, int tileRows) {
+SlotDesc makeDesc(const cp_params& p, int precision, int tileRows, int mgRows0) {
     SlotDesc d;
     memset(&d, 0, sizeof(d));
     d.M = p.M;
@@ -473,7 +475,7 @@ SlotDesc makeDesc(const cp_params& p, int precision, int tileRows) {
             d.lv[l].pitch = l == 0 ? d.pitch : int(mgPitch(W[l]));
         }
     }
-    setTiling(d, tileRows);
+    setTiling(d, tileRows, mgRows0);
     d.nU = (p.M - 1) * (p.N - 2);
     d.nV = (p.M - 2) * (p.N - 1);
     d.convScheme = p.convectionScheme;
@@ -639,7 +641,6 @@ void batchDestroy(cp_batch* b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->graphExec) (void)hipGraphExecDestroy(b->graphExec);
-    for (hipEvent_t ev : b->probeEv) (void)hipEventDestroy(ev);
     for (hipEvent_t ev : b->burstEv)
         if (ev) (void)hipEventDestroy(ev);
     (void)hipFree(b->arenaBase);
@@ -671,7 +672,7 @@ int batchAdmit(cp_batch* b, const cp_params* prm, const double* ct, int* slotOut
     for (int k = 0; k < b->maxSlots; ++k)
         if (!b->slots[k].used) { s = k; break; }
     if (s < 0) return setError(&err, CP_E_CAPACITY, "no free slot");
-    SlotDesc d = makeDesc(*prm, b->precision, b->tileRows);
+    SlotDesc d = makeDesc(*prm, b->precision, b->tileRows, b->mgRowsEff());
     if (b->totalTiles + size_t(d.ntiles) > b->maxTiles) return setError(&err, CP_E_CAPACITY, "tile table full");
     const size_t bytes = b->footprint(prm->M, prm->N);
     size_t off = 0;
@@ -841,20 +842,10 @@ int graphRun(cp_batch* b, int steps) {
     const int J = maxOverSlots(b, &SlotDesc::maxMomIter);
     const int P = maxOverSlots(b, &SlotDesc::maxPresIter);
     const int R = maxOverSlots(b, &SlotDesc::refineCycles);
-    if (b->probeKernel >= 0) {
-        const size_t need = size_t(2) * (J + 2 * P * R + 8);
-        while (b->probeEv.size() < need) {
-            hipEvent_t ev;
-            HIP_TRY(hipEventCreate(&ev));
-            b->probeEv.push_back(ev);
-        }
-    }
     if (!b->useGraph) {
-        // the same fixed-budget sequence, launched kernel by kernel; the
-        // probe keeps the events of the last step
+        // the same fixed-budget sequence, launched kernel by kernel
         const int grid = gridFor(b, 1 << 30);
         for (int k = 0; k < steps; ++k) {
-            b->probeUsed = 0;
             const int rc = enqueueFixedStep(b, J, P, R, grid);
             if (rc) return rc;
         }
@@ -867,7 +858,6 @@ int graphRun(cp_batch* b, int steps) {
         }
         const int grid = gridFor(b, 1 << 30);  // fixed: independent of the admitted slots
         hipGraph_t g;
-        b->probeUsed = 0;
         HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         const int rc = enqueueFixedStep(b, J, P, R, grid);
         hipGraph_t captured = nullptr;
@@ -1080,11 +1070,119 @@ int cp_batch_set_tiling(cp_batch* b, int tile_rows, int wg_per_cu) {
         // the tiling is only a decomposition of each slot's grid: retile all
         b->tileRows = tile_rows;
         for (int s = 0; s < b->maxSlots; ++s)
-            if (b->slots[s].used) setTiling(b->hDesc[s], tile_rows);
+            if (b->slots[s].used) setTiling(b->hDesc[s], tile_rows, b->mgRowsEff());
         const int rc = rebuildTiles(b);
         if (rc) return rc;
     }
     if (wg_per_cu > 0) b->wgPerCU = wg_per_cu;
+    b->graphJ = -1;
+    return CP_OK;
+}
+
+constexpr int kProbeEvery = 10;     // polls between timing bursts
+constexpr int kProbeLaunches = 16;  // k_pcg_q launches per burst
+
+namespace {
+int burst(cp_batch* b, int launches, double* ms_per_launch);
+}
+
+int cp_serve(cp_batch* b, const cp_request* reqs, int n, int steps_per_poll, cp_result* results) {
+    if (!b) return setError(nullptr, CP_E_ARG, "null batch");
+    std::string& err = b->err;
+    if (n < 0 || (n > 0 && (!reqs || !results))) return setError(&err, CP_E_ARG, "null requests or results");
+    if (steps_per_poll < 1) return setError(&err, CP_E_ARG, "steps_per_poll must be >= 1");
+    for (int s = 0; s < b->maxSlots; ++s)
+        if (b->slots[s].used) return setError(&err, CP_E_STATE, "cp_serve needs an empty batch");
+    std::vector<int> order(n);
+    for (int k = 0; k < n; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return reqs[x].arrival < reqs[y].arrival; });
+    for (int k = 0; k < n; ++k) {
+        results[k] = cp_result();
+        results[k].slot = -1;
+        results[k].status = CP_SLOT_FAILED;
+    }
+    std::vector<int> reqOf(b->maxSlots, -1);
+    b->probeMs.clear();
+    b->probeUnknowns.clear();
+    long polls = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    auto now = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+    int next = 0, done = 0, running = 0;
+    while (done < n) {
+        // admissions, first come first served
+        while (next < n && reqs[order[next]].arrival <= now()) {
+            const int q = order[next];
+            int slot = -1;
+            int rc = batchAdmit(b, reqs[q].prm, reqs[q].cellType, &slot);
+            if (rc == CP_E_CAPACITY && running > 0) break;  // wait for an eviction
+            ++next;
+            if (rc) {  // invalid, or larger than the whole batch
+                results[q].rc = rc;
+                results[q].tAdmit = results[q].tDone = now();
+                ++done;
+                continue;
+            }
+            results[q].slot = slot;
+            results[q].tAdmit = now();
+            reqOf[slot] = q;
+            ++running;
+        }
+        if (running == 0) {
+            if (next < n) {
+                const double wait = reqs[order[next]].arrival - now();
+                if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.05)));
+            }
+            continue;
+        }
+        int rc = graphRun(b, steps_per_poll);
+        if (!rc) rc = pollState(b);
+        if (rc) return rc;
+        if (b->probeKernel == int(K_PCG_Q) && polls++ % kProbeEvery == 0) {
+            // timing burst of the PCG stencil kernel over the running slots
+            double ms = 0.0, unknowns = 0.0;
+            for (int s = 0; s < b->maxSlots; ++s)
+                if (reqOf[s] >= 0 && b->hState[s].status == ST_RUNNING)
+                    unknowns += double(b->slots[s].M - 1) * double(b->slots[s].N - 2);
+            if (unknowns > 0) {
+                if ((rc = burst(b, kProbeLaunches, &ms))) return rc;
+                b->probeMs.push_back(ms);
+                b->probeUnknowns.push_back(unknowns);
+            }
+        }
+        for (int s = 0; s < b->maxSlots; ++s) {
+            const int q = reqOf[s];
+            if (q < 0 || b->hState[s].status == ST_RUNNING) continue;
+            cp_result& r = results[q];
+            const SlotState& st = b->hState[s];
+            r.status = st.status;
+            r.iter = st.iter;
+            r.error = st.error;
+            const double xOut = double(b->slots[s].N) * b->slots[s].prm.hx;
+            cp_plane_metrics in, out;
+            if ((rc = planeSample(b, s, 0.0, reqs[q].gammaMin, &in))) return rc;
+            if ((rc = planeSample(b, s, xOut, reqs[q].gammaMin, &out))) return rc;
+            r.dp = in.avgTotal - out.avgTotal;
+            r.tDone = now();
+            if ((rc = batchEvict(b, s))) return rc;
+            reqOf[s] = -1;
+            --running;
+            ++done;
+        }
+    }
+    return CP_OK;
+}
+
+int cp_batch_set_mg_tiling(cp_batch* b, int rows) {
+    if (!b) return setError(nullptr, CP_E_ARG, "null batch");
+    const int maxRows = b->precision == CP_PREC_MIXED ? 4 : 2;
+    if (!(rows == 0 || rows == 1 || rows == 2 || rows == 4) || rows > maxRows)
+        return setError(&b->err, CP_E_ARG, "multigrid tile rows %d out of range (0, 1, 2%s)", rows,
+                        maxRows == 4 ? ", 4" : "");
+    b->mgTileRows = rows;
+    for (int s = 0; s < b->maxSlots; ++s)
+        if (b->slots[s].used) setTiling(b->hDesc[s], b->tileRows, b->mgRowsEff());
+    const int rc = rebuildTiles(b);
+    if (rc) return rc;
     b->graphJ = -1;
     return CP_OK;
 }
@@ -1097,21 +1195,18 @@ int cp_batch_set_graph(cp_batch* b, int use_graph) {
 
 int cp_batch_probe(cp_batch* b, int kernel) {
     if (!b) return setError(nullptr, CP_E_ARG, "null batch");
-    if (kernel < -1 || kernel > int(K_CORR_OUTER)) return setError(&b->err, CP_E_ARG, "unknown kernel %d", kernel);
-    b->probeKernel = kernel == int(K_CORR) ? int(K_CORR_OUTER) : kernel;
-    b->graphJ = -1;  // recapture with (or without) the event nodes
+    if (kernel != -1 && kernel != int(K_PCG_Q))
+        return setError(&b->err, CP_E_ARG, "timing bursts are defined for CP_K_PCG_Q only (an idempotent kernel)");
+    b->probeKernel = kernel;
     return CP_OK;
 }
 
-int cp_batch_probe_read(cp_batch* b, double* ms, int cap) {
+int cp_batch_probe_read(cp_batch* b, double* ms, double* unknowns, int cap) {
     if (!b || !ms) return setError(nullptr, CP_E_ARG, "null argument");
-    std::string& err = b->err;
-    HIP_TRY(hipStreamSynchronize(b->stream));
     int n = 0;
-    for (int k = 0; k + 1 < b->probeUsed && n < cap; k += 2) {
-        float t = 0.f;
-        HIP_TRY(hipEventElapsedTime(&t, b->probeEv[k], b->probeEv[k + 1]));
-        ms[n++] = double(t);
+    for (; n < int(b->probeMs.size()) && n < cap; ++n) {
+        ms[n] = b->probeMs[n];
+        if (unknowns) unknowns[n] = b->probeUnknowns[n];
     }
     return n;
 }
@@ -1122,6 +1217,12 @@ int cp_batch_probe_burst(cp_batch* b, int kernel, int launches, double* ms_per_l
     if (kernel != int(K_PCG_Q))
         return setError(&err, CP_E_ARG, "timing bursts are defined for CP_K_PCG_Q only (an idempotent kernel)");
     if (launches < 1 || launches > 4096) return setError(&err, CP_E_ARG, "launches must be in [1, 4096]");
+    return burst(b, launches, ms_per_launch);
+}
+
+namespace {
+int burst(cp_batch* b, int launches, double* ms_per_launch) {
+    std::string& err = b->err;
     for (hipEvent_t& ev : b->burstEv)
         if (!ev) HIP_TRY(hipEventCreate(&ev));
     KArgs a = kargs(b);
@@ -1140,6 +1241,7 @@ int cp_batch_probe_burst(cp_batch* b, int kernel, int launches, double* ms_per_l
     *ms_per_launch = double(t) / launches;
     return CP_OK;
 }
+}  // namespace
 
 // ---- single-geometry context ------------------------------------------------
 int cp_create(const cp_params* prm, const double* cellType, cp_ctx** out) {

@@ -274,30 +274,61 @@ int cp_batch_set_tiling(cp_batch* b, int tile_rows, int wg_per_cu);
    graph replays). */
 int cp_batch_set_graph(cp_batch* b, int use_graph);
 
-/* Work kernels of one outer iteration, for cp_batch_probe. */
-#define CP_K_MOM_ASSEMBLE 0
-#define CP_K_JACOBI       1
-#define CP_K_MOM_FINISH   2
-#define CP_K_PRES_ASSEMBLE 3
-#define CP_K_PCG_Q        4
-#define CP_K_PCG_UPDATE   5
-#define CP_K_CORRECT      8
+/* Rows per wave R of the finest multigrid level's 64 x 4R tiles (no
+   counterpart in the reference): 1, 2 or 4 (4 in mixed precision only);
+   0 derives it from the batch tile height. Every slot is retiled. */
+int cp_batch_set_mg_tiling(cp_batch* b, int rows);
 
-/* Measurement hook (no counterpart in the reference, which has no device
-   timing): record HIP events on the batch stream around every launch of
-   work kernel `kernel` (CP_K_*) inside the captured outer-iteration graph;
-   -1 disables. cp_batch_probe_read returns the durations (ms) of those
-   launches in the most recently completed replay, in launch order, and
-   their count. Call it after cp_batch_poll (which synchronises). */
+/* Kernel of the PCG iteration timed by the measurement hooks below. */
+#define CP_K_PCG_Q        4
+
+/* Measurement hooks (no counterpart in the reference, which has no device
+   timing). cp_batch_probe(b, CP_K_PCG_Q) makes cp_serve time the PCG
+   stencil kernel every 10 polls: a burst of 16 back-to-back launches over
+   the running slots between two HIP events on the batch stream (the
+   kernel recomputes the search direction and q = A p from unchanged
+   inputs, so a burst leaves every solve as it was); -1 disables.
+   cp_batch_probe_read returns the bursts of the last cp_serve, oldest
+   first: ms per launch, and the p' unknowns the launches covered
+   (unknowns may be NULL); the return value is their count.
+   cp_batch_probe_burst runs one such burst now and synchronises. */
 int cp_batch_probe(cp_batch* b, int kernel);
-int cp_batch_probe_read(cp_batch* b, double* ms, int cap);
-/* Timing burst (no counterpart in the reference): launch the PCG stencil
-   kernel (CP_K_PCG_Q, the only kernel accepted) `launches` times back to
-   back on the batch stream between two HIP events, over every running
-   slot, and return the mean time per launch in ms. The kernel only
-   recomputes the search direction and q = A p from unchanged inputs, so
-   the slots' state and results are unaffected. Synchronises the stream. */
+int cp_batch_probe_read(cp_batch* b, double* ms, double* unknowns, int cap);
 int cp_batch_probe_burst(cp_batch* b, int kernel, int launches, double* ms_per_launch);
+
+/* ------------------------------------------------------------------ */
+/* request loop with continuous batching                               */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference, which solves one geometry per process
+   (SIMPLE.cpp:238-512). A request is one solve: parameters, geometry, its
+   arrival time, and the plane row rule of its reported pressure drop. */
+typedef struct cp_request {
+    const cp_params* prm;     /* precision must match the batch */
+    const double* cellType;   /* M x N row-major */
+    double arrival;           /* seconds after the start of cp_serve */
+    double gammaMin;          /* samplePlaneAtX row rule (1 - 1e-6: reference) */
+} cp_request;
+
+typedef struct cp_result {
+    int32_t status;           /* CP_SLOT_CONVERGED / _MAXITER / _FAILED */
+    int32_t iter;             /* outer iterations */
+    int32_t slot;             /* slot it ran in (-1: never admitted) */
+    int32_t rc;               /* CP_OK, or the error that rejected it */
+    double  error;            /* final max(residU, residV, residMass) */
+    double  dp;               /* full-system total-pressure drop, x = 0 .. N hx */
+    double  tAdmit, tDone;    /* seconds after the start of cp_serve */
+} cp_result;
+
+/* Serve n requests with continuous batching, in real time: requests are
+   admitted in arrival order as soon as they have arrived and a slot and
+   arena space (the HBM footprint) are free, a request that does not fit
+   waits for evictions (first come, first served); every running slot
+   advances by steps_per_poll outer iterations per launch of the step
+   graph, and each finished solve is read out (pressure drop) and evicted
+   at once, freeing its space for the next arrivals. A request that can
+   never fit, or whose parameters are invalid, gets its error code in rc.
+   Returns CP_OK once every request has finished. */
+int cp_serve(cp_batch* b, const cp_request* reqs, int n, int steps_per_poll, cp_result* results);
 
 /* ------------------------------------------------------------------ */
 /* seeded geometry generators (cellType, M x N row-major)              */
