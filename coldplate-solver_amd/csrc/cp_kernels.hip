@@ -21,6 +21,18 @@
 
 namespace cpdev {
 
+// Field arrays live in device global memory, but their pointers come from
+// the slot descriptor as plain (generic) pointers; accessed as such, every
+// load and store is a flat access, which also counts against the LDS wait
+// counter and serialises with the LDS traffic of the halo tiles. G() states
+// the address space, so the accesses compile to global loads and stores.
+template <typename T>
+using gp = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gp<T> G(T* p) {
+    return (gp<T>)p;
+}
+
 // ---------------------------------------------------------------------------
 // reductions
 // ---------------------------------------------------------------------------
@@ -114,7 +126,9 @@ struct Tile {
 template <int R>
 __device__ __forceinline__ Tile decodeTile(const KArgs& a, int t) {
     Tile tl;
-    tl.s = a.tileSlot[t];
+    // the slot is uniform over the workgroup: keep it (and every descriptor
+    // field read through it) in scalar registers
+    tl.s = __builtin_amdgcn_readfirstlane(a.tileSlot[t]);
     const SlotDesc& d = a.desc[tl.s];
     const int lt = t - d.tile0;
     const int ty = lt / d.tilesX;
@@ -146,7 +160,7 @@ __device__ __forceinline__ double clampv(double x, double m) { return fmax(-m, f
 // Second-order upwind deferred correction (convection.cpp:53-186). The index
 // guards are the reference's, including its U/V asymmetry.
 template <bool IsU>
-__device__ __forceinline__ double souCorrection(const double* f, size_t k, int i, int j, int P, int M, int N,
+__device__ __forceinline__ double souCorrection(gp<const double> f, size_t k, int i, int j, int P, int M, int N,
                                                 double Fe, double Fw, double Fn, double Fs) {
     double S = 0.0;
     const bool eOK = IsU ? (j + 2 < N) : (j + 2 <= N);
@@ -183,8 +197,8 @@ __device__ __forceinline__ double souCorrection(const double* f, size_t k, int i
 template <typename T, bool IsU>
 __device__ __forceinline__ void momRow(const SlotDesc& d, size_t k, int i, int j, double maxVel, double cfl) {
     const int P = d.pitch;
-    const double* u = d.u;
-    const double* v = d.v;
+    const auto u = G(d.u);
+    const auto v = G(d.v);
     double ue, uw, vn, vs;
     if (IsU) {
         ue = clampv(0.5 * (u[k] + u[k + 1]), maxVel);
@@ -205,7 +219,7 @@ __device__ __forceinline__ void momRow(const SlotDesc& d, size_t k, int i, int j
     const double aN = d.Dn + fmax(0.0, -Fn);
     const double aS = d.Dn + fmax(0.0, Fs);
     const double sumA = aE + aW + aN + aS;
-    const double* f = IsU ? u : v;
+    const auto f = IsU ? u : v;
     const double cur = f[k];
     double trans = 0.0;
     if (d.pseudo) {
@@ -217,25 +231,25 @@ __device__ __forceinline__ void momRow(const SlotDesc& d, size_t k, int i, int j
         }
         trans = d.rho * d.vol / dt;
     }
-    const double alphaLocal = IsU ? d.aU[k] : d.aV[k];
+    const double alphaLocal = IsU ? G(d.aU)[k] : G(d.aV)[k];
     double aP0 = sumA + (Fe - Fw + Fn - Fs) + trans + d.sinkDiag + alphaLocal * d.vol;
     aP0 = fmax(aP0, 1e-10);
     const double aP = aP0 / d.uvAlpha;
-    double Sp = IsU ? (d.p[k] - d.p[k + 1]) * d.hy : (d.p[k] - d.p[k + P]) * d.hx;
+    double Sp = IsU ? (G(d.p)[k] - G(d.p)[k + 1]) * d.hy : (G(d.p)[k] - G(d.p)[k + P]) * d.hx;
     Sp += trans * cur;  // uOld == u at momentum time (iterations.cpp:118-119)
     Sp += (1.0 - d.uvAlpha) / d.uvAlpha * aP0 * cur;
     if (d.convScheme == 1) Sp += souCorrection<IsU>(f, k, i, j, P, d.M, d.N, Fe, Fw, Fn, Fs);
-    if (IsU) d.dE[k] = d.hy / aP0;
-    else d.dN[k] = d.hx / aP0;
+    if (IsU) G(d.dE)[k] = d.hy / aP0;
+    else G(d.dN)[k] = d.hx / aP0;
     const double r0 = Sp + aE * f[k + 1] + aW * f[k - 1] + aN * f[k + P] + aS * f[k - P] - aP * cur;
     void* const* A = IsU ? d.mu : d.mv;
-    static_cast<T*>(A[M_AP])[k] = T(aP);
-    static_cast<T*>(A[M_AE])[k] = T(aE);
-    static_cast<T*>(A[M_AW])[k] = T(aW);
-    static_cast<T*>(A[M_AN])[k] = T(aN);
-    static_cast<T*>(A[M_AS])[k] = T(aS);
-    static_cast<T*>(A[M_R0])[k] = T(r0);
-    static_cast<T*>(A[M_D0])[k] = T(0);
+    G(static_cast<T*>(A[M_AP]))[k] = T(aP);
+    G(static_cast<T*>(A[M_AE]))[k] = T(aE);
+    G(static_cast<T*>(A[M_AW]))[k] = T(aW);
+    G(static_cast<T*>(A[M_AN]))[k] = T(aN);
+    G(static_cast<T*>(A[M_AS]))[k] = T(aS);
+    G(static_cast<T*>(A[M_R0]))[k] = T(r0);
+    G(static_cast<T*>(A[M_D0]))[k] = T(0);
 }
 
 template <typename T, int R>
@@ -301,8 +315,8 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(KArgs a) {
         if (doneU && doneV) continue;
         const int M = d.M, N = d.N, P = d.pitch;
         const int pu = st.jk[0] & 1, pv = st.jk[1] & 1;
-        const T* srcU = static_cast<const T*>(d.mu[M_D0 + pu]);
-        const T* srcV = static_cast<const T*>(d.mv[M_D0 + pv]);
+        const auto srcU = G(static_cast<const T*>(d.mu[M_D0 + pu]));
+        const auto srcV = G(static_cast<const T*>(d.mv[M_D0 + pv]));
         __syncthreads();  // the previous tile's LDS reads are done
         for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
             const int hi = idx / HW, hj = idx - hi * HW;
@@ -320,26 +334,26 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(KArgs a) {
             // U unknowns: i in [1, M-1], j in [1, N-2]; V: i in [1, M-2], j in [1, N-1]
             if (!doneU && i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2) {
                 void* const* A = d.mu;
-                const T ap = static_cast<const T*>(A[M_AP])[k];
+                const T ap = G(static_cast<const T*>(A[M_AP]))[k];
                 const T dc = sU[row + 1][lane + 1];
-                const T Ax = ap * dc - static_cast<const T*>(A[M_AE])[k] * sU[row + 1][lane + 2] -
-                             static_cast<const T*>(A[M_AW])[k] * sU[row + 1][lane] -
-                             static_cast<const T*>(A[M_AN])[k] * sU[row + 2][lane + 1] -
-                             static_cast<const T*>(A[M_AS])[k] * sU[row][lane + 1];
-                const T res = static_cast<const T*>(A[M_R0])[k] - Ax;
-                static_cast<T*>(A[M_D0 + (pu ^ 1)])[k] = dc + res / ap;
+                const T Ax = ap * dc - G(static_cast<const T*>(A[M_AE]))[k] * sU[row + 1][lane + 2] -
+                             G(static_cast<const T*>(A[M_AW]))[k] * sU[row + 1][lane] -
+                             G(static_cast<const T*>(A[M_AN]))[k] * sU[row + 2][lane + 1] -
+                             G(static_cast<const T*>(A[M_AS]))[k] * sU[row][lane + 1];
+                const T res = G(static_cast<const T*>(A[M_R0]))[k] - Ax;
+                G(static_cast<T*>(A[M_D0 + (pu ^ 1)]))[k] = dc + res / ap;
                 v[0] += double(res) * double(res);
             }
             if (!doneV && i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1) {
                 void* const* A = d.mv;
-                const T ap = static_cast<const T*>(A[M_AP])[k];
+                const T ap = G(static_cast<const T*>(A[M_AP]))[k];
                 const T dc = sV[row + 1][lane + 1];
-                const T Ax = ap * dc - static_cast<const T*>(A[M_AE])[k] * sV[row + 1][lane + 2] -
-                             static_cast<const T*>(A[M_AW])[k] * sV[row + 1][lane] -
-                             static_cast<const T*>(A[M_AN])[k] * sV[row + 2][lane + 1] -
-                             static_cast<const T*>(A[M_AS])[k] * sV[row][lane + 1];
-                const T res = static_cast<const T*>(A[M_R0])[k] - Ax;
-                static_cast<T*>(A[M_D0 + (pv ^ 1)])[k] = dc + res / ap;
+                const T Ax = ap * dc - G(static_cast<const T*>(A[M_AE]))[k] * sV[row + 1][lane + 2] -
+                             G(static_cast<const T*>(A[M_AW]))[k] * sV[row + 1][lane] -
+                             G(static_cast<const T*>(A[M_AN]))[k] * sV[row + 2][lane + 1] -
+                             G(static_cast<const T*>(A[M_AS]))[k] * sV[row][lane + 1];
+                const T res = G(static_cast<const T*>(A[M_R0]))[k] - Ax;
+                G(static_cast<T*>(A[M_D0 + (pv ^ 1)]))[k] = dc + res / ap;
                 v[1] += double(res) * double(res);
             }
         }
@@ -362,8 +376,8 @@ __global__ __launch_bounds__(kBlock) void k_mom_finish(KArgs a) {
         const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
         const double maxVel = 3.0 * fmax(fabs(st.inletVelocity), 0.1);
         const double Uin = st.inletVelocity;
-        const T* dU = static_cast<const T*>(d.mu[M_D0 + (st.jiters[0] & 1)]);
-        const T* dV = static_cast<const T*>(d.mv[M_D0 + (st.jiters[1] & 1)]);
+        const auto dU = G(static_cast<const T*>(d.mu[M_D0 + (st.jiters[0] & 1)]));
+        const auto dV = G(static_cast<const T*>(d.mv[M_D0 + (st.jiters[1] & 1)]));
         double v[3] = {0.0, 0.0, 0.0};
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -376,14 +390,14 @@ __global__ __launch_bounds__(kBlock) void k_mom_finish(KArgs a) {
                 if (i == 0 || i == M) {
                     val = 0.0;
                 } else if (j == 0) {
-                    val = (d.gam[k + 1] > 0.5) ? Uin : 0.0;  // gamma(i-1, 0) > 0.5
+                    val = (G(d.gam)[k + 1] > 0.5) ? Uin : 0.0;  // gamma(i-1, 0) > 0.5
                 } else if (j == N - 1) {
-                    val = clampv(d.u[k - 1] + double(dU[k - 1]), maxVel);  // outlet: u(i,N-1) = u(i,N-2)
+                    val = clampv(G(d.u)[k - 1] + double(dU[k - 1]), maxVel);  // outlet: u(i,N-1) = u(i,N-2)
                 } else {
-                    val = clampv(d.u[k] + double(dU[k]), maxVel);
-                    v[0] = nanmax(v[0], fabs(val - d.u[k]));
+                    val = clampv(G(d.u)[k] + double(dU[k]), maxVel);
+                    v[0] = nanmax(v[0], fabs(val - G(d.u)[k]));
                 }
-                d.us[k] = val;
+                G(d.us)[k] = val;
                 if (!(val == val)) v[2] = 1.0;
             }
             // v* : M x (N+1)
@@ -392,12 +406,12 @@ __global__ __launch_bounds__(kBlock) void k_mom_finish(KArgs a) {
                 if (i == 0 || i == M - 1 || j == 0) {
                     val = 0.0;
                 } else if (j == N) {
-                    val = clampv(d.v[k - 1] + double(dV[k - 1]), maxVel);  // outlet: v(i,N) = v(i,N-1)
+                    val = clampv(G(d.v)[k - 1] + double(dV[k - 1]), maxVel);  // outlet: v(i,N) = v(i,N-1)
                 } else {
-                    val = clampv(d.v[k] + double(dV[k]), maxVel);
-                    v[1] = nanmax(v[1], fabs(val - d.v[k]));
+                    val = clampv(G(d.v)[k] + double(dV[k]), maxVel);
+                    v[1] = nanmax(v[1], fabs(val - G(d.v)[k]));
                 }
-                d.vs[k] = val;
+                G(d.vs)[k] = val;
                 if (!(val == val)) v[2] = 1.0;
             }
         }
@@ -426,25 +440,25 @@ __global__ __launch_bounds__(kBlock) void k_pres_assemble(KArgs a) {
             if (!(i >= 1 && i <= M - 1 && j >= 1 && j <= N - 1)) continue;
             const size_t k = size_t(i) * P + j;
             const double rho = d.rho, hx = d.hx, hy = d.hy;
-            const double mE = rho * d.us[k] * hy, mW = rho * d.us[k - 1] * hy;
-            const double mN = rho * d.vs[k] * hx, mS = rho * d.vs[k - P] * hx;
+            const double mE = rho * G(d.us)[k] * hy, mW = rho * G(d.us)[k - 1] * hy;
+            const double mN = rho * G(d.vs)[k] * hx, mS = rho * G(d.vs)[k - P] * hx;
             const double b = mW - mE + mS - mN;
             if (j <= jMass) v[0] = nanmax(v[0], fabs(b));
             if (j <= N - 2) {
-                const double aE = rho * d.dE[k] * hy;
-                const double aW = (j > 1) ? rho * d.dE[k - 1] * hy : 0.0;
-                const double aN = (i < M - 1) ? rho * d.dN[k] * hx : 0.0;
-                const double aS = (i > 1) ? rho * d.dN[k - P] * hx : 0.0;
+                const double aE = rho * G(d.dE)[k] * hy;
+                const double aW = (j > 1) ? rho * G(d.dE)[k - 1] * hy : 0.0;
+                const double aN = (i < M - 1) ? rho * G(d.dN)[k] * hx : 0.0;
+                const double aS = (i > 1) ? rho * G(d.dN)[k - P] * hx : 0.0;
                 double aP = aE + aW + aN + aS;
                 if (aP < 1e-20) aP = 1.0;
-                static_cast<T*>(d.pr[P_CE])[k] = T(aE);
-                static_cast<T*>(d.pr[P_CN])[k] = T(aN);
-                static_cast<T*>(d.pr[P_AP])[k] = T(aP);
+                G(static_cast<T*>(d.pr[P_CE]))[k] = T(aE);
+                G(static_cast<T*>(d.pr[P_CN]))[k] = T(aN);
+                G(static_cast<T*>(d.pr[P_AP]))[k] = T(aP);
                 const T rr = T(b);
-                static_cast<T*>(d.pr[P_R])[k] = rr;
-                static_cast<T*>(d.pr[P_X])[k] = T(0);
-                d.bm[k] = b;
-                d.pc[k] = 0.0;
+                G(static_cast<T*>(d.pr[P_R]))[k] = rr;
+                G(static_cast<T*>(d.pr[P_X]))[k] = T(0);
+                G(d.bm)[k] = b;
+                G(d.pc)[k] = 0.0;
                 const T z = rr / T(aP);
                 v[1] += double(rr) * double(z);
             }
@@ -466,8 +480,8 @@ __device__ __forceinline__ bool isPDof(int i, int j, int M, int N) {
 // p_k at (i, j): z + beta p_{k-1} on p' unknowns, 0 elsewhere. z is the
 // multigrid result when Z is given, else the Jacobi z = r / aP.
 template <typename T>
-__device__ __forceinline__ T pAt(const T* r, const T* aP, const T* Z, const T* pold, int i, int j, int M, int N,
-                                 int P, int kk, T beta) {
+__device__ __forceinline__ T pAt(gp<const T> r, gp<const T> aP, gp<const T> Z, gp<const T> pold, int i, int j,
+                                 int M, int N, int P, int kk, T beta) {
     if (!isPDof(i, j, M, N)) return T(0);
     const size_t k = size_t(i) * P + j;
     const T z = Z ? Z[k] : r[k] / aP[k];
@@ -493,18 +507,18 @@ __global__ __launch_bounds__(kBlock) void k_pcg_q(KArgs a) {
         const int M = d.M, N = d.N, P = d.pitch;
         const int kk = st.pk;
         const T beta = T(st.beta);
-        const T* r = static_cast<const T*>(d.pr[P_R]);
-        const T* aP = static_cast<const T*>(d.pr[P_AP]);
-        const T* cE = static_cast<const T*>(d.pr[P_CE]);
-        const T* cN = static_cast<const T*>(d.pr[P_CN]);
-        const T* pold = static_cast<const T*>(d.pr[P_PK0 + (kk & 1)]);
-        const T* Z = d.precond ? static_cast<const T*>(d.pr[P_Z]) : nullptr;
-        T* pnew = static_cast<T*>(d.pr[P_PK0 + ((kk & 1) ^ 1)]);
-        T* Q = static_cast<T*>(d.pr[P_Q]);
+        const auto r = G(static_cast<const T*>(d.pr[P_R]));
+        const auto aP = G(static_cast<const T*>(d.pr[P_AP]));
+        const auto cE = G(static_cast<const T*>(d.pr[P_CE]));
+        const auto cN = G(static_cast<const T*>(d.pr[P_CN]));
+        const auto pold = G(static_cast<const T*>(d.pr[P_PK0 + (kk & 1)]));
+        const gp<const T> Z = d.precond ? G(static_cast<const T*>(d.pr[P_Z])) : gp<const T>(nullptr);
+        const auto pnew = G(static_cast<T*>(d.pr[P_PK0 + ((kk & 1) ^ 1)]));
+        const auto Q = G(static_cast<T*>(d.pr[P_Q]));
         __syncthreads();  // the previous tile's LDS reads are done
         for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
             const int hi = idx / HW, hj = idx - hi * HW;
-            sp[hi][hj] = pAt(r, aP, Z, pold, tl.i0 - 1 + hi, tl.j0 - 1 + hj, M, N, P, kk, beta);
+            sp[hi][hj] = pAt<T>(r, aP, Z, pold, tl.i0 - 1 + hi, tl.j0 - 1 + hj, M, N, P, kk, beta);
         }
         for (int idx = int(threadIdx.x); idx < TH * (kTileW + 1); idx += kBlock) {
             const int ci = idx / (kTileW + 1), cj = idx - ci * (kTileW + 1);
@@ -550,11 +564,11 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
         const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
         const int kk = st.pk;
         const T alpha = T(st.alpha);
-        const T* pk = static_cast<const T*>(d.pr[P_PK0 + ((kk & 1) ^ 1)]);
-        T* x = static_cast<T*>(d.pr[P_X]);
-        T* r = static_cast<T*>(d.pr[P_R]);
-        const T* Q = static_cast<const T*>(d.pr[P_Q]);
-        const T* aP = static_cast<const T*>(d.pr[P_AP]);
+        const auto pk = G(static_cast<const T*>(d.pr[P_PK0 + ((kk & 1) ^ 1)]));
+        const auto x = G(static_cast<T*>(d.pr[P_X]));
+        const auto r = G(static_cast<T*>(d.pr[P_R]));
+        const auto Q = G(static_cast<const T*>(d.pr[P_Q]));
+        const auto aP = G(static_cast<const T*>(d.pr[P_AP]));
         double v[2] = {0.0, 0.0};
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
@@ -584,13 +598,13 @@ __global__ __launch_bounds__(kBlock) void k_refine_accum(KArgs a) {
         SlotState& st = a.state[tl.s];
         if (st.status != ST_RUNNING || st.stepPhase) continue;
         const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
-        const T* x = static_cast<const T*>(d.pr[P_X]);
+        const auto x = G(static_cast<const T*>(d.pr[P_X]));
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int i = tl.i + r;
             if (!isPDof(i, j, M, N)) continue;
             const size_t k = size_t(i) * P + j;
-            d.pc[k] += double(x[k]);
+            G(d.pc)[k] += double(x[k]);
         }
     }
 }
@@ -604,10 +618,10 @@ __global__ __launch_bounds__(kBlock) void k_refine_resid(KArgs a) {
         SlotState& st = a.state[tl.s];
         if (st.status != ST_RUNNING || st.stepPhase) continue;
         const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
-        const T* aP = static_cast<const T*>(d.pr[P_AP]);
-        const T* cE = static_cast<const T*>(d.pr[P_CE]);
-        const T* cN = static_cast<const T*>(d.pr[P_CN]);
-        const double* pc = d.pc;
+        const auto aP = G(static_cast<const T*>(d.pr[P_AP]));
+        const auto cE = G(static_cast<const T*>(d.pr[P_CE]));
+        const auto cN = G(static_cast<const T*>(d.pr[P_CN]));
+        const auto pc = G(d.pc);
         double v[2] = {0.0, 0.0};
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
@@ -618,10 +632,10 @@ __global__ __launch_bounds__(kBlock) void k_refine_resid(KArgs a) {
             const double Ap = -double(cN[k - P]) * pAt64(i - 1, j, k - P) - double(cE[k - 1]) * pAt64(i, j - 1, k - 1) +
                               double(aP[k]) * pc[k] - double(cE[k]) * pAt64(i, j + 1, k + 1) -
                               double(cN[k]) * pAt64(i + 1, j, k + P);
-            const double r64 = d.bm[k] - Ap;
+            const double r64 = G(d.bm)[k] - Ap;
             const T r = T(r64);
-            static_cast<T*>(d.pr[P_R])[k] = r;
-            static_cast<T*>(d.pr[P_X])[k] = T(0);
+            G(static_cast<T*>(d.pr[P_R]))[k] = r;
+            G(static_cast<T*>(d.pr[P_X]))[k] = T(0);
             v[0] += r64 * r64;
             v[1] += double(r) * double(r / aP[k]);
         }
@@ -647,7 +661,7 @@ struct MgTile {
 template <int R>
 __device__ __forceinline__ MgTile decodeMgTile(const KArgs& a, int t) {
     MgTile tl;
-    tl.s = a.tileSlot[t];
+    tl.s = __builtin_amdgcn_readfirstlane(a.tileSlot[t]);
     const MgLevel& L = a.desc[tl.s].lv[a.level];
     const int lt = t - L.tile0;
     const int ty = lt / L.tilesX;
@@ -665,6 +679,28 @@ __device__ __forceinline__ bool mgActive(const SlotDesc& d, const SlotState& st)
 __device__ __forceinline__ size_t mgAt(const MgLevel& L, int I, int J) {
     return size_t(I + 1) * L.pitch + (J + 1);
 }
+
+// Level shape and arrays copied to registers once per tile (reading them
+// through the descriptor inside the loops would reload them after every
+// store, since the compiler cannot rule out aliasing).
+template <typename T>
+struct MgView {
+    int H, W, P;
+    gp<const T> aP;
+    gp<const T> cE;
+    gp<const T> cN;
+    gp<const T> b;
+    gp<const T> x;
+    gp<T> xo;
+    gp<T> z;
+    gp<T> bo;
+    __device__ __forceinline__ explicit MgView(const MgLevel& L)
+        : H(L.H), W(L.W), P(L.pitch), aP(G(static_cast<const T*>(L.aP))), cE(G(static_cast<const T*>(L.cE))),
+          cN(G(static_cast<const T*>(L.cN))), b(G(static_cast<const T*>(L.b))), x(G(static_cast<const T*>(L.x))),
+          xo(G(static_cast<T*>(L.x))), z(G(static_cast<T*>(L.z))), bo(G(static_cast<T*>(L.b))) {}
+    __device__ __forceinline__ bool in(int I, int J) const { return I >= 0 && I < H && J >= 0 && J < W; }
+    __device__ __forceinline__ size_t at(int I, int J) const { return size_t(I + 1) * P + (J + 1); }
+};
 
 // A x at LDS position (h, w) of a halo tile, terms in the CSR column order
 // S, W, P, E, N (absent neighbours hold zero coefficients and values).
@@ -692,15 +728,14 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarsen(KArgs a) {
         const SlotDesc& d = a.desc[tl.s];
         const SlotState& st = a.state[tl.s];
         if (!mgActive(d, st) || a.level >= d.nLevels) continue;
-        const MgLevel& C = d.lv[a.level];
-        const MgLevel& F = d.lv[a.level - 1];
-        const T* faP = static_cast<const T*>(F.aP);
-        const T* fcE = static_cast<const T*>(F.cE);
-        const T* fcN = static_cast<const T*>(F.cN);
-        auto in = [&](int I, int J) { return I >= 0 && I < F.H && J >= 0 && J < F.W; };
-        auto g = [&](int I, int J) { return in(I, J) ? double(faP[mgAt(F, I, J)]) : 0.0; };
-        auto eE = [&](int I, int J) { return (in(I, J) && J + 1 < F.W) ? double(fcE[mgAt(F, I, J)]) : 0.0; };
-        auto eN = [&](int I, int J) { return (in(I, J) && I + 1 < F.H) ? double(fcN[mgAt(F, I, J)]) : 0.0; };
+        const MgView<T> C(d.lv[a.level]);
+        const MgView<T> F(d.lv[a.level - 1]);
+        auto g = [&](int I, int J) { return F.in(I, J) ? double(F.aP[F.at(I, J)]) : 0.0; };
+        auto eE = [&](int I, int J) { return (F.in(I, J) && J + 1 < F.W) ? double(F.cE[F.at(I, J)]) : 0.0; };
+        auto eN = [&](int I, int J) { return (F.in(I, J) && I + 1 < F.H) ? double(F.cN[F.at(I, J)]) : 0.0; };
+        const auto caP = G(static_cast<T*>(d.lv[a.level].aP));
+        const auto ccE = G(static_cast<T*>(d.lv[a.level].cE));
+        const auto ccN = G(static_cast<T*>(d.lv[a.level].cN));
         const int lane = int(threadIdx.x & 63), w = int(threadIdx.x >> 6);
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
@@ -717,10 +752,10 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarsen(KArgs a) {
             internal += eE(i0 + 1, j0);
             internal += eN(i0, j0);
             internal += eN(i0, j0 + 1);
-            const size_t k = mgAt(C, I, J);
-            static_cast<T*>(C.aP)[k] = T(acc - 2.0 * internal);
-            static_cast<T*>(C.cE)[k] = T(eE(i0, j0 + 1) + eE(i0 + 1, j0 + 1));
-            static_cast<T*>(C.cN)[k] = T(eN(i0 + 1, j0) + eN(i0 + 1, j0 + 1));
+            const size_t k = C.at(I, J);
+            caP[k] = T(acc - 2.0 * internal);
+            ccE[k] = T(eE(i0, j0 + 1) + eE(i0 + 1, j0 + 1));
+            ccN[k] = T(eN(i0 + 1, j0) + eN(i0 + 1, j0 + 1));
         }
     }
 }
@@ -740,7 +775,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_factor(KArgs a) {
     const bool f32 = d.mixed != 0;
     auto ld = [&](void* p, int r) {
         const size_t k = mgAt(L, r / W, r % W);
-        return f32 ? double(static_cast<const float*>(p)[k]) : static_cast<const double*>(p)[k];
+        return f32 ? double(G(static_cast<const float*>(p))[k]) : G(static_cast<const double*>(p))[k];
     };
     for (int idx = int(threadIdx.x); idx < n * n; idx += kBlock) {
         const int r = idx / n, c = idx - r * n;
@@ -773,7 +808,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_factor(KArgs a) {
         }
         __syncthreads();
     }
-    for (int idx = int(threadIdx.x); idx < n * n; idx += kBlock) d.coarseInv[idx] = A[idx / n][idx % n];
+    for (int idx = int(threadIdx.x); idx < n * n; idx += kBlock) G(d.coarseInv)[idx] = A[idx / n][idx % n];
 }
 
 // Down-sweep at level a.level (not the slot's coarsest): x1 = w b/aP,
@@ -789,24 +824,20 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(KArgs a) {
         const SlotDesc& d = a.desc[tl.s];
         const SlotState& st = a.state[tl.s];
         if (!mgActive(d, st) || a.level >= d.nLevels - 1) continue;
-        const MgLevel& L = d.lv[a.level];
-        const MgLevel& C = d.lv[a.level + 1];
+        const MgView<T> L(d.lv[a.level]);
+        const MgView<T> C(d.lv[a.level + 1]);
         const T om = T(d.mgOmega);
-        const T* b = static_cast<const T*>(L.b);
-        const T* aP = static_cast<const T*>(L.aP);
-        const T* cE = static_cast<const T*>(L.cE);
-        const T* cN = static_cast<const T*>(L.cN);
         __syncthreads();  // the previous tile's LDS reads are done
         for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
             const int h = idx / HW, w = idx - h * HW;
             const int I = tl.I0 - 2 + h, J = tl.J0 - 2 + w;
-            const bool in = I >= 0 && I < L.H && J >= 0 && J < L.W;
-            const size_t k = in ? mgAt(L, I, J) : 0;
-            const T bv = in ? b[k] : T(0), av = in ? aP[k] : T(0);
+            const bool in = L.in(I, J);
+            const size_t k = in ? L.at(I, J) : 0;
+            const T bv = in ? L.b[k] : T(0), av = in ? L.aP[k] : T(0);
             sb[h][w] = bv;
             saP[h][w] = av;
-            scE[h][w] = in ? cE[k] : T(0);
-            scN[h][w] = in ? cN[k] : T(0);
+            scE[h][w] = in ? L.cE[k] : T(0);
+            scN[h][w] = in ? L.cN[k] : T(0);
             sx1[h][w] = in ? om * (bv / av) : T(0);
         }
         __syncthreads();
@@ -814,13 +845,12 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(KArgs a) {
             const int h = idx / HW, w = idx - h * HW;
             const int I = tl.I0 - 2 + h, J = tl.J0 - 2 + w;
             T v = T(0);
-            if (h >= 1 && h < HH - 1 && w >= 1 && w < HW - 1 && I >= 0 && I < L.H && J >= 0 && J < L.W)
+            if (h >= 1 && h < HH - 1 && w >= 1 && w < HW - 1 && L.in(I, J))
                 v = sx1[h][w] + om * ((sb[h][w] - mgAx(saP, scE, scN, sx1, h, w)) / saP[h][w]);
             sx2[h][w] = v;
         }
         __syncthreads();
         const int lane = int(threadIdx.x & 63), wv = int(threadIdx.x >> 6);
-        T* xo = static_cast<T*>(L.x);
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
             const int row = wv * R + rr, h = row + 2, w = lane + 2;
@@ -828,12 +858,11 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(KArgs a) {
             T r = T(0);
             if (I < L.H && J < L.W) {
                 r = sb[h][w] - mgAx(saP, scE, scN, sx2, h, w);
-                xo[mgAt(L, I, J)] = sx2[h][w];
+                L.xo[L.at(I, J)] = sx2[h][w];
             }
             sx1[h][w] = r;  // x1 is no longer read: reuse for the residual
         }
         __syncthreads();
-        T* cb = static_cast<T*>(C.b);
         for (int idx = int(threadIdx.x); idx < (TH / 2) * (kTileW / 2); idx += kBlock) {
             const int ci = idx / (kTileW / 2), cj = idx - ci * (kTileW / 2);
             const int Ic = tl.I0 / 2 + ci, Jc = tl.J0 / 2 + cj;
@@ -844,7 +873,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(KArgs a) {
             acc += sx1[h][w + 1];
             acc += sx1[h + 1][w];
             acc += sx1[h + 1][w + 1];
-            cb[mgAt(C, Ic, Jc)] = acc;
+            C.bo[C.at(Ic, Jc)] = acc;
         }
     }
 }
@@ -860,12 +889,12 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarsest(KArgs a) {
     const MgLevel& L = d.lv[d.nLevels - 1];
     const int W = L.W, n = L.H * L.W;
     const int t = int(threadIdx.x);
-    if (t < n) sb[t] = double(static_cast<const T*>(L.b)[mgAt(L, t / W, t % W)]);
+    if (t < n) sb[t] = double(G(static_cast<const T*>(L.b))[mgAt(L, t / W, t % W)]);
     __syncthreads();
     if (t < n) {
         double acc = 0.0;
-        for (int k = 0; k < n; ++k) acc += d.coarseInv[size_t(t) * n + k] * sb[k];
-        static_cast<T*>(L.z)[mgAt(L, t / W, t % W)] = T(acc);
+        for (int k = 0; k < n; ++k) acc += G(d.coarseInv)[size_t(t) * n + k] * sb[k];
+        G(static_cast<T*>(L.z))[mgAt(L, t / W, t % W)] = T(acc);
     }
 }
 
@@ -882,39 +911,32 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(KArgs a) {
         const SlotDesc& d = a.desc[tl.s];
         const SlotState& st = a.state[tl.s];
         if (!mgActive(d, st) || a.level >= d.nLevels - 1) continue;
-        const MgLevel& L = d.lv[a.level];
-        const MgLevel& C = d.lv[a.level + 1];
+        const MgView<T> L(d.lv[a.level]);
+        const MgView<T> C(d.lv[a.level + 1]);
         const T om = T(d.mgOmega), sc = T(d.mgScale);
-        const T* b = static_cast<const T*>(L.b);
-        const T* aP = static_cast<const T*>(L.aP);
-        const T* cE = static_cast<const T*>(L.cE);
-        const T* cN = static_cast<const T*>(L.cN);
-        const T* x2 = static_cast<const T*>(L.x);
-        const T* zc = static_cast<const T*>(C.z);
         __syncthreads();
         for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
             const int h = idx / HW, w = idx - h * HW;
             const int I = tl.I0 - 2 + h, J = tl.J0 - 2 + w;
-            const bool in = I >= 0 && I < L.H && J >= 0 && J < L.W;
-            const size_t k = in ? mgAt(L, I, J) : 0;
-            sb[h][w] = in ? b[k] : T(0);
-            saP[h][w] = in ? aP[k] : T(0);
-            scE[h][w] = in ? cE[k] : T(0);
-            scN[h][w] = in ? cN[k] : T(0);
-            sx3[h][w] = in ? x2[k] + sc * zc[mgAt(C, I >> 1, J >> 1)] : T(0);
+            const bool in = L.in(I, J);
+            const size_t k = in ? L.at(I, J) : 0;
+            sb[h][w] = in ? L.b[k] : T(0);
+            saP[h][w] = in ? L.aP[k] : T(0);
+            scE[h][w] = in ? L.cE[k] : T(0);
+            scN[h][w] = in ? L.cN[k] : T(0);
+            sx3[h][w] = in ? L.x[k] + sc * C.z[C.at(I >> 1, J >> 1)] : T(0);
         }
         __syncthreads();
         for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
             const int h = idx / HW, w = idx - h * HW;
             const int I = tl.I0 - 2 + h, J = tl.J0 - 2 + w;
             T v = T(0);
-            if (h >= 1 && h < HH - 1 && w >= 1 && w < HW - 1 && I >= 0 && I < L.H && J >= 0 && J < L.W)
+            if (h >= 1 && h < HH - 1 && w >= 1 && w < HW - 1 && L.in(I, J))
                 v = sx3[h][w] + om * ((sb[h][w] - mgAx(saP, scE, scN, sx3, h, w)) / saP[h][w]);
             sx4[h][w] = v;
         }
         __syncthreads();
         const int lane = int(threadIdx.x & 63), wv = int(threadIdx.x >> 6);
-        T* zo = static_cast<T*>(L.z);
         double v[1] = {0.0};
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
@@ -922,7 +944,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(KArgs a) {
             const int I = tl.I0 + row, J = tl.J0 + lane;
             if (I < L.H && J < L.W) {
                 const T z = sx4[h][w] + om * ((sb[h][w] - mgAx(saP, scE, scN, sx4, h, w)) / saP[h][w]);
-                zo[mgAt(L, I, J)] = z;
+                L.z[L.at(I, J)] = z;
                 v[0] += double(sb[h][w]) * double(z);
             }
         }
@@ -1022,39 +1044,39 @@ __global__ __launch_bounds__(kBlock) void k_correct(KArgs a) {
             const int i = tl.i + rr;
             if (i > M || j > N) continue;
             const size_t k = size_t(i) * P + j;
-            const double* pc = d.pc;
+            const auto pc = G(d.pc);
             // pressure update + ghosts; each ghost has exactly one writer
             if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 1) {
-                const double pn = d.p[k] + d.pAlpha * pc[k];
-                d.p[k] = pn;
-                if (i == 1) d.p[k - P] = pn;      // p(0,j) = p(1,j)
-                if (i == M - 1) d.p[k + P] = pn;  // p(M,j) = p(M-1,j)
-                if (j == 1) d.p[k - 1] = pn;      // p(i,0) = p(i,1)
-                if (j == N - 1) d.p[k + 1] = 0.0; // p(i,N) = 0
+                const double pn = G(d.p)[k] + d.pAlpha * pc[k];
+                G(d.p)[k] = pn;
+                if (i == 1) G(d.p)[k - P] = pn;      // p(0,j) = p(1,j)
+                if (i == M - 1) G(d.p)[k + P] = pn;  // p(M,j) = p(M-1,j)
+                if (j == 1) G(d.p)[k - 1] = pn;      // p(i,0) = p(i,1)
+                if (j == N - 1) G(d.p)[k + 1] = 0.0; // p(i,N) = 0
                 if (!(pn == pn)) bad = 1.0;
             }
             auto uCorr = [&](size_t kk) {
-                const double du = clampv(d.dE[kk] * (pc[kk] - pc[kk + 1]), maxVel);
-                return clampv(d.us[kk] + du, maxVel);
+                const double du = clampv(G(d.dE)[kk] * (pc[kk] - pc[kk + 1]), maxVel);
+                return clampv(G(d.us)[kk] + du, maxVel);
             };
             auto vCorr = [&](size_t kk) {
-                const double dv = clampv(d.dN[kk] * (pc[kk] - pc[kk + P]), maxVel);
-                return clampv(d.vs[kk] + dv, maxVel);
+                const double dv = clampv(G(d.dN)[kk] * (pc[kk] - pc[kk + P]), maxVel);
+                return clampv(G(d.vs)[kk] + dv, maxVel);
             };
             if (j <= N - 1) {
                 double val;
                 if (i == 0 || i == M) val = 0.0;
-                else if (j == 0) val = (d.gam[k + 1] > 0.5) ? Uin : 0.0;
+                else if (j == 0) val = (G(d.gam)[k + 1] > 0.5) ? Uin : 0.0;
                 else if (j == N - 1) val = uCorr(k - 1);
                 else val = uCorr(k);
-                d.u[k] = val;
+                G(d.u)[k] = val;
             }
             if (i <= M - 1) {
                 double val;
                 if (i == 0 || i == M - 1 || j == 0) val = 0.0;
                 else if (j == N) val = vCorr(k - 1);
                 else val = vCorr(k);
-                d.v[k] = val;
+                G(d.v)[k] = val;
             }
         }
         double vb[1] = {bad};
@@ -1085,14 +1107,14 @@ __global__ __launch_bounds__(kBlock) void k_plane(const SlotDesc* desc, const Pl
     double acc[5] = {0, 0, 0, 0, 0};  // staticArea, dynArea, totalWeighted, area, flux
     for (int i = 1 + int(threadIdx.x); i < M; i += kBlock) {
         // gamma(i-1, c) lives at gam(i, c+1)
-        if (d.gam[size_t(i) * P + cl + 1] < rq.gammaMin || d.gam[size_t(i) * P + cr + 1] < rq.gammaMin) continue;
+        if (G(d.gam)[size_t(i) * P + cl + 1] < rq.gammaMin || G(d.gam)[size_t(i) * P + cr + 1] < rq.gammaMin) continue;
         const double fa = hy;
         acc[3] += fa;
-        const double sp = (1.0 - tt) * d.p[size_t(i) * P + pl] + tt * d.p[size_t(i) * P + pr];
-        const double un = (1.0 - tt) * d.u[size_t(i) * P + ul] + tt * d.u[size_t(i) * P + ur];
+        const double sp = (1.0 - tt) * G(d.p)[size_t(i) * P + pl] + tt * G(d.p)[size_t(i) * P + pr];
+        const double un = (1.0 - tt) * G(d.u)[size_t(i) * P + ul] + tt * G(d.u)[size_t(i) * P + ur];
         const int vt = min(i - 1, M - 1), vb = min(i, M - 1);
-        const double vT = (1.0 - tt) * d.v[size_t(vt) * P + vl] + tt * d.v[size_t(vt) * P + vr];
-        const double vB = (1.0 - tt) * d.v[size_t(vb) * P + vl] + tt * d.v[size_t(vb) * P + vr];
+        const double vT = (1.0 - tt) * G(d.v)[size_t(vt) * P + vl] + tt * G(d.v)[size_t(vt) * P + vr];
+        const double vB = (1.0 - tt) * G(d.v)[size_t(vb) * P + vl] + tt * G(d.v)[size_t(vb) * P + vr];
         const double vtan = 0.5 * (vT + vB);
         const double vm = sqrt(un * un + vtan * vtan);
         const double dyn = 0.5 * rho * vm * vm;
