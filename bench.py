@@ -73,7 +73,7 @@ def parse():
     ap.add_argument("--batch", type=int, default=8, help="slots of the request loop")
     ap.add_argument("--requests", type=int, default=24, help="solve requests per step")
     ap.add_argument("--eps", type=float, default=1e-7)
-    ap.add_argument("--jacobi", type=int, default=4, help="Jacobi sweeps per momentum solve")
+    ap.add_argument("--jacobi", type=int, default=3, help="Jacobi sweeps per momentum solve")
     ap.add_argument("--pcg", type=int, default=4, help="CG iterations per refinement cycle")
     ap.add_argument("--cycles", type=int, default=1, help="FP64 refinement cycles of p'")
     ap.add_argument("--precond", choices=["amg", "jacobi"], default="amg",

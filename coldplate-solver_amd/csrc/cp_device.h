@@ -25,6 +25,11 @@ constexpr int kNumPartials = 4;  // per-tile reduction partials (doubles)
 constexpr int kNumXCD = 8;       // accelerator complex dies of an MI355X
 constexpr int kMaxLevels = 12;   // aggregation-multigrid levels (incl. the finest)
 constexpr int kCoarseMax = 64;   // unknowns of the coarsest level (dense inverse)
+// The coarse end of the hierarchy runs in one workgroup per slot, in LDS:
+// from the first level whose remaining levels hold at most this many
+// unknowns in total (per array) down to the coarsest and back.
+constexpr int kTailCapF32 = 3072;
+constexpr int kTailCapF64 = 1536;
 
 // slot status
 constexpr int ST_RUNNING = 1;
@@ -66,6 +71,7 @@ struct alignas(64) SlotDesc {
     int enableSER, serMinIter, serUseMax, enableLS, lsMaxTries, enableRamp;
     int mixed;
     int precond, nLevels;        // CP_PRECOND_*; multigrid levels (>= 2 when precond == 1)
+    int mgTail;                  // first level handled by the one-workgroup tail kernel
     double mgOmega, mgScale;
     double hx, hy, rho, vol, De, Dn, convScale, sinkDiag, uvAlpha, pAlpha, hChar;
     double timeStep, minSpeed, momTol, pTol, epsilon, targetVel;
@@ -140,7 +146,7 @@ enum Kern {
     K_MG_COARSEN,   // multigrid: Galerkin coarse operator of level a.level
     K_MG_FACTOR,    // multigrid: dense inverse of the coarsest level (one block per slot)
     K_MG_DOWN,      // multigrid: pre-smoothing + restriction at level a.level
-    K_MG_COARSEST,  // multigrid: coarsest-level solve (one block per slot)
+    K_MG_TAIL,      // multigrid: levels mgTail.. down, coarsest solve, up (one block per slot)
     K_MG_UP,        // multigrid: prolongation + post-smoothing at level a.level
 };
 

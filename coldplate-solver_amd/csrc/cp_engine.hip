@@ -192,6 +192,7 @@ struct cp_batch {
     int* dMgTileSlot = nullptr;
     int* dMgNTiles = nullptr;
     int mgLevels = 0;  // deepest hierarchy of any multigrid slot admitted so far
+    int mgTail = 0;    // deepest first tail level of any multigrid slot admitted so far
     double* dPartials = nullptr;
     PlaneReq* dPlaneReq = nullptr;
     PlaneOut* dPlaneOut = nullptr;
@@ -278,7 +279,7 @@ int mgGrid(const cp_batch* b, int level) {
 
 int launchMg(cp_batch* b, Kern k, int level) {
     const bool f32 = b->precision == CP_PREC_MIXED;
-    const bool perSlot = k == K_MG_FACTOR || k == K_MG_COARSEST;
+    const bool perSlot = k == K_MG_FACTOR || k == K_MG_TAIL;
     KArgs a = kargs(b);
     a.tileSlot = b->dMgTileSlot + size_t(level) * b->maxTiles;
     a.nTiles = b->dMgNTiles + level;
@@ -298,15 +299,17 @@ int mgSetup(cp_batch* b) {
     return b->mgLevels > 0 ? launchMg(b, K_MG_FACTOR, 0) : CP_OK;
 }
 
-// z = V-cycle(r) for every multigrid slot whose CG is running, then r.z.
+// z = V-cycle(r) for every multigrid slot whose CG is running, then r.z:
+// per-level launches above the tails, one tail launch, per-level launches
+// back up.
 int mgVcycle(cp_batch* b) {
-    const int L = b->mgLevels;
-    if (L == 0) return CP_OK;
+    if (b->mgLevels == 0) return CP_OK;
+    const int T0 = b->mgTail;
     int rc;
-    for (int l = 0; l + 1 < L; ++l)
+    for (int l = 0; l < T0; ++l)
         if ((rc = launchMg(b, K_MG_DOWN, l))) return rc;
-    if ((rc = launchMg(b, K_MG_COARSEST, 0))) return rc;
-    for (int l = L - 2; l >= 0; --l)
+    if ((rc = launchMg(b, K_MG_TAIL, 0))) return rc;
+    for (int l = T0 - 1; l >= 0; --l)
         if ((rc = launchMg(b, K_MG_UP, l))) return rc;
     return CP_OK;
 }
@@ -350,6 +353,7 @@ int rebuildTiles(cp_batch* b) {
             d.lv[l].tile0 = int(ml.size());
             for (int k = 0; k < d.lv[l].ntiles; ++k) ml.push_back(s);
             b->mgLevels = std::max(b->mgLevels, d.nLevels);
+            b->mgTail = std::max(b->mgTail, d.mgTail);
         }
         if (ml.size() > b->maxTiles) return setError(&err, CP_E_CAPACITY, "multigrid tile table full");
         counts[l] = int(ml.size());
@@ -473,6 +477,18 @@ SlotDesc makeDesc(const cp_params& p, int precision, int tileRows, int mgRows0) 
             d.lv[l].H = H[l];
             d.lv[l].W = W[l];
             d.lv[l].pitch = l == 0 ? d.pitch : int(mgPitch(W[l]));
+        }
+        // the tail kernel takes over from the first level whose remaining
+        // levels fit its LDS arrays (the coarsest alone always does)
+        const long cap = precision == CP_PREC_MIXED ? kTailCapF32 : kTailCapF64;
+        d.mgTail = d.nLevels - 1;
+        for (int l = 1; l < d.nLevels; ++l) {
+            long sum = 0;
+            for (int k = l; k < d.nLevels; ++k) sum += long(H[k]) * W[k];
+            if (sum <= cap) {
+                d.mgTail = l;
+                break;
+            }
         }
     }
     setTiling(d, tileRows, mgRows0);
@@ -851,7 +867,8 @@ int graphRun(cp_batch* b, int steps) {
         }
         return CP_OK;
     }
-    if (!b->graphExec || J != b->graphJ || P != b->graphP || R != b->graphR || b->mgLevels != b->graphL) {
+    const int shapeL = b->mgLevels * (kMaxLevels + 1) + b->mgTail;
+    if (!b->graphExec || J != b->graphJ || P != b->graphP || R != b->graphR || shapeL != b->graphL) {
         if (b->graphExec) {
             HIP_TRY(hipGraphExecDestroy(b->graphExec));
             b->graphExec = nullptr;
@@ -870,7 +887,7 @@ int graphRun(cp_batch* b, int steps) {
         b->graphJ = J;
         b->graphP = P;
         b->graphR = R;
-        b->graphL = b->mgLevels;
+        b->graphL = shapeL;
     }
     for (int k = 0; k < steps; ++k) HIP_TRY(hipGraphLaunch(b->graphExec, b->stream));
     return CP_OK;

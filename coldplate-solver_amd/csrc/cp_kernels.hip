@@ -159,8 +159,8 @@ __device__ __forceinline__ double clampv(double x, double m) { return fmax(-m, f
 // ---------------------------------------------------------------------------
 // Second-order upwind deferred correction (convection.cpp:53-186). The index
 // guards are the reference's, including its U/V asymmetry.
-template <bool IsU>
-__device__ __forceinline__ double souCorrection(gp<const double> f, size_t k, int i, int j, int P, int M, int N,
+template <bool IsU, typename Ptr>
+__device__ __forceinline__ double souCorrection(Ptr f, size_t k, int i, int j, int P, int M, int N,
                                                 double Fe, double Fw, double Fn, double Fs) {
     double S = 0.0;
     const bool eOK = IsU ? (j + 2 < N) : (j + 2 <= N);
@@ -194,22 +194,22 @@ __device__ __forceinline__ double souCorrection(gp<const double> f, size_t k, in
 // four neighbour coefficients, dE/dN, the initial residual r0 = b - A u of
 // the Jacobi system (so the sweeps iterate on the correction delta = x - u
 // from delta = 0, which is the reference's x0 = u), and delta = 0.
+// u, v, p are read from the workgroup's LDS tile (index ks, row pitch P);
+// k indexes the node's global arrays.
 template <typename T, bool IsU>
-__device__ __forceinline__ void momRow(const SlotDesc& d, size_t k, int i, int j, double maxVel, double cfl) {
-    const int P = d.pitch;
-    const auto u = G(d.u);
-    const auto v = G(d.v);
+__device__ __forceinline__ void momRow(const SlotDesc& d, const double* u, const double* v, const double* pp,
+                                       size_t ks, int P, size_t k, int i, int j, double maxVel, double cfl) {
     double ue, uw, vn, vs;
     if (IsU) {
-        ue = clampv(0.5 * (u[k] + u[k + 1]), maxVel);
-        uw = clampv(0.5 * (u[k - 1] + u[k]), maxVel);
-        vn = clampv(0.5 * (v[k] + v[k + 1]), maxVel);
-        vs = clampv(0.5 * (v[k - P] + v[k - P + 1]), maxVel);
+        ue = clampv(0.5 * (u[ks] + u[ks + 1]), maxVel);
+        uw = clampv(0.5 * (u[ks - 1] + u[ks]), maxVel);
+        vn = clampv(0.5 * (v[ks] + v[ks + 1]), maxVel);
+        vs = clampv(0.5 * (v[ks - P] + v[ks - P + 1]), maxVel);
     } else {
-        ue = clampv(0.5 * (u[k] + u[k + P]), maxVel);
-        uw = clampv(0.5 * (u[k - 1] + u[k + P - 1]), maxVel);
-        vn = clampv(0.5 * (v[k] + v[k + P]), maxVel);
-        vs = clampv(0.5 * (v[k - P] + v[k]), maxVel);
+        ue = clampv(0.5 * (u[ks] + u[ks + P]), maxVel);
+        uw = clampv(0.5 * (u[ks - 1] + u[ks + P - 1]), maxVel);
+        vn = clampv(0.5 * (v[ks] + v[ks + P]), maxVel);
+        vs = clampv(0.5 * (v[ks - P] + v[ks]), maxVel);
     }
     const double cs = d.convScale;
     const double Fe = cs * d.rho * d.hy * ue, Fw = cs * d.rho * d.hy * uw;
@@ -220,7 +220,7 @@ __device__ __forceinline__ void momRow(const SlotDesc& d, size_t k, int i, int j
     const double aS = d.Dn + fmax(0.0, Fs);
     const double sumA = aE + aW + aN + aS;
     const auto f = IsU ? u : v;
-    const double cur = f[k];
+    const double cur = f[ks];
     double trans = 0.0;
     if (d.pseudo) {
         double dt = fmax(d.timeStep, 1e-20);
@@ -235,13 +235,13 @@ __device__ __forceinline__ void momRow(const SlotDesc& d, size_t k, int i, int j
     double aP0 = sumA + (Fe - Fw + Fn - Fs) + trans + d.sinkDiag + alphaLocal * d.vol;
     aP0 = fmax(aP0, 1e-10);
     const double aP = aP0 / d.uvAlpha;
-    double Sp = IsU ? (G(d.p)[k] - G(d.p)[k + 1]) * d.hy : (G(d.p)[k] - G(d.p)[k + P]) * d.hx;
+    double Sp = IsU ? (pp[ks] - pp[ks + 1]) * d.hy : (pp[ks] - pp[ks + P]) * d.hx;
     Sp += trans * cur;  // uOld == u at momentum time (iterations.cpp:118-119)
     Sp += (1.0 - d.uvAlpha) / d.uvAlpha * aP0 * cur;
-    if (d.convScheme == 1) Sp += souCorrection<IsU>(f, k, i, j, P, d.M, d.N, Fe, Fw, Fn, Fs);
+    if (d.convScheme == 1) Sp += souCorrection<IsU>(f, ks, i, j, P, d.M, d.N, Fe, Fw, Fn, Fs);
     if (IsU) G(d.dE)[k] = d.hy / aP0;
     else G(d.dN)[k] = d.hx / aP0;
-    const double r0 = Sp + aE * f[k + 1] + aW * f[k - 1] + aN * f[k + P] + aS * f[k - P] - aP * cur;
+    const double r0 = Sp + aE * f[ks + 1] + aW * f[ks - 1] + aN * f[ks + P] + aS * f[ks - P] - aP * cur;
     void* const* A = IsU ? d.mu : d.mv;
     G(static_cast<T*>(A[M_AP]))[k] = T(aP);
     G(static_cast<T*>(A[M_AE]))[k] = T(aE);
@@ -252,24 +252,46 @@ __device__ __forceinline__ void momRow(const SlotDesc& d, size_t k, int i, int j
     G(static_cast<T*>(A[M_D0]))[k] = T(0);
 }
 
+// The workgroup stages u, v and p of its tile with a two-deep halo (the SOU
+// stencil reaches two nodes out) in LDS, so every neighbour read of the
+// momentum rows is an LDS read; nodes outside the grid hold 0, and no row
+// reads them.
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_mom_assemble(KArgs a) {
+    constexpr int TH = kTileH * R, HH = TH + 4, HW = kTileW + 4;
+    __shared__ double su[HH * HW], sv[HH * HW], sp[HH * HW];
     const int nT = *a.nTiles;
     XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
         if (st.status != ST_RUNNING) continue;
-        const int M = d.M, N = d.N, j = tl.j;
+        const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
         const double maxVel = 3.0 * fmax(fabs(st.inletVelocity), 0.1);
         const double cfl = st.pseudoCFL;
+        const auto gu = G(d.u), gv = G(d.v), gpp = G(d.p);
+        __syncthreads();  // the previous tile's LDS reads are done
+        for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
+            const int h = idx / HW, w = idx - h * HW;
+            const int i = tl.i0 - 2 + h, jj = tl.j0 - 2 + w;
+            const bool in = i >= 0 && i <= M && jj >= 0 && jj <= N;
+            const size_t k = in ? size_t(i) * P + jj : 0;
+            su[idx] = in ? gu[k] : 0.0;
+            sv[idx] = in ? gv[k] : 0.0;
+            sp[idx] = in ? gpp[k] : 0.0;
+        }
+        __syncthreads();
+        const int lane = int(threadIdx.x & 63), wv = int(threadIdx.x >> 6);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int i = tl.i + r;
             if (i > M || j > N) continue;
-            const size_t k = size_t(i) * d.pitch + j;
-            if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2) momRow<T, true>(d, k, i, j, maxVel, cfl);
-            if (i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1) momRow<T, false>(d, k, i, j, maxVel, cfl);
+            const size_t k = size_t(i) * P + j;
+            const size_t ks = size_t(wv * R + r + 2) * HW + (lane + 2);
+            if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2)
+                momRow<T, true>(d, su, sv, sp, ks, HW, k, i, j, maxVel, cfl);
+            if (i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1)
+                momRow<T, false>(d, su, sv, sp, ks, HW, k, i, j, maxVel, cfl);
         }
         // the workgroup holding the slot's first tile resets the Jacobi state
         // (no tile of this kernel reads it)
@@ -811,7 +833,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_factor(KArgs a) {
     for (int idx = int(threadIdx.x); idx < n * n; idx += kBlock) G(d.coarseInv)[idx] = A[idx / n][idx % n];
 }
 
-// Down-sweep at level a.level (not the slot's coarsest): x1 = w b/aP,
+// Down-sweep at level a.level (above the slot's tail): x1 = w b/aP,
 // x2 = x1 + w (b - A x1)/aP, r = b - A x2; stores x2 and the aggregate sums
 // of r as the next level's right-hand side.
 template <typename T, int R>
@@ -823,7 +845,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(KArgs a) {
         const MgTile tl = decodeMgTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         const SlotState& st = a.state[tl.s];
-        if (!mgActive(d, st) || a.level >= d.nLevels - 1) continue;
+        if (!mgActive(d, st) || a.level >= d.mgTail) continue;
         const MgView<T> L(d.lv[a.level]);
         const MgView<T> C(d.lv[a.level + 1]);
         const T om = T(d.mgOmega);
@@ -878,27 +900,119 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(KArgs a) {
     }
 }
 
-// Coarsest level: z = A^-1 b with the dense inverse, one workgroup per slot.
+// The coarse end of the V-cycle in one workgroup per slot: levels mgTail ..
+// nLevels-1 live in LDS (compact H x W arrays), the sweeps of a level are
+// separated by workgroup barriers instead of kernel launches, and the
+// coarsest level is solved with the dense inverse. The arithmetic is the
+// per-level kernels' (and the oracle's), point for point.
+constexpr int kTailBlock = 1024;  // threads of the one-workgroup tail
+
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mg_coarsest(KArgs a) {
+__device__ __forceinline__ T tailAx(const T* aP, const T* cE, const T* cN, const T* x, int H, int W, int I, int J) {
+    const int k = I * W + J;
+    T s = T(0);
+    if (I > 0) s += -cN[k - W] * x[k - W];
+    if (J > 0) s += -cE[k - 1] * x[k - 1];
+    s += aP[k] * x[k];
+    if (J + 1 < W) s += -cE[k] * x[k + 1];
+    if (I + 1 < H) s += -cN[k] * x[k + W];
+    return s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTailBlock) void k_mg_tail(KArgs a) {
+    constexpr int kCap = sizeof(T) == 4 ? kTailCapF32 : kTailCapF64;
+    __shared__ T sAP[kCap], sCE[kCap], sCN[kCap], sB[kCap], sX[kCap], sZ[kCap], sT[kCap];
+    __shared__ double sc64[kCoarseMax];
     const int s = blockIdx.x;
     const SlotDesc& d = a.desc[s];
     const SlotState& st = a.state[s];
     if (d.ntiles == 0 || !mgActive(d, st)) return;
-    __shared__ double sb[kCoarseMax];
-    const MgLevel& L = d.lv[d.nLevels - 1];
-    const int W = L.W, n = L.H * L.W;
-    const int t = int(threadIdx.x);
-    if (t < n) sb[t] = double(G(static_cast<const T*>(L.b))[mgAt(L, t / W, t % W)]);
+    const int L0 = d.mgTail, L = d.nLevels;
+    const T om = T(d.mgOmega), sc = T(d.mgScale);
+    const int tid = int(threadIdx.x);
+    __shared__ int off[kMaxLevels + 1];  // compact offsets of the tail levels
+    if (tid == 0) {
+        off[L0] = 0;
+        for (int l = L0; l < L; ++l) off[l + 1] = off[l] + d.lv[l].H * d.lv[l].W;
+    }
     __syncthreads();
-    if (t < n) {
-        double acc = 0.0;
-        for (int k = 0; k < n; ++k) acc += G(d.coarseInv)[size_t(t) * n + k] * sb[k];
-        G(static_cast<T*>(L.z))[mgAt(L, t / W, t % W)] = T(acc);
+    for (int l = L0; l < L; ++l) {
+        const MgView<T> V(d.lv[l]);
+        const int n = V.H * V.W, o = off[l];
+        for (int idx = tid; idx < n; idx += kTailBlock) {
+            const int I = idx / V.W, J = idx - I * V.W;
+            const size_t k = V.at(I, J);
+            sAP[o + idx] = V.aP[k];
+            sCE[o + idx] = V.cE[k];
+            sCN[o + idx] = V.cN[k];
+            if (l == L0) sB[o + idx] = V.b[k];
+        }
+    }
+    __syncthreads();
+    for (int l = L0; l + 1 < L; ++l) {
+        const int H = d.lv[l].H, W = d.lv[l].W, n = H * W, o = off[l];
+        const T *aP = sAP + o, *cE = sCE + o, *cN = sCN + o, *b = sB + o;
+        T *x1 = sT + o, *x2 = sX + o;
+        for (int idx = tid; idx < n; idx += kTailBlock) x1[idx] = om * (b[idx] / aP[idx]);
+        __syncthreads();
+        for (int idx = tid; idx < n; idx += kTailBlock) {
+            const int I = idx / W, J = idx - I * W;
+            x2[idx] = x1[idx] + om * ((b[idx] - tailAx(aP, cE, cN, x1, H, W, I, J)) / aP[idx]);
+        }
+        __syncthreads();
+        const int Hc = d.lv[l + 1].H, Wc = d.lv[l + 1].W;
+        for (int idx = tid; idx < Hc * Wc; idx += kTailBlock) {
+            const int Ic = idx / Wc, Jc = idx - Ic * Wc;
+            T acc = T(0);
+            for (int q = 0; q < 4; ++q) {
+                const int I = 2 * Ic + (q >> 1), J = 2 * Jc + (q & 1);
+                if (I < H && J < W) acc += b[I * W + J] - tailAx(aP, cE, cN, x2, H, W, I, J);
+            }
+            sB[off[l + 1] + idx] = acc;
+        }
+        __syncthreads();
+    }
+    {
+        const int n = d.lv[L - 1].H * d.lv[L - 1].W, o = off[L - 1];
+        if (tid < n) sc64[tid] = double(sB[o + tid]);
+        __syncthreads();
+        if (tid < n) {
+            double acc = 0.0;
+            for (int k = 0; k < n; ++k) acc += G(d.coarseInv)[size_t(tid) * n + k] * sc64[k];
+            sZ[o + tid] = T(acc);
+        }
+        __syncthreads();
+    }
+    for (int l = L - 2; l >= L0; --l) {
+        const int H = d.lv[l].H, W = d.lv[l].W, n = H * W, o = off[l];
+        const int Wc = d.lv[l + 1].W, oc = off[l + 1];
+        const T *aP = sAP + o, *cE = sCE + o, *cN = sCN + o, *b = sB + o;
+        T *x3 = sT + o, *x4 = sX + o, *z = sZ + o;
+        for (int idx = tid; idx < n; idx += kTailBlock) {
+            const int I = idx / W, J = idx - I * W;
+            x3[idx] = x4[idx] + sc * sZ[oc + (I >> 1) * Wc + (J >> 1)];  // x4 still holds x2
+        }
+        __syncthreads();
+        for (int idx = tid; idx < n; idx += kTailBlock) {
+            const int I = idx / W, J = idx - I * W;
+            x4[idx] = x3[idx] + om * ((b[idx] - tailAx(aP, cE, cN, x3, H, W, I, J)) / aP[idx]);
+        }
+        __syncthreads();
+        for (int idx = tid; idx < n; idx += kTailBlock) {
+            const int I = idx / W, J = idx - I * W;
+            z[idx] = x4[idx] + om * ((b[idx] - tailAx(aP, cE, cN, x4, H, W, I, J)) / aP[idx]);
+        }
+        __syncthreads();
+    }
+    const MgView<T> V0(d.lv[L0]);
+    for (int idx = tid; idx < V0.H * V0.W; idx += kTailBlock) {
+        const int I = idx / V0.W, J = idx - I * V0.W;
+        V0.z[V0.at(I, J)] = sZ[off[L0] + idx];
     }
 }
 
-// Up-sweep at level a.level (not the slot's coarsest): x3 = x2 + s z_c,
+// Up-sweep at level a.level (above the slot's tail): x3 = x2 + s z_c,
 // x4 = x3 + w (b - A x3)/aP, z = x4 + w (b - A x4)/aP. On the finest level
 // the tile also reduces r.z (b is the PCG residual there).
 template <typename T, int R>
@@ -910,7 +1024,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(KArgs a) {
         const MgTile tl = decodeMgTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         const SlotState& st = a.state[tl.s];
-        if (!mgActive(d, st) || a.level >= d.nLevels - 1) continue;
+        if (!mgActive(d, st) || a.level >= d.mgTail) continue;
         const MgView<T> L(d.lv[a.level]);
         const MgView<T> C(d.lv[a.level + 1]);
         const T om = T(d.mgOmega), sc = T(d.mgScale);
@@ -1353,7 +1467,7 @@ static void launchTR(Kern k, int grid, hipStream_t s, const KArgs& a) {
             if constexpr (mgTileFits<T, R>()) hipLaunchKernelGGL((k_mg_up<T, R>), g, b, 0, s, a);
             break;
         case K_MG_FACTOR: hipLaunchKernelGGL(k_mg_factor, g, b, 0, s, a); break;
-        case K_MG_COARSEST: hipLaunchKernelGGL((k_mg_coarsest<T>), g, b, 0, s, a); break;
+        case K_MG_TAIL: hipLaunchKernelGGL((k_mg_tail<T>), g, dim3(kTailBlock), 0, s, a); break;
     }
 }
 
