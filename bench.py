@@ -79,6 +79,10 @@ def parse():
     ap.add_argument("--precond", choices=["amg", "jacobi"], default="amg",
                     help="pressure-CG preconditioner of the served form")
     ap.add_argument("--check", type=int, default=10, help="outer iterations per graph launch between polls")
+    ap.add_argument("--workload", choices=["network", "batch64", "stream", "gamma2048"], default="network",
+                    help="network: the headline line (1024x512 networks); batch64, stream, gamma2048: the "
+                         "batched, served and 2048^2 gamma-field configurations, reported on their own line")
+    ap.add_argument("--rate", type=float, default=2.0, help="stream: Poisson arrival rate (requests/s)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dropin", action="store_true")
@@ -202,6 +206,118 @@ def copy_calibration(mib: int) -> None:
     log(f"copy calibration: 3 x {mib} MiB device-to-device")
 
 
+def served_params(p, a, eps):
+    q = wl.fast_mode(p, a.jacobi, a.pcg, a.cycles)
+    q.pressurePrecond = cb.CP_PRECOND_AMG if a.precond == "amg" else cb.CP_PRECOND_JACOBI
+    q.epsilon = eps
+    q.pTol = 1e-12
+    q.maxIterations = 20000
+    return q
+
+
+def pcg_roofline(b: cb.Batch) -> dict:
+    ms, un = b.probe_read()
+    if len(ms) == 0:
+        return None
+    t = float(np.sum(ms)) * PROBE_LAUNCHES
+    byt = float(np.sum(un)) * PCG_Q_BYTES_FP32 * PROBE_LAUNCHES
+    ach = byt / t / 1e6
+    return {"bound": "hbm", "kernel": "k_pcg_q<float>", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None, "avg_launch_us": round(t / len(ms) / PROBE_LAUNCHES * 1e3, 2),
+            "bytes_per_launch": int(byt / len(ms) / PROBE_LAUNCHES), "launches_timed": int(len(ms) * PROBE_LAUNCHES)}
+
+
+def other_workload(a) -> dict:
+    """The batched (64 slots), served (Poisson stream) and 2048^2 gamma-field
+    configurations; reported on their own line, never as the headline."""
+    base = {"metric": METRIC, "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32+f64", "data": "synthetic", "steps": 1, "warmup": 0}
+    if a.workload == "batch64":
+        geoms = wl.batch_64()
+        eps = 1e-6
+        reqs = [(served_params(p, a, eps), ct, 0.0, 0.5 if k % 2 else GAMMA_MIN) for k, (p, ct) in enumerate(geoms)]
+        tmp = cb.Batch(cb.CP_PREC_MIXED, 1 << 20, 1)
+        need = sum(tmp.footprint(q.M, q.N) for q, _, _, _ in reqs)
+        tmp.close()
+        b = cb.Batch(cb.CP_PREC_MIXED, need + (256 << 20), 64)
+        b.probe(cb.K_PCG_Q)
+        t0 = time.perf_counter()
+        res = b.serve(reqs, a.check)
+        dt = time.perf_counter() - t0
+        conv = sum(r["status"] == cb.SLOT_CONVERGED for r in res)
+        lat = [r["tDone"] - r["tAdmit"] for r in res]
+        out = dict(base, value=round(conv / dt, 4), unit="converged solves/s", ms_per_step=round(dt * 1e3, 1),
+                   config={"workload": "batch64", "geometries": 64, "sizes": "log-uniform 256x128 .. 1024x512",
+                           "mix": "32 networks + 32 gamma fields", "stop_test": f"error <= {eps:g}, cap 20000",
+                           "slots": 64},
+                   converged=conv, capped=len(res) - conv,
+                   latency_s={"p50": round(float(np.percentile(lat, 50)), 3),
+                              "p99": round(float(np.percentile(lat, 99)), 3)},
+                   outer_iterations={"mean": round(float(np.mean([r["iter"] for r in res])), 1),
+                                     "max": int(max(r["iter"] for r in res))},
+                   roofline=pcg_roofline(b), cpu_baseline=None)
+        b.close()
+        return out
+    if a.workload == "stream":
+        n = a.requests
+        specs = wl.stream_2000(n, 5)
+        arr = wl.poisson_arrivals(n, a.rate)
+        eps = 1e-6
+        reqs = []
+        for r, t in zip(specs, arr):
+            p, ct = wl.build_request(r)
+            reqs.append((served_params(p, a, eps), ct, float(t), 0.5 if r["kind"] == 2 else GAMMA_MIN))
+        b = cb.Batch(cb.CP_PREC_MIXED, 24 << 30, a.batch)
+        b.probe(cb.K_PCG_Q)
+        t0 = time.perf_counter()
+        res = b.serve(reqs, a.check)
+        dt = time.perf_counter() - t0
+        conv = sum(r["status"] == cb.SLOT_CONVERGED for r in res)
+        lat = [r["tDone"] - t for r, t in zip(res, arr)]
+        queue = [r["tAdmit"] - t for r, t in zip(res, arr)]
+        out = dict(base, value=round(conv / dt, 4), unit="converged solves/s", ms_per_step=round(dt * 1e3, 1),
+                   config={"workload": "stream", "requests": n, "arrivals": f"Poisson, {a.rate:g}/s",
+                           "sizes": "log-uniform 128^2 .. 2048x1024", "mix": "straight / network / gamma, 2.5D 50%",
+                           "slots": a.batch, "stop_test": f"error <= {eps:g}, cap 20000"},
+                   converged=conv, capped=len(res) - conv,
+                   latency_s={"p50": round(float(np.percentile(lat, 50)), 3),
+                              "p99": round(float(np.percentile(lat, 99)), 3), "basis": "arrival to completion",
+                              "queue_p50": round(float(np.percentile(queue, 50)), 3)},
+                   roofline=pcg_roofline(b), cpu_baseline=None)
+        b.close()
+        return out
+    # gamma2048: mixed against the FP64 device path (pinned to the oracle on smaller grids)
+    eps = 1e-5
+    runs = {}
+    for prec in (cb.CP_PREC_MIXED, cb.CP_PREC_FP64):
+        p, ct = wl.gamma_2048(prec)
+        if prec == cb.CP_PREC_MIXED:
+            q = served_params(p, a, eps)
+        else:
+            q = p.copy()
+            q.pressurePrecond = cb.CP_PRECOND_AMG
+            q.maxPressureIter = a.pcg
+            q.maxMomentumIter = a.jacobi
+            q.momentumTol = 0.0
+            q.pTol = 1e-12
+            q.massResidualScope = 1
+            q.epsilon = eps
+            q.maxIterations = 20000
+        b = cb.Batch(prec, 8 << 30, 1)
+        t0 = time.perf_counter()
+        r = b.serve([(q, ct, 0.0, 0.5)], a.check)[0]
+        runs[prec] = dict(r, seconds=time.perf_counter() - t0)
+        b.close()
+    m, f = runs[cb.CP_PREC_MIXED], runs[cb.CP_PREC_FP64]
+    return dict(base, value=round((1.0 if m["status"] == cb.SLOT_CONVERGED else 0.0) / m["seconds"], 4),
+                unit="converged solves/s", ms_per_step=round(m["seconds"] * 1e3, 1),
+                config={"workload": "gamma2048", "grid": "2048x2048", "alpha_max": 1e9, "two_five_d": True,
+                        "stop_test": f"error <= {eps:g}, cap 20000"},
+                mixed={k: m[k] for k in ("status", "iter", "error", "dp", "seconds")},
+                fp64_device={k: f[k] for k in ("status", "iter", "error", "dp", "seconds")},
+                dp_rel_err_vs_fp64=abs(m["dp"] / f["dp"] - 1), roofline=None, cpu_baseline=None)
+
+
 def main() -> int:
     a = parse()
     if a.gpus != 1:
@@ -211,6 +327,9 @@ def main() -> int:
     lib = cb.load()
     if lib.cp_device_count() < 1:
         raise SystemExit("no HIP device")
+    if a.workload != "network":
+        print(json.dumps(other_workload(a)), flush=True)
+        return 0
     if a.calibrate_copy:
         copy_calibration(a.calibrate_copy)
     B = a.batch

@@ -137,8 +137,10 @@ def batch_64(seed: int = 4):
 
 
 def stream_2000(n: int = 2000, seed: int = 5, max_cells: int | None = None):
-    """Request list of the served workload: (arrival index, params, cellType
-    factory). Geometry is built lazily so the list itself stays small."""
+    """Request list of the served workload: grid sizes log-uniform 128^2 ..
+    2048 x 1024, kind uniform over {parallel straight channels, channel
+    network, gamma field}, 2.5D on/off at random, U_in in 0.05-0.3. Each
+    entry is a small dict; build_request() makes its geometry."""
     rng = np.random.default_rng(seed)
     reqs = []
     for k in range(n):
@@ -157,12 +159,25 @@ def stream_2000(n: int = 2000, seed: int = 5, max_cells: int | None = None):
     return reqs
 
 
+def geom_parallel(M: int, N: int, width: int = 12, wall: int = 4) -> np.ndarray:
+    """Parallel straight microchannels across the plate: fluid bands of
+    `width` cells separated by `wall`-cell fins, solid border rows."""
+    ct = np.ones((M, N))
+    r = wall
+    while r + width <= M - wall:
+        ct[r:r + width, :] = 0.0
+        r += width + wall
+    if r == wall:  # too thin for one band with fins: one centred channel
+        lo = max(1, M // 4)
+        ct[lo:M - lo, :] = 0.0
+    return ct
+
+
 def build_request(r: dict):
     M, N = r["M"], r["N"]
     if r["kind"] == 0:
         p = base_params(M, N, 25e-6, r["U"])
-        lo = M // 4
-        ct = geom_straight(M, N, lo, M - 1 - lo)
+        ct = geom_parallel(M, N)
     elif r["kind"] == 1:
         p = network_params(M, N, r["U"])
         ct = geom_network(M, N, r["seed"])
@@ -172,3 +187,9 @@ def build_request(r: dict):
     p.Ht = 2e-4
     p.precision = CP_PREC_MIXED
     return p, ct
+
+
+def poisson_arrivals(n: int, rate: float, seed: int = 5) -> np.ndarray:
+    """Arrival times (s) of n requests of a Poisson process of `rate` per s."""
+    rng = np.random.default_rng(seed + 1)
+    return np.cumsum(rng.exponential(1.0 / rate, n))
