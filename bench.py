@@ -90,6 +90,7 @@ def parse():
                     help="no k_pcg_q timing bursts")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run 3 device-to-device copies of MIB MiB (byte-counter calibration)")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tiling (no autotuning pass)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the step's kernels directly instead of replaying the captured graph")
     ap.add_argument("--profile-steps", type=int, default=0,
@@ -342,6 +343,16 @@ def main() -> int:
     b = cb.Batch(cb.CP_PREC_MIXED, need + (64 << 20), B)
     if a.no_graph:
         b.set_graph(False)
+    tuned = None
+    if not a.no_tune:
+        # autotune the tiling on the workload's own geometries, then start
+        # every measured stream from an empty batch
+        q0 = [served_params(p, a, a.eps) for p, _ in GEOMS]
+        slots = [b.admit(q, ct) for q, (_, ct) in zip(q0[:B], GEOMS[:B])]
+        tuned = b.autotune(2)
+        for s_ in slots:
+            b.evict(s_)
+        log(f"autotuned tiling: {tuned}")
     for w in range(a.warmup):
         r = solve_stream(b, a, probe=False)
         log(f"warmup {w}: {r['elapsed']:.2f}s iters {[x['iter'] for x in r['res']]}")
@@ -411,6 +422,7 @@ def main() -> int:
             "inner_budgets": {"jacobi_sweeps": a.jacobi, "cg_per_cycle": a.pcg, "fp64_refine_cycles": a.cycles},
             "pressure_preconditioner": "aggregation-multigrid V-cycle" if a.precond == "amg" else "Jacobi",
             "serving": "cp_serve continuous batching, all requests arrived at t=0",
+            "tiling": tuned if tuned else "default (no autotuning)",
             "precision": "mixed: fp32 stencil/jacobi/cg/v-cycle, fp64 residuals/refinement/reductions",
         },
         "latency_s": {"p50": round(float(np.percentile(lat_service, 50)), 3),

@@ -173,6 +173,7 @@ EXPORTS = [
     ("cp_batch_set_graph", C.c_int, [_vp, C.c_int]),
     ("cp_batch_set_mg_tiling", C.c_int, [_vp, C.c_int]),
     ("cp_serve", C.c_int, [_vp, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    ("cp_batch_autotune", C.c_int, [_vp, C.c_int, _ip, _ip, _ip, _dp]),
     ("cp_batch_probe", C.c_int, [_vp, C.c_int]),
     ("cp_batch_probe_read", C.c_int, [_vp, _dp, _dp, C.c_int]),
     ("cp_batch_probe_burst", C.c_int, [_vp, C.c_int, C.c_int, _dp]),
@@ -371,6 +372,13 @@ class Batch:
         res = (Result * max(n, 1))()
         check(load().cp_serve(self.h, C.cast(reqs, C.c_void_p), n, steps_per_poll, C.cast(res, C.c_void_p)), self.h)
         return [{f: getattr(res[k], f) for f, _ in Result._fields_} for k in range(n)]
+
+    def autotune(self, steps: int = 2) -> dict:
+        """Time every candidate tiling on the admitted slots; keep the fastest."""
+        r, m, w = C.c_int(0), C.c_int(0), C.c_int(0)
+        ms = C.c_double(0.0)
+        check(load().cp_batch_autotune(self.h, steps, C.byref(r), C.byref(m), C.byref(w), C.byref(ms)), self.h)
+        return {"tile_rows": r.value, "mg_rows": m.value, "wg_per_cu": w.value, "ms_per_step": ms.value}
 
     def set_mg_tiling(self, rows: int) -> None:
         check(load().cp_batch_set_mg_tiling(self.h, rows), self.h)

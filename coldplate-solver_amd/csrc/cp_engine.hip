@@ -214,7 +214,7 @@ struct cp_batch {
     // tile height R (rows per wave) shared by every slot: 1, 2, 4 or 8
     int tileRows = 4;
     // workgroups per CU of the replayed graph's launches
-    int wgPerCU = 8;
+    int wgPerCU = 16;
     // rows per wave of the finest multigrid level's tiles (0: from tileRows)
     int mgTileRows = 0;
     int mgRowsEff() const {
@@ -1201,6 +1201,71 @@ int cp_batch_set_mg_tiling(cp_batch* b, int rows) {
     const int rc = rebuildTiles(b);
     if (rc) return rc;
     b->graphJ = -1;
+    return CP_OK;
+}
+
+int cp_batch_autotune(cp_batch* b, int steps, int* rows_out, int* mg_rows_out, int* wg_out, double* ms_out) {
+    if (!b) return setError(nullptr, CP_E_ARG, "null batch");
+    std::string& err = b->err;
+    if (steps < 1 || steps > 1000) return setError(&err, CP_E_ARG, "steps must be in [1, 1000]");
+    bool any = false;
+    for (int s = 0; s < b->maxSlots; ++s) any = any || (b->slots[s].used && b->hState[s].status == ST_RUNNING);
+    if (!any) return setError(&err, CP_E_STATE, "autotune needs running slots");
+    const bool f32 = b->precision == CP_PREC_MIXED;
+    const int rowsC[] = {1, 2, 4, 8};
+    const int mgC[] = {1, 2, 4};
+    const int wgC[] = {4, 8, 16};
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    double best = 1e300;
+    int bestR = b->tileRows, bestMg = b->mgTileRows, bestWg = b->wgPerCU;
+    int rc = CP_OK;
+    for (int r : rowsC) {
+        for (int m : mgC) {
+            if (m > (f32 ? 4 : 2) || m > r) continue;
+            for (int w : wgC) {
+                b->tileRows = r;
+                b->mgTileRows = m;
+                b->wgPerCU = w;
+                for (int s = 0; s < b->maxSlots; ++s)
+                    if (b->slots[s].used) setTiling(b->hDesc[s], r, b->mgRowsEff());
+                if ((rc = rebuildTiles(b))) break;
+                b->graphJ = -1;
+                if ((rc = graphRun(b, 1))) break;  // captures the graph for this shape
+                HIP_TRY(hipEventRecord(e0, b->stream));
+                if ((rc = graphRun(b, steps))) break;
+                HIP_TRY(hipEventRecord(e1, b->stream));
+                HIP_TRY(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                const double per = double(ms) / steps;
+                if (per < best) {
+                    best = per;
+                    bestR = r;
+                    bestMg = m;
+                    bestWg = w;
+                }
+            }
+            if (rc) break;
+        }
+        if (rc) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    b->tileRows = bestR;
+    b->mgTileRows = bestMg;
+    b->wgPerCU = bestWg;
+    for (int s = 0; s < b->maxSlots; ++s)
+        if (b->slots[s].used) setTiling(b->hDesc[s], bestR, b->mgRowsEff());
+    if ((rc = rebuildTiles(b))) return rc;
+    b->graphJ = -1;
+    if ((rc = pollState(b))) return rc;
+    if (rows_out) *rows_out = bestR;
+    if (mg_rows_out) *mg_rows_out = bestMg;
+    if (wg_out) *wg_out = bestWg;
+    if (ms_out) *ms_out = best;
     return CP_OK;
 }
 

@@ -279,6 +279,17 @@ int cp_batch_set_graph(cp_batch* b, int use_graph);
    0 derives it from the batch tile height. Every slot is retiled. */
 int cp_batch_set_mg_tiling(cp_batch* b, int rows);
 
+/* Autotuner (no counterpart in the reference): for every candidate tiling
+   (rows per wave R in {1, 2, 4, 8}, finest-multigrid-level rows in {1, 2, 4}
+   up to R (2 in FP64), workgroups per CU in {4, 8, 16}) replays `steps`
+   outer iterations of the admitted slots, timed with HIP events on the batch
+   stream, and keeps the fastest tiling for the batch. The replays are real
+   work: every candidate advances the running slots by steps + 1 outer
+   iterations. Tilings differ only in the summation order of reductions.
+   Outputs (any may be NULL): the chosen R, multigrid rows, workgroups per
+   CU and ms per outer iteration. */
+int cp_batch_autotune(cp_batch* b, int steps, int* rows, int* mg_rows, int* wg_per_cu, double* ms_per_step);
+
 /* Kernel of the PCG iteration timed by the measurement hooks below. */
 #define CP_K_PCG_Q        4
 

@@ -166,3 +166,32 @@ def test_timing_burst_leaves_results_unchanged():
         b.close()
     for x, y in zip(*runs):
         np.testing.assert_array_equal(x, y)
+
+
+def test_autotune_keeps_results():
+    """The autotuner picks a valid tiling; the solve then converges to the
+    same pressure drop as with the default tiling (tilings change only the
+    summation order of reductions)."""
+    p, ct = wl.network(64, 160, 27, cb.CP_PREC_MIXED)
+    q = wl.fast_mode(p, 3, 4, 1)
+    q.pressurePrecond = cb.CP_PRECOND_AMG
+    q.epsilon = 1e-9
+    q.pTol = 1e-12
+    q.maxIterations = 20000
+    say("[batch] autotune")
+    dps = []
+    for tune in (False, True):
+        b = cb.Batch(cb.CP_PREC_MIXED, 128 << 20, 2)
+        s = b.admit(q, ct)
+        if tune:
+            t = b.autotune(1)
+            assert t["tile_rows"] in (1, 2, 4, 8) and t["mg_rows"] in (1, 2, 4) and t["wg_per_cu"] in (4, 8, 16)
+            assert t["mg_rows"] <= t["tile_rows"] and t["ms_per_step"] > 0
+        for _ in range(400):
+            b.run(50)
+            if b.poll()[s]["status"] != cb.SLOT_RUNNING:
+                break
+        assert b.poll()[s]["status"] == cb.SLOT_CONVERGED
+        dps.append(b.full_dp(s))
+        b.close()
+    assert abs(dps[1] / dps[0] - 1) <= 1e-7, dps
