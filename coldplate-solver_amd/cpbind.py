@@ -65,6 +65,11 @@ class Params(C.Structure):
         ("pressurePrecond", C.c_int32),
         ("mgOmega", C.c_double),
         ("mgScale", C.c_double),
+        ("usePressureDropConvergence", C.c_int32), ("dpConvergenceWindow", C.c_int32),
+        ("dpConvergencePercent", C.c_double),
+        ("usePressureDropSlopeGate", C.c_int32), ("dpSlopeWindowIters", C.c_int32),
+        ("dpSlopeMaxDelta", C.c_double),
+        ("xPlaneCoreInlet", C.c_double), ("xPlaneCoreOutlet", C.c_double),
     ]
 
     def copy(self) -> "Params":

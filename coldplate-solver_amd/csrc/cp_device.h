@@ -70,6 +70,9 @@ struct alignas(64) SlotDesc {
     int maxIterations, inletRamp, rampSteps;
     int enableSER, serMinIter, serUseMax, enableLS, lsMaxTries, enableRamp;
     int mixed;
+    int dpConv, dpWindow;        // pressure-drop window exit and its length
+    int slopeGate, slopeWindow;  // slope gate of the window exit and its length
+    double dpPercent, slopeMaxDelta, xCoreIn, xCoreOut;
     int precond, nLevels;        // CP_PRECOND_*; multigrid levels (>= 2 when precond == 1)
     int mgTail;                  // first level handled by the one-workgroup tail kernel
     double mgOmega, mgScale;
@@ -84,6 +87,9 @@ struct alignas(64) SlotDesc {
     void* mu[M_COUNT];
     void* mv[M_COUNT];
     void* pr[P_COUNT];
+    // pressure-drop window and slope buffer (CP_DP_WINDOW_MAX each)
+    double* dpHist;
+    double* dpSlope;
     // multigrid hierarchy and the dense inverse of its coarsest level
     double* coarseInv;
     MgLevel lv[kMaxLevels];
@@ -103,7 +109,8 @@ struct alignas(256) SlotState {
     double residU, residV, residMass, error;
     double inletVelocity, pseudoCFL, serResidPrev, lsAlpha;
     int lsBacktrack;
-    int pad;
+    int dpIdx, dpFilled, dpConverged, slopeHead, slopeCount;
+    double dpLast;  // core-plane total-pressure drop after the last iteration
 };
 
 // Plane-sample request/result (samplePlaneAtX).
@@ -147,6 +154,7 @@ enum Kern {
     K_MG_FACTOR,    // multigrid: dense inverse of the coarsest level (one block per slot)
     K_MG_DOWN,      // multigrid: pre-smoothing + restriction at level a.level
     K_MG_TAIL,      // multigrid: levels mgTail.. down, coarsest solve, up (one block per slot)
+    K_DP_MONITOR,   // pressure-drop window exit after the controller (one block per slot)
     K_MG_UP,        // multigrid: prolongation + post-smoothing at level a.level
 };
 

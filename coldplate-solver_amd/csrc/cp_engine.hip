@@ -164,6 +164,7 @@ size_t slotMgBytes(int precision, int M, int N) {
     int H[kMaxLevels], W[kMaxLevels];
     const int n = mgShapes(M, N, H, W);
     size_t bytes = roundUp(sizeof(double) * kCoarseMax * kCoarseMax, 256);
+    bytes += 2 * roundUp(sizeof(double) * CP_DP_WINDOW_MAX, 256);  // pressure-drop window and slope buffer
     for (int l = 1; l < n; ++l) bytes += kMgArrays * roundUp(size_t(H[l] + 2) * mgPitch(W[l]) * tE, 256);
     return bytes;
 }
@@ -193,6 +194,7 @@ struct cp_batch {
     int* dMgNTiles = nullptr;
     int mgLevels = 0;  // deepest hierarchy of any multigrid slot admitted so far
     int mgTail = 0;    // deepest first tail level of any multigrid slot admitted so far
+    int dpAny = 0;     // some slot admitted so far uses the pressure-drop exit
     double* dPartials = nullptr;
     PlaneReq* dPlaneReq = nullptr;
     PlaneOut* dPlaneOut = nullptr;
@@ -339,6 +341,8 @@ int rebuildTiles(cp_batch* b) {
     }
     if (ts.size() > b->maxTiles) return setError(&err, CP_E_CAPACITY, "tile table full");
     b->totalTiles = int(ts.size());
+    for (int s = 0; s < b->maxSlots; ++s)
+        if (b->slots[s].used && (b->hDesc[s].dpConv || b->hDesc[s].slopeGate)) b->dpAny = 1;
     int counts[kMaxLevels];
     std::vector<std::vector<int>> lists(kMaxLevels);  // alive until the stream is synchronised
     // mgLevels never decreases: the captured step graph keeps its shape as
@@ -427,6 +431,10 @@ int validate(const cp_params* p, std::string* err) {
     if (p->pressurePrecond == CP_PRECOND_AMG &&
         (!(p->mgOmega > 0) || !(p->mgOmega <= 1) || !(p->mgScale > 0) || !(p->mgScale <= 4)))
         return setError(err, CP_E_ARG, "mgOmega must be in (0, 1] and mgScale in (0, 4]");
+    if (p->usePressureDropConvergence && !(p->dpConvergenceWindow >= 1 && p->dpConvergenceWindow <= CP_DP_WINDOW_MAX))
+        return setError(err, CP_E_ARG, "dpConvergenceWindow must be in [1, %d]", CP_DP_WINDOW_MAX);
+    if (p->usePressureDropSlopeGate && !(p->dpSlopeWindowIters >= 0 && p->dpSlopeWindowIters <= CP_DP_WINDOW_MAX))
+        return setError(err, CP_E_ARG, "dpSlopeWindowIters must be in [0, %d]", CP_DP_WINDOW_MAX);
     if (p->pressurePrecond == CP_PRECOND_AMG) {
         int H[kMaxLevels], W[kMaxLevels];
         const int n = mgShapes(p->M, p->N, H, W);
@@ -467,6 +475,14 @@ SlotDesc makeDesc(const cp_params& p, int precision, int tileRows, int mgRows0) 
     d.N = p.N;
     d.pitch = int(roundUp(size_t(p.N) + 1, 32));
     d.mixed = precision == CP_PREC_MIXED ? 1 : 0;
+    d.dpConv = p.usePressureDropConvergence ? 1 : 0;
+    d.dpWindow = p.dpConvergenceWindow;
+    d.slopeGate = p.usePressureDropSlopeGate ? 1 : 0;
+    d.slopeWindow = p.dpSlopeWindowIters;
+    d.dpPercent = p.dpConvergencePercent;
+    d.slopeMaxDelta = p.dpSlopeMaxDelta;
+    d.xCoreIn = p.xPlaneCoreInlet;
+    d.xCoreOut = p.xPlaneCoreOutlet;
     d.precond = p.pressurePrecond;
     d.mgOmega = p.mgOmega;
     d.mgScale = p.mgScale;
@@ -569,6 +585,10 @@ void assignPointers(cp_batch* b, SlotDesc& d, char* base) {
     char* mp = base + b->mainBytes(d.M, d.N);
     d.coarseInv = reinterpret_cast<double*>(mp);
     mp += roundUp(sizeof(double) * kCoarseMax * kCoarseMax, 256);
+    d.dpHist = reinterpret_cast<double*>(mp);
+    mp += roundUp(sizeof(double) * CP_DP_WINDOW_MAX, 256);
+    d.dpSlope = reinterpret_cast<double*>(mp);
+    mp += roundUp(sizeof(double) * CP_DP_WINDOW_MAX, 256);
     if (d.nLevels > 0) {
         MgLevel& L0 = d.lv[0];
         L0.aP = d.pr[P_AP];
@@ -819,11 +839,21 @@ int polledPressure(cp_batch* b) {
     return CP_OK;
 }
 
+// Pressure-drop window exit of every slot, after the controller.
+int dpMonitor(cp_batch* b) {
+    if (!b->dpAny) return CP_OK;
+    const hipError_t e =
+        launchKernel(K_DP_MONITOR, b->precision == CP_PREC_MIXED, 1, b->maxSlots, b->maxSlots, b->stream, kargs(b));
+    if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+    return CP_OK;
+}
+
 int polledStep(cp_batch* b, bool outer) {
     int rc = polledMomentum(b);
     if (rc) return rc;
     if ((rc = polledPressure(b))) return rc;
-    return launch(b, outer ? K_CORR_OUTER : K_CORR, gridFor(b, b->totalTiles));
+    if ((rc = launch(b, outer ? K_CORR_OUTER : K_CORR, gridFor(b, b->totalTiles)))) return rc;
+    return outer ? dpMonitor(b) : CP_OK;
 }
 
 // ---- graph mode (fixed inner budgets) -------------------------------------
@@ -850,7 +880,8 @@ int enqueueFixedStep(cp_batch* b, int J, int P, int R, int grid) {
             if ((rc = mgVcycle(b))) return rc;
         }
     }
-    return launch(b, K_CORR_OUTER, grid);
+    if ((rc = launch(b, K_CORR_OUTER, grid))) return rc;
+    return dpMonitor(b);
 }
 
 int graphRun(cp_batch* b, int steps) {
@@ -867,7 +898,7 @@ int graphRun(cp_batch* b, int steps) {
         }
         return CP_OK;
     }
-    const int shapeL = b->mgLevels * (kMaxLevels + 1) + b->mgTail;
+    const int shapeL = (b->mgLevels * (kMaxLevels + 1) + b->mgTail) * 2 + b->dpAny;
     if (!b->graphExec || J != b->graphJ || P != b->graphP || R != b->graphR || shapeL != b->graphL) {
         if (b->graphExec) {
             HIP_TRY(hipGraphExecDestroy(b->graphExec));

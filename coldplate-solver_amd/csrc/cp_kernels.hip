@@ -1199,11 +1199,11 @@ __global__ __launch_bounds__(kBlock) void k_correct(KArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// samplePlaneAtX (postprocessing.cpp:53-231): one block per request.
+// samplePlaneAtX (postprocessing.cpp:53-231) by one workgroup; thread 0
+// writes the result, which every thread may read after the call.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_plane(const SlotDesc* desc, const PlaneReq* req, PlaneOut* out) {
-    const PlaneReq rq = req[blockIdx.x];
-    const SlotDesc& d = desc[rq.slot];
+__device__ void planeBlock(const SlotDesc& d, double xq, double gammaMin, PlaneOut* result) {
+    const PlaneReq rq = {0, xq, gammaMin};
     const int M = d.M, N = d.N, P = d.pitch;
     const double hx = d.hx, hy = d.hy, rho = d.rho;
     const double xMin = 0.5 * hx, xMax = (double(N) - 0.5) * hx;
@@ -1264,7 +1264,100 @@ __global__ __launch_bounds__(kBlock) void k_plane(const SlotDesc* desc, const Pl
         }
         if (fs > 0.0) o.avgTotal = res[2] / fs;
         else if (area > 0.0) o.avgTotal = o.avgStatic + o.avgDynamic;
-        out[blockIdx.x] = o;
+        *result = o;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kBlock) void k_plane(const SlotDesc* desc, const PlaneReq* req, PlaneOut* out) {
+    const PlaneReq rq = req[blockIdx.x];
+    __shared__ PlaneOut o;
+    planeBlock(desc[rq.slot], rq.x, rq.gammaMin, &o);
+    if (threadIdx.x == 0) out[blockIdx.x] = o;
+}
+
+// ---------------------------------------------------------------------------
+// Pressure-drop window exit (SIMPLE.cpp:338-409), one workgroup per slot,
+// after the controller: the core-plane total-pressure drop of the iteration
+// just completed enters the slope buffer and, from iteration rampSteps on,
+// the moving window; a full window whose standard deviation and latest
+// deviation from the mean are below dpPercent % of max(|mean|, 1) (and
+// whose slope passes the gate) ends the run as converged.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_dp_monitor(KArgs a) {
+    const int s = blockIdx.x;
+    const SlotDesc& d = a.desc[s];
+    SlotState& st = a.state[s];
+    const bool gate = d.slopeGate && d.slopeWindow > 0;
+    if (d.ntiles == 0 || st.status != ST_RUNNING || (!d.dpConv && !gate)) return;
+    __shared__ PlaneOut po[2];
+    __shared__ int check;
+    __shared__ double stat[2];
+    planeBlock(d, d.xCoreIn, 1.0 - 1e-6, &po[0]);
+    planeBlock(d, d.xCoreOut, 1.0 - 1e-6, &po[1]);
+    const int it = st.iter - 1;  // index of the iteration just completed
+    const int W = d.dpWindow, SW = d.slopeWindow;
+    if (threadIdx.x == 0) {
+        const double dp = po[0].avgTotal - po[1].avgTotal;
+        st.dpLast = dp;
+        if (gate) {
+            if (st.slopeCount < SW) {
+                G(d.dpSlope)[(st.slopeHead + st.slopeCount) % SW] = dp;
+                st.slopeCount += 1;
+            } else {
+                G(d.dpSlope)[st.slopeHead] = dp;  // drop the oldest
+                st.slopeHead = (st.slopeHead + 1) % SW;
+            }
+        }
+        check = 0;
+        if (d.dpConv && it >= d.rampSteps) {
+            G(d.dpHist)[st.dpIdx] = dp;
+            st.dpIdx = (st.dpIdx + 1) % W;
+            if (st.dpIdx == 0) st.dpFilled = 1;
+            check = st.dpFilled;
+        }
+    }
+    __syncthreads();
+    if (!check) return;
+    double v[1] = {0.0};
+    for (int k = int(threadIdx.x); k < W; k += kBlock) v[0] += G(d.dpHist)[k];
+    blockReduceTo<1, 0u>(v, &stat[0]);
+    __syncthreads();
+    const double mean = stat[0] / double(W);
+    double q[1] = {0.0};
+    for (int k = int(threadIdx.x); k < W; k += kBlock) {
+        const double df = G(d.dpHist)[k] - mean;
+        q[0] += df * df;
+    }
+    blockReduceTo<1, 0u>(q, &stat[1]);
+    __syncthreads();
+    bool slopeOK = true;
+    if (gate && st.slopeCount >= SW) {
+        // least-squares slope over the buffer, oldest first at x = 0
+        __shared__ double ls[4];
+        double w4[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int k = int(threadIdx.x); k < st.slopeCount; k += kBlock) {
+            const double x = double(k), y = G(d.dpSlope)[(st.slopeHead + k) % SW];
+            w4[0] += x;
+            w4[1] += y;
+            w4[2] += x * x;
+            w4[3] += x * y;
+        }
+        blockReduceTo<4, 0u>(w4, ls);
+        __syncthreads();
+        const double n = double(st.slopeCount);
+        const double den = n * ls[2] - ls[0] * ls[0];
+        const double slope = fabs(den) < 1e-12 ? 0.0 : (n * ls[3] - ls[0] * ls[1]) / den;
+        slopeOK = fabs(slope) <= d.slopeMaxDelta / double(SW);
+    }
+    if (threadIdx.x == 0) {
+        const double sd = sqrt(stat[1] / double(W));
+        const double den = fmax(fabs(mean), 1.0);
+        const double latest = G(d.dpHist)[(st.dpIdx - 1 + W) % W];
+        if (sd / den * 100.0 < d.dpPercent && fabs(latest - mean) / den * 100.0 < d.dpPercent && slopeOK) {
+            st.dpConverged = 1;
+            st.status = ST_CONVERGED;
+        }
     }
 }
 
@@ -1468,6 +1561,7 @@ static void launchTR(Kern k, int grid, hipStream_t s, const KArgs& a) {
             break;
         case K_MG_FACTOR: hipLaunchKernelGGL(k_mg_factor, g, b, 0, s, a); break;
         case K_MG_TAIL: hipLaunchKernelGGL((k_mg_tail<T>), g, dim3(kTailBlock), 0, s, a); break;
+        case K_DP_MONITOR: hipLaunchKernelGGL(k_dp_monitor, g, b, 0, s, a); break;
     }
 }
 

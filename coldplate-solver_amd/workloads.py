@@ -38,6 +38,9 @@ def base_params(M: int, N: int, h: float, U: float) -> Params:
     p.uvAlpha, p.pAlpha = 0.7, 0.3
     p.epsilon = 1e-6
     p.maxIterations = 200000
+    # the configurations converge on the residual stop test alone
+    # (usePressureDropConvergence = false)
+    p.usePressureDropConvergence = 0
     return p
 
 

@@ -39,6 +39,11 @@ static inline void cp_fill_reference_defaults(cp_params* p) {
     p->pressurePrecond = 0;
     p->mgOmega = 0.8;
     p->mgScale = 1.8;
+    p->usePressureDropConvergence = 1; p->dpConvergenceWindow = 500;    /* SIMPLE.h:98-99 */
+    p->dpConvergencePercent = 1.0;                                       /* SIMPLE.h:100 */
+    p->usePressureDropSlopeGate = 0; p->dpSlopeWindowIters = 1000;      /* SIMPLE.h:101-102 */
+    p->dpSlopeMaxDelta = 10.0;                                           /* SIMPLE.h:103 */
+    p->xPlaneCoreInlet = 0.01; p->xPlaneCoreOutlet = 0.050;             /* SIMPLE.h:88-89 */
 }
 
 #endif /* COLDPLATE_DEFAULTS_H */

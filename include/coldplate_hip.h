@@ -127,7 +127,23 @@ typedef struct cp_params {
        unchanged. */
     int32_t pressurePrecond;
     double  mgOmega, mgScale;
+    /* Pressure-drop convergence (SIMPLE.h:88-102, SIMPLE.cpp:356-409): after
+       iteration rampSteps, the core-plane total-pressure drop between x =
+       xPlaneCoreInlet and x = xPlaneCoreOutlet enters a moving window of
+       dpConvergenceWindow iterations; the run stops when the window's
+       standard deviation and its latest value's distance from the mean are
+       both below dpConvergencePercent % of |mean| (at least 1), and, with
+       the slope gate, the least-squares slope of the last dpSlopeWindowIters
+       drops is at most dpSlopeMaxDelta / dpSlopeWindowIters. Windows hold
+       at most CP_DP_WINDOW_MAX iterations. */
+    int32_t usePressureDropConvergence, dpConvergenceWindow;
+    double  dpConvergencePercent;
+    int32_t usePressureDropSlopeGate, dpSlopeWindowIters;
+    double  dpSlopeMaxDelta;
+    double  xPlaneCoreInlet, xPlaneCoreOutlet;
 } cp_params;
+
+#define CP_DP_WINDOW_MAX 4096
 
 /* Statistics of one outer iteration (one pass of SIMPLE.cpp:299-439). */
 typedef struct cp_step_stats {

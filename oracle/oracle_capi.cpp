@@ -43,7 +43,7 @@ void fillStats(const S& s, cp_step_stats* st) {
     st->itV = s.lastItV;
     st->itP = s.lastItP;
     st->iter = s.iter;
-    st->converged = (s.error <= s.epsilon) ? 1 : 0;
+    st->converged = (s.error <= s.epsilon || s.dpConverged) ? 1 : 0;
     st->health = 0;
 }
 
@@ -128,7 +128,7 @@ void or_destroy(void* hp) {
 
 int or_step(void* hp, cp_step_stats* st) {
     Handle* h = static_cast<Handle*>(hp);
-    DISPATCH(h, { s.outerStep(); fillStats(s, st); });
+    DISPATCH(h, { s.loopStep(); fillStats(s, st); });
     return 0;
 }
 

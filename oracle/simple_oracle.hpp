@@ -841,11 +841,83 @@ struct Simple {
         return error;
     }
 
-    // runIterations loop condition (SIMPLE.cpp:299), Δp-window exit off.
+    // Pressure-drop convergence state (SIMPLE.cpp:266-294, 356-409).
+    std::vector<T> dpHistory;
+    int dpHistoryIdx = 0;
+    bool dpHistoryFilled = false;
+    bool dpConverged = false;
+    std::vector<std::pair<int, T>> dpSlope;  // (iteration, core drop), oldest first
+
+    // Least-squares slope of the core drop over the slope window
+    // (SIMPLE.cpp:275-294).
+    T dpSlopeValue() const {
+        if (dpSlope.size() < 2) return T(0);
+        const T x0 = T(dpSlope.front().first);
+        T sx = 0, sy = 0, sxx = 0, sxy = 0;
+        const T n = T(dpSlope.size());
+        for (const auto& e : dpSlope) {
+            const T x = T(e.first) - x0, y = e.second;
+            sx += x;
+            sy += y;
+            sxx += x * x;
+            sxy += x * y;
+        }
+        const T den = n * sxx - sx * sx;
+        if (std::abs(den) < T(1e-12)) return T(0);
+        return (n * sxy - sx * sy) / den;
+    }
+
+    // After outer iteration `it` (0-based, before the loop increments it):
+    // core-plane total-pressure drop into the slope buffer and, from
+    // iteration rampSteps on, into the moving window (SIMPLE.cpp:338-409).
+    void monitorPressureDrop(int it) {
+        const bool gate = prm.usePressureDropSlopeGate && prm.dpSlopeWindowIters > 0;
+        if (!prm.usePressureDropConvergence && !gate) return;
+        const T gmin = T(1) - T(1e-6);
+        const T dp = plane(T(prm.xPlaneCoreInlet), gmin).avgTotal - plane(T(prm.xPlaneCoreOutlet), gmin).avgTotal;
+        if (gate) {
+            dpSlope.emplace_back(it, dp);
+            if (int(dpSlope.size()) > prm.dpSlopeWindowIters) dpSlope.erase(dpSlope.begin());
+        }
+        if (!prm.usePressureDropConvergence || it < prm.rampSteps) return;
+        const int W = prm.dpConvergenceWindow;
+        if (int(dpHistory.size()) != W) dpHistory.assign(W, T(0));
+        dpHistory[dpHistoryIdx] = dp;
+        dpHistoryIdx = (dpHistoryIdx + 1) % W;
+        if (dpHistoryIdx == 0) dpHistoryFilled = true;
+        if (!dpHistoryFilled) return;
+        double sum = 0.0;
+        for (int k = 0; k < W; ++k) sum += double(dpHistory[k]);
+        const double mean = sum / double(W);
+        double var = 0.0;
+        for (int k = 0; k < W; ++k) {
+            const double df = double(dpHistory[k]) - mean;
+            var += df * df;
+        }
+        const double sd = std::sqrt(var / double(W));
+        const double den = std::max(std::abs(mean), 1.0);
+        const double sdPct = sd / den * 100.0;
+        const T latest = dpHistory[(dpHistoryIdx - 1 + W) % W];
+        const double latestPct = std::abs(double(latest) - mean) / den * 100.0;
+        bool slopeOK = true;
+        if (gate && int(dpSlope.size()) >= prm.dpSlopeWindowIters)
+            slopeOK = std::abs(dpSlopeValue()) <= T(prm.dpSlopeMaxDelta) / T(prm.dpSlopeWindowIters);
+        if (sdPct < prm.dpConvergencePercent && latestPct < prm.dpConvergencePercent && slopeOK) dpConverged = true;
+    }
+
+    // One pass of the runIterations loop body (SIMPLE.cpp:300-436, no I/O).
+    T loopStep() {
+        const int it = iter;
+        outerStep();
+        monitorPressureDrop(it);
+        return error;
+    }
+
+    // runIterations loop condition (SIMPLE.cpp:299).
     int run(int maxIter) {
         int n = 0;
-        while (iter < prm.maxIterations && error > epsilon && n < maxIter) {
-            outerStep();
+        while (iter < prm.maxIterations && error > epsilon && !dpConverged && n < maxIter) {
+            loopStep();
             n++;
         }
         return n;

@@ -204,3 +204,39 @@ def test_fp64_health_flag_on_nonfinite():
     assert st["health"] != 0
     with pytest.raises(cb.CpError):
         g.step()
+
+
+@pytest.mark.parametrize("gate", [False, True], ids=["window", "window+slope"])
+def test_pressure_drop_exit_matches_oracle(gate):
+    """The pressure-drop window exit (and its slope gate) stops the device
+    run at the oracle's iteration, in the polled single-geometry context and
+    in a graph-replayed batch."""
+    p = wl.base_params(20, 60, 5e-5, 0.1)
+    ct = wl.geom_straight(20, 60, 4, 15)
+    p.epsilon = 1e-14
+    p.maxIterations = 3000
+    p.usePressureDropConvergence = 1
+    p.dpConvergenceWindow = 30
+    p.dpConvergencePercent = 1.0
+    p.rampSteps = 10
+    p.usePressureDropSlopeGate = 1 if gate else 0
+    p.dpSlopeWindowIters = 80
+    p.dpSlopeMaxDelta = 0.5
+    p.xPlaneCoreInlet, p.xPlaneCoreOutlet = 0.5e-3, 2.5e-3
+    say(f"[parity] pressure-drop exit, gate {gate}")
+    o = op.Oracle(p, ct)
+    n, so, _ = o.run(3000)
+    assert so["converged"] == 1 and so["iter"] < 3000
+    g = cb.Solver(p, ct)
+    sg = g.run()
+    assert sg["converged"] == 1 and sg["iter"] == so["iter"], (sg, so)
+    assert rel_err(g.field(cb.F_P), o.field(cb.F_P)) <= 1e-9
+    g.close()
+    b = cb.Batch(cb.CP_PREC_FP64, 64 << 20, 2)
+    b.set_tiling(1)
+    s = b.admit(p, ct)
+    while b.poll()[s]["status"] == cb.SLOT_RUNNING:
+        b.run(7)
+    info = b.poll()[s]
+    assert info["status"] == cb.SLOT_CONVERGED and info["iter"] == so["iter"], info
+    b.close()

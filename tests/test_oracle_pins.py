@@ -31,6 +31,7 @@ def straight_params(M=50, N=200, U=0.1):
     p.uvAlpha, p.pAlpha = 0.7, 0.3
     p.epsilon = 1e-6
     p.maxIterations = 100000
+    p.usePressureDropConvergence = 0  # residual stop test only
     return p
 
 
