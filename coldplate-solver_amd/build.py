@@ -45,9 +45,10 @@ def build_lib() -> Path:
         obj = OBJ_DIR / (src + ".o")
         run([HIPCC, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(obj)
-    gobj = OBJ_DIR / "cp_geometry.o"
-    run(["g++", *HOST_FLAGS, "-c", str(CSRC / "cp_geometry.cpp"), "-o", str(gobj)])
-    objs.append(gobj)
+    for src in ("cp_geometry.cpp", "cp_io.cpp"):
+        obj = OBJ_DIR / (Path(src).stem + ".o")
+        run(["g++", *HOST_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
+        objs.append(obj)
     tmp = LIB.with_suffix(".so.tmp")
     run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-lgomp", "-o", str(tmp)])
     os.replace(tmp, LIB)

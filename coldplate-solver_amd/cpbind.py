@@ -179,6 +179,12 @@ EXPORTS = [
     ("cp_batch_set_mg_tiling", C.c_int, [_vp, C.c_int]),
     ("cp_serve", C.c_int, [_vp, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     ("cp_batch_autotune", C.c_int, [_vp, C.c_int, _ip, _ip, _ip, _dp]),
+    ("cp_io_read_params", C.c_int, [C.c_char_p, C.POINTER(Params), _ip, _ip]),
+    ("cp_io_read_geometry", C.c_int, [C.c_char_p, C.c_int, C.c_int, _dp]),
+    ("cp_io_read_matrix", C.c_int, [C.c_char_p, C.c_int, C.c_int, _dp]),
+    ("cp_io_write_matrix", C.c_int, [C.c_char_p, _dp, C.c_int, C.c_int, C.c_int]),
+    ("cp_io_save_all", C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                 _dp, _dp, _dp, _dp, _dp, _dp, C.c_int]),
     ("cp_batch_probe", C.c_int, [_vp, C.c_int]),
     ("cp_batch_probe_read", C.c_int, [_vp, _dp, _dp, C.c_int]),
     ("cp_batch_probe_burst", C.c_int, [_vp, C.c_int, C.c_int, _dp]),
@@ -287,6 +293,19 @@ class Solver:
         m = PlaneMetrics()
         check(load().cp_plane(self.h, x, gamma_min, C.byref(m)), self.h)
         return {k: getattr(m, k) for k, _ in m._fields_}
+
+    def save_all(self, directory, cell_type: np.ndarray, n_in: int = 0, n_out: int = 0, precision: int = 0) -> None:
+        """saveAll's files (u/v/p, cell-centred, thermal crops, VTK)."""
+        save_all(directory, self.prm, self.field(F_U), self.field(F_V), self.field(F_P), cell_type,
+                 self.field(F_GAMMA), self.field(F_ALPHA), n_in, n_out, precision)
+
+    @classmethod
+    def from_files(cls, params_path, geometry_path, base: "Params | None" = None) -> "tuple[Solver, np.ndarray]":
+        """A solve from the reference's fluid_params.txt / geometry_fluid.txt
+        (other parameters from `base`, else the reference defaults)."""
+        prm, _, _ = read_params(params_path, base)
+        ct, _ = read_geometry(geometry_path, prm.M, prm.N)
+        return cls(prm, ct), ct
 
     def full_dp(self, gamma_min: float = 1.0 - 1e-6) -> float:
         """Full-system total-pressure drop (SIMPLE.cpp:343-349)."""
@@ -409,3 +428,42 @@ class Batch:
     def evict(self, slot: int) -> None:
         check(load().cp_batch_evict(self.h, slot), self.h)
         self.shapes.pop(slot, None)
+
+
+# ---------------------------------------------------------------------------
+# the reference's file formats
+# ---------------------------------------------------------------------------
+def read_params(path, base: Params | None = None) -> tuple[Params, int, int]:
+    """fluid_params.txt -> (params, N_in_buffer, N_out_buffer)."""
+    p = base.copy() if base is not None else default_params()
+    ni, no = C.c_int32(0), C.c_int32(0)
+    check(load().cp_io_read_params(str(path).encode(), C.byref(p), C.byref(ni), C.byref(no)))
+    return p, ni.value, no.value
+
+
+def read_geometry(path, M: int, N: int) -> tuple[np.ndarray, bool]:
+    """geometry_fluid.txt -> (cellType M x N, file_found)."""
+    ct = np.zeros((M, N))
+    rc = load().cp_io_read_geometry(str(path).encode(), M, N, dptr(ct))
+    if rc < 0:
+        check(rc)
+    return ct, rc == 0
+
+
+def read_matrix(path, rows: int, cols: int) -> np.ndarray:
+    a = np.zeros((rows, cols))
+    check(load().cp_io_read_matrix(str(path).encode(), rows, cols, dptr(a)))
+    return a
+
+
+def write_matrix(path, a: np.ndarray, precision: int = 0) -> None:
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    check(load().cp_io_write_matrix(str(path).encode(), dptr(a), a.shape[0], a.shape[1], precision))
+
+
+def save_all(directory, prm: Params, u, v, p, cell_type, gamma, alpha, n_in: int = 0, n_out: int = 0,
+             precision: int = 0) -> None:
+    """saveAll's files for fields in the reference shapes."""
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (u, v, p, cell_type, gamma, alpha)]
+    check(load().cp_io_save_all(str(directory).encode(), prm.M, prm.N, prm.hx, prm.hy, n_in, n_out,
+                                 *(dptr(x) for x in arrs), precision))

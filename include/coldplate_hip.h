@@ -358,6 +358,31 @@ typedef struct cp_result {
 int cp_serve(cp_batch* b, const cp_request* reqs, int n, int steps_per_poll, cp_result* results);
 
 /* ------------------------------------------------------------------ */
+/* the reference's file formats (host only, no device needed)          */
+/* ------------------------------------------------------------------ */
+/* Parameter file "M N hy hx U_in N_in_buffer N_out_buffer [Ht [rho nu]]"
+   (SIMPLE.cpp:31-60): sets M, N, hy, hx, U_in and, when present, Ht of p;
+   rho and nu columns are ignored as the reference ignores them. */
+int cp_io_read_params(const char* path, cp_params* p, int* n_in_buffer, int* n_out_buffer);
+/* Geometry file: M x N whitespace-separated cellType values, row-major
+   (SIMPLE.cpp:517-551). Returns 1 (all fluid) when the file is missing;
+   values missing at the end stay fluid. */
+int cp_io_read_geometry(const char* path, int M, int N, double* cellType);
+/* Whitespace-separated rows x cols matrix, row-major (the restart files
+   u.txt, v.txt, p.txt: SIMPLE.cpp:203-232); CP_E_ARG when incomplete. */
+int cp_io_read_matrix(const char* path, int rows, int cols, double* a);
+/* Tab-separated rows, as saveMatrix (output.cpp:47-71). precision 0 writes
+   the reference's 6 significant digits; 17 round-trips doubles. */
+int cp_io_write_matrix(const char* path, const double* a, int rows, int cols, int precision);
+/* saveAll (output.cpp:73-266) into `dir`: u/v/p.txt (staggered),
+   u_full/v_full/pressure_full.txt (cell-centred), u_thermal/v_thermal/
+   pressure_thermal/p_gradient.txt (buffers cropped) and fluid_results.vtk.
+   gamma and alpha are the M x N fields (cp_get_field CP_F_GAMMA / _ALPHA). */
+int cp_io_save_all(const char* dir, int M, int N, double hx, double hy, int n_in_buffer, int n_out_buffer,
+                   const double* u, const double* v, const double* p, const double* cellType, const double* gamma,
+                   const double* alpha, int precision);
+
+/* ------------------------------------------------------------------ */
 /* seeded geometry generators (cellType, M x N row-major)              */
 /* ------------------------------------------------------------------ */
 /* The reference ships Python generators (GeometryGeneratorV1.py ...

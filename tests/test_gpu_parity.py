@@ -240,3 +240,33 @@ def test_pressure_drop_exit_matches_oracle(gate):
     info = b.poll()[s]
     assert info["status"] == cb.SLOT_CONVERGED and info["iter"] == so["iter"], info
     b.close()
+
+
+def test_reference_files_and_restart(tmp_path):
+    """A solve set up from the reference's input files, saved in its output
+    format with 17 digits, restarts from u/v/p.txt and continues exactly as
+    the uninterrupted solve."""
+    (tmp_path / "fluid_params.txt").write_text("20 60 5e-05 5e-05 0.1 5 5\n")
+    ct = wl.geom_straight(20, 60, 4, 15)
+    cb.write_matrix(tmp_path / "geometry_fluid.txt", ct)
+    base = wl.base_params(20, 60, 5e-5, 0.1)
+    g, ct2 = cb.Solver.from_files(tmp_path / "fluid_params.txt", tmp_path / "geometry_fluid.txt", base)
+    np.testing.assert_array_equal(ct2, ct)
+    say("[parity] files + restart")
+    for _ in range(30):
+        g.step()
+    out = tmp_path / "out"
+    out.mkdir()
+    g.save_all(out, ct, 5, 5, precision=17)
+    for _ in range(10):
+        g.step()
+    r = cb.Solver(base, ct)
+    r.set_fields(cb.read_matrix(out / "u.txt", 21, 60), cb.read_matrix(out / "v.txt", 20, 61),
+                 cb.read_matrix(out / "p.txt", 21, 61))
+    for _ in range(10):
+        r.step()
+    for f in (cb.F_U, cb.F_V, cb.F_P):
+        np.testing.assert_array_equal(r.field(f), g.field(f))
+    assert (out / "fluid_results.vtk").exists() and (out / "u_thermal.txt").exists()
+    g.close()
+    r.close()
