@@ -510,62 +510,95 @@ __device__ __forceinline__ T pAt(gp<const T> r, gp<const T> aP, gp<const T> Z, g
     return kk == 0 ? z : z + beta * pold[k];
 }
 
-// q = A p_k on LDS halo tiles: the workgroup forms p_k = z + beta p_{k-1}
-// (z = r / aP) once per node of its (4R+2) x 66 halo region, stages the east
-// couplings with their west neighbour and the north couplings with their
-// south neighbour, then each thread computes its R nodes from LDS.
+// Row-major index of (i, j) clamped into the (M+1) x P array: lanes outside
+// the unknowns load from a valid address and their values are discarded, so
+// every load of a tile is issued unconditionally, ahead of any use.
+__device__ __forceinline__ size_t clampIdx(int i, int j, int M, int P) {
+    return size_t(min(max(i, 0), M)) * P + min(max(j, 0), P - 1);
+}
+
+template <typename T, bool HasZ, bool First>
+__device__ __forceinline__ T pLoad(gp<const T> r, gp<const T> aP, gp<const T> Z, gp<const T> pold, size_t k,
+                                   T beta) {
+    const T z = HasZ ? Z[k] : r[k] / aP[k];
+    return First ? z : z + beta * pold[k];
+}
+
+// q = A p_k for one wave's R rows x 64 columns. The wave loads p_k for its
+// rows and for the rows above and below them (lanes 0-31 also at column
+// j0-1, lanes 32-63 at column j0+64), then marches down the rows taking the
+// west and east neighbours from the adjacent lanes.
+template <typename T, int R, bool HasZ, bool First>
+__device__ __forceinline__ void pcgQWave(const SlotDesc& d, const Tile& tl, int kk, T beta, double& v) {
+    const int M = d.M, N = d.N, P = d.pitch;
+    const auto r = G(static_cast<const T*>(d.pr[P_R]));
+    const auto aP = G(static_cast<const T*>(d.pr[P_AP]));
+    const auto cE = G(static_cast<const T*>(d.pr[P_CE]));
+    const auto cN = G(static_cast<const T*>(d.pr[P_CN]));
+    const gp<const T> Z = HasZ ? G(static_cast<const T*>(d.pr[P_Z])) : gp<const T>(nullptr);
+    const auto pold = G(static_cast<const T*>(d.pr[P_PK0 + (kk & 1)]));
+    const auto pnew = G(static_cast<T*>(d.pr[P_PK0 + ((kk & 1) ^ 1)]));
+    const auto Q = G(static_cast<T*>(d.pr[P_Q]));
+    const int lane = int(threadIdx.x & 63);
+    const int j = tl.j, ib = tl.i;
+    const int je = lane < 32 ? tl.j0 - 1 : tl.j0 + kTileW;
+    T p[R + 2], pe[R + 2], ap[R], ce[R], cee[R], cn[R + 1];
+#pragma unroll
+    for (int h = 0; h < R + 2; ++h) {
+        const int i = ib - 1 + h;
+        const T pc = pLoad<T, HasZ, First>(r, aP, Z, pold, clampIdx(i, j, M, P), beta);
+        const T px = pLoad<T, HasZ, First>(r, aP, Z, pold, clampIdx(i, je, M, P), beta);
+        p[h] = isPDof(i, j, M, N) ? pc : T(0);
+        pe[h] = isPDof(i, je, M, N) ? px : T(0);
+    }
+#pragma unroll
+    for (int rr = 0; rr < R; ++rr) {
+        const size_t k = clampIdx(ib + rr, j, M, P);
+        ap[rr] = aP[k];
+        ce[rr] = cE[k];
+        cee[rr] = cE[clampIdx(ib + rr, je, M, P)];
+    }
+#pragma unroll
+    for (int h = 0; h < R + 1; ++h) cn[h] = cN[clampIdx(ib - 1 + h, j, M, P)];
+#pragma unroll
+    for (int rr = 0; rr < R; ++rr) {
+        const int i = ib + rr;
+        const T pC = p[rr + 1];
+        T pW = __shfl_up(pC, 1, 64), pE = __shfl_down(pC, 1, 64), cEw = __shfl_up(ce[rr], 1, 64);
+        pW = lane == 0 ? pe[rr + 1] : pW;
+        cEw = lane == 0 ? cee[rr] : cEw;
+        pE = lane == 63 ? pe[rr + 1] : pE;
+        // CSR column order S, W, P, E, N (pressure_solver.cpp:108-118)
+        const T q = -cn[rr] * p[rr] - cEw * pW + ap[rr] * pC - ce[rr] * pE - cn[rr + 1] * p[rr + 2];
+        if (isPDof(i, j, M, N)) {
+            const size_t k = size_t(i) * P + j;
+            pnew[k] = pC;
+            Q[k] = q;
+            v += double(pC) * double(q);
+        }
+    }
+}
+
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_pcg_q(KArgs a) {
-    constexpr int TH = kTileH * R, HH = TH + 2, HW = kTileW + 2;
-    __shared__ T sp[HH][HW];
-    __shared__ T scE[TH][kTileW + 1];
-    __shared__ T scN[TH + 1][kTileW];
     const int nT = *a.nTiles;
     XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
-        if (st.status != ST_RUNNING || ((st.pdone || st.stepPhase) && !a.force)) continue;
-        const int M = d.M, N = d.N, P = d.pitch;
-        const int kk = st.pk;
+        // read the gate fields together and test them without short-circuit
+        // branches, so their loads overlap
+        const int status = st.status, pdone = st.pdone, phase = st.stepPhase, kk = st.pk;
+        if ((status != ST_RUNNING) | ((pdone | phase) != 0 && !a.force) | (kk < 0)) continue;
         const T beta = T(st.beta);
-        const auto r = G(static_cast<const T*>(d.pr[P_R]));
-        const auto aP = G(static_cast<const T*>(d.pr[P_AP]));
-        const auto cE = G(static_cast<const T*>(d.pr[P_CE]));
-        const auto cN = G(static_cast<const T*>(d.pr[P_CN]));
-        const auto pold = G(static_cast<const T*>(d.pr[P_PK0 + (kk & 1)]));
-        const gp<const T> Z = d.precond ? G(static_cast<const T*>(d.pr[P_Z])) : gp<const T>(nullptr);
-        const auto pnew = G(static_cast<T*>(d.pr[P_PK0 + ((kk & 1) ^ 1)]));
-        const auto Q = G(static_cast<T*>(d.pr[P_Q]));
-        __syncthreads();  // the previous tile's LDS reads are done
-        for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
-            const int hi = idx / HW, hj = idx - hi * HW;
-            sp[hi][hj] = pAt<T>(r, aP, Z, pold, tl.i0 - 1 + hi, tl.j0 - 1 + hj, M, N, P, kk, beta);
-        }
-        for (int idx = int(threadIdx.x); idx < TH * (kTileW + 1); idx += kBlock) {
-            const int ci = idx / (kTileW + 1), cj = idx - ci * (kTileW + 1);
-            scE[ci][cj] = at(cE, tl.i0 + ci, tl.j0 - 1 + cj, M, N, P);
-        }
-        for (int idx = int(threadIdx.x); idx < (TH + 1) * kTileW; idx += kBlock) {
-            const int ci = idx / kTileW, cj = idx - ci * kTileW;
-            scN[ci][cj] = at(cN, tl.i0 - 1 + ci, tl.j0 + cj, M, N, P);
-        }
-        __syncthreads();
-        const int lane = int(threadIdx.x & 63), w = int(threadIdx.x >> 6);
         double v[1] = {0.0};
-#pragma unroll
-        for (int rr = 0; rr < R; ++rr) {
-            const int row = w * R + rr, i = tl.i0 + row, j = tl.j0 + lane;
-            if (isPDof(i, j, M, N)) {
-                const size_t k = size_t(i) * P + j;
-                const T pC = sp[row + 1][lane + 1];
-                // CSR column order S, W, P, E, N (pressure_solver.cpp:108-118)
-                const T q = -scN[row][lane] * sp[row][lane + 1] - scE[row][lane] * sp[row + 1][lane] + aP[k] * pC -
-                            scE[row][lane + 1] * sp[row + 1][lane + 2] - scN[row + 1][lane] * sp[row + 2][lane + 1];
-                pnew[k] = pC;
-                Q[k] = q;
-                v[0] += double(pC) * double(q);
-            }
+        // the preconditioner and the first iteration are uniform per slot
+        if (d.precond) {
+            if (kk == 0) pcgQWave<T, R, true, true>(d, tl, kk, beta, v[0]);
+            else pcgQWave<T, R, true, false>(d, tl, kk, beta, v[0]);
+        } else {
+            if (kk == 0) pcgQWave<T, R, false, true>(d, tl, kk, beta, v[0]);
+            else pcgQWave<T, R, false, false>(d, tl, kk, beta, v[0]);
         }
         blockReduceTo<1, 0u>(v, a.partials + size_t(t) * kNumPartials);
     }
@@ -582,9 +615,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
-        if (st.status != ST_RUNNING || st.pdone || st.stepPhase) continue;
+        const int status = st.status, pdone = st.pdone, phase = st.stepPhase, kk = st.pk;
+        if ((status != ST_RUNNING) | (pdone != 0) | (phase != 0) | (kk < 0)) continue;
         const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
-        const int kk = st.pk;
         const T alpha = T(st.alpha);
         const auto pk = G(static_cast<const T*>(d.pr[P_PK0 + ((kk & 1) ^ 1)]));
         const auto x = G(static_cast<T*>(d.pr[P_X]));
@@ -592,16 +625,27 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
         const auto Q = G(static_cast<const T*>(d.pr[P_Q]));
         const auto aP = G(static_cast<const T*>(d.pr[P_AP]));
         double v[2] = {0.0, 0.0};
+        // all loads first, from clamped addresses; stores only on unknowns
+        T xs[R], ps[R], rs[R], qs[R], as[R];
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+            const size_t k = clampIdx(tl.i + rr, j, M, P);
+            xs[rr] = x[k];
+            ps[rr] = pk[k];
+            rs[rr] = r[k];
+            qs[rr] = Q[k];
+            as[rr] = aP[k];
+        }
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
             const int i = tl.i + rr;
             if (!isPDof(i, j, M, N)) continue;
             const size_t k = size_t(i) * P + j;
-            x[k] += alpha * pk[k];
-            const T rn = r[k] - alpha * Q[k];
+            x[k] = xs[rr] + alpha * ps[rr];
+            const T rn = rs[rr] - alpha * qs[rr];
             r[k] = rn;
             v[0] += double(rn) * double(rn);
-            v[1] += double(rn) * double(rn / aP[k]);
+            v[1] += double(rn) * double(rn / as[rr]);
         }
         blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials);
     }
