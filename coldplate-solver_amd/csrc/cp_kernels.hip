@@ -154,6 +154,13 @@ __device__ __forceinline__ Tile decodeTile(const KArgs& a, int t) {
 
 __device__ __forceinline__ double clampv(double x, double m) { return fmax(-m, fmin(m, x)); }
 
+// Row-major index of (i, j) clamped into the (M+1) x P array: lanes outside
+// the unknowns load from a valid address and their values are discarded, so
+// every load of a tile is issued unconditionally, ahead of any use.
+__device__ __forceinline__ size_t clampIdx(int i, int j, int M, int P) {
+    return size_t(min(max(i, 0), M)) * P + min(max(j, 0), P - 1);
+}
+
 // ---------------------------------------------------------------------------
 // momentum assembly (momentum_solver.cpp:185-298, delta form)
 // ---------------------------------------------------------------------------
@@ -196,9 +203,32 @@ __device__ __forceinline__ double souCorrection(Ptr f, size_t k, int i, int j, i
 // from delta = 0, which is the reference's x0 = u), and delta = 0.
 // u, v, p are read from the workgroup's LDS tile (index ks, row pitch P);
 // k indexes the node's global arrays.
+// Descriptor fields of the momentum rows, copied to registers once per tile
+// (read through the descriptor, they would be reloaded after every store,
+// since the compiler cannot rule out aliasing).
+template <typename T>
+struct MomConst {
+    int M, N, pseudo, localPseudo, convScheme;
+    double convScale, rho, hx, hy, De, Dn, timeStep, minSpeed, hChar, sinkDiag, vol, uvAlpha;
+    gp<double> dE, dN;
+    gp<T> mu[M_D1], mv[M_D1];
+    __device__ __forceinline__ explicit MomConst(const SlotDesc& d)
+        : M(d.M), N(d.N), pseudo(d.pseudo), localPseudo(d.localPseudo), convScheme(d.convScheme),
+          convScale(d.convScale), rho(d.rho), hx(d.hx), hy(d.hy), De(d.De), Dn(d.Dn), timeStep(d.timeStep),
+          minSpeed(d.minSpeed), hChar(d.hChar), sinkDiag(d.sinkDiag), vol(d.vol), uvAlpha(d.uvAlpha), dE(G(d.dE)),
+          dN(G(d.dN)) {
+#pragma unroll
+        for (int q = 0; q < M_D1; ++q) {
+            mu[q] = G(static_cast<T*>(d.mu[q]));
+            mv[q] = G(static_cast<T*>(d.mv[q]));
+        }
+    }
+};
+
 template <typename T, bool IsU>
-__device__ __forceinline__ void momRow(const SlotDesc& d, const double* u, const double* v, const double* pp,
-                                       size_t ks, int P, size_t k, int i, int j, double maxVel, double cfl) {
+__device__ __forceinline__ void momRow(const MomConst<T>& d, const double* u, const double* v, const double* pp,
+                                       size_t ks, int P, size_t k, int i, int j, double maxVel, double cfl,
+                                       double alphaLocal) {
     double ue, uw, vn, vs;
     if (IsU) {
         ue = clampv(0.5 * (u[ks] + u[ks + 1]), maxVel);
@@ -231,7 +261,6 @@ __device__ __forceinline__ void momRow(const SlotDesc& d, const double* u, const
         }
         trans = d.rho * d.vol / dt;
     }
-    const double alphaLocal = IsU ? G(d.aU)[k] : G(d.aV)[k];
     double aP0 = sumA + (Fe - Fw + Fn - Fs) + trans + d.sinkDiag + alphaLocal * d.vol;
     aP0 = fmax(aP0, 1e-10);
     const double aP = aP0 / d.uvAlpha;
@@ -239,17 +268,17 @@ __device__ __forceinline__ void momRow(const SlotDesc& d, const double* u, const
     Sp += trans * cur;  // uOld == u at momentum time (iterations.cpp:118-119)
     Sp += (1.0 - d.uvAlpha) / d.uvAlpha * aP0 * cur;
     if (d.convScheme == 1) Sp += souCorrection<IsU>(f, ks, i, j, P, d.M, d.N, Fe, Fw, Fn, Fs);
-    if (IsU) G(d.dE)[k] = d.hy / aP0;
-    else G(d.dN)[k] = d.hx / aP0;
+    if (IsU) d.dE[k] = d.hy / aP0;
+    else d.dN[k] = d.hx / aP0;
     const double r0 = Sp + aE * f[ks + 1] + aW * f[ks - 1] + aN * f[ks + P] + aS * f[ks - P] - aP * cur;
-    void* const* A = IsU ? d.mu : d.mv;
-    G(static_cast<T*>(A[M_AP]))[k] = T(aP);
-    G(static_cast<T*>(A[M_AE]))[k] = T(aE);
-    G(static_cast<T*>(A[M_AW]))[k] = T(aW);
-    G(static_cast<T*>(A[M_AN]))[k] = T(aN);
-    G(static_cast<T*>(A[M_AS]))[k] = T(aS);
-    G(static_cast<T*>(A[M_R0]))[k] = T(r0);
-    G(static_cast<T*>(A[M_D0]))[k] = T(0);
+    const gp<T>* A = IsU ? d.mu : d.mv;
+    A[M_AP][k] = T(aP);
+    A[M_AE][k] = T(aE);
+    A[M_AW][k] = T(aW);
+    A[M_AN][k] = T(aN);
+    A[M_AS][k] = T(aS);
+    A[M_R0][k] = T(r0);
+    A[M_D0][k] = T(0);
 }
 
 // The workgroup stages u, v and p of its tile with a two-deep halo (the SOU
@@ -270,15 +299,38 @@ __global__ __launch_bounds__(kBlock) void k_mom_assemble(KArgs a) {
         const double maxVel = 3.0 * fmax(fabs(st.inletVelocity), 0.1);
         const double cfl = st.pseudoCFL;
         const auto gu = G(d.u), gv = G(d.v), gpp = G(d.p);
+        const MomConst<T> c(d);
+        // every staging load of the tile first (clamped addresses), then the
+        // LDS stores with the out-of-grid nodes zeroed
+        constexpr int NE = HH * HW, NIT = (NE + kBlock - 1) / kBlock;
+        double lu[NIT], lv[NIT], lp[NIT];
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int idx = int(threadIdx.x) + it * kBlock;
+            const int h = idx / HW, w = idx - h * HW;
+            const size_t k = clampIdx(tl.i0 - 2 + h, tl.j0 - 2 + w, M, P);
+            lu[it] = gu[k];
+            lv[it] = gv[k];
+            lp[it] = gpp[k];
+        }
+        double aUr[R], aVr[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const size_t k = clampIdx(tl.i + r, j, M, P);
+            aUr[r] = G(d.aU)[k];
+            aVr[r] = G(d.aV)[k];
+        }
         __syncthreads();  // the previous tile's LDS reads are done
-        for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int idx = int(threadIdx.x) + it * kBlock;
+            if (NE % kBlock != 0 && idx >= NE) break;
             const int h = idx / HW, w = idx - h * HW;
             const int i = tl.i0 - 2 + h, jj = tl.j0 - 2 + w;
             const bool in = i >= 0 && i <= M && jj >= 0 && jj <= N;
-            const size_t k = in ? size_t(i) * P + jj : 0;
-            su[idx] = in ? gu[k] : 0.0;
-            sv[idx] = in ? gv[k] : 0.0;
-            sp[idx] = in ? gpp[k] : 0.0;
+            su[idx] = in ? lu[it] : 0.0;
+            sv[idx] = in ? lv[it] : 0.0;
+            sp[idx] = in ? lp[it] : 0.0;
         }
         __syncthreads();
         const int lane = int(threadIdx.x & 63), wv = int(threadIdx.x >> 6);
@@ -289,9 +341,9 @@ __global__ __launch_bounds__(kBlock) void k_mom_assemble(KArgs a) {
             const size_t k = size_t(i) * P + j;
             const size_t ks = size_t(wv * R + r + 2) * HW + (lane + 2);
             if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2)
-                momRow<T, true>(d, su, sv, sp, ks, HW, k, i, j, maxVel, cfl);
+                momRow<T, true>(c, su, sv, sp, ks, HW, k, i, j, maxVel, cfl, aUr[r]);
             if (i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1)
-                momRow<T, false>(d, su, sv, sp, ks, HW, k, i, j, maxVel, cfl);
+                momRow<T, false>(c, su, sv, sp, ks, HW, k, i, j, maxVel, cfl, aVr[r]);
         }
         // the workgroup holding the slot's first tile resets the Jacobi state
         // (no tile of this kernel reads it)
@@ -325,6 +377,7 @@ __device__ __forceinline__ T at(const T* f, int i, int j, int M, int jmax, int P
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_jacobi(KArgs a) {
     constexpr int TH = kTileH * R, HH = TH + 2, HW = kTileW + 2;
+    constexpr int NE = HH * HW, NIT = (NE + kBlock - 1) / kBlock;
     __shared__ T sU[HH][HW];
     __shared__ T sV[HH][HW];
     const int nT = *a.nTiles;
@@ -332,19 +385,51 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(KArgs a) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
-        if (st.status != ST_RUNNING) continue;
-        const int doneU = st.jdone[0], doneV = st.jdone[1];
-        if (doneU && doneV) continue;
+        const int status = st.status, doneU = st.jdone[0], doneV = st.jdone[1], jkU = st.jk[0], jkV = st.jk[1];
+        if ((status != ST_RUNNING) | ((doneU != 0) & (doneV != 0))) continue;
         const int M = d.M, N = d.N, P = d.pitch;
-        const int pu = st.jk[0] & 1, pv = st.jk[1] & 1;
+        const int pu = jkU & 1, pv = jkV & 1;
+        gp<const T> cu[M_D0], cv[M_D0];
+#pragma unroll
+        for (int q = 0; q < M_D0; ++q) {
+            cu[q] = G(static_cast<const T*>(d.mu[q]));
+            cv[q] = G(static_cast<const T*>(d.mv[q]));
+        }
         const auto srcU = G(static_cast<const T*>(d.mu[M_D0 + pu]));
         const auto srcV = G(static_cast<const T*>(d.mv[M_D0 + pv]));
+        const auto dstU = G(static_cast<T*>(d.mu[M_D0 + (pu ^ 1)]));
+        const auto dstV = G(static_cast<T*>(d.mv[M_D0 + (pv ^ 1)]));
+        // every load of the tile first, from clamped addresses: the halo of
+        // delta for both systems, then the coefficients of this thread's rows
+        T lu[NIT], lv[NIT];
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int idx = int(threadIdx.x) + it * kBlock;
+            const int hi = idx / HW, hj = idx - hi * HW;
+            const size_t k = clampIdx(tl.i0 - 1 + hi, tl.j0 - 1 + hj, M, P);
+            lu[it] = doneU ? T(0) : srcU[k];
+            lv[it] = doneV ? T(0) : srcV[k];
+        }
+        T au[R][M_D0], av[R][M_D0];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const size_t k = clampIdx(tl.i + r, tl.j, M, P);
+#pragma unroll
+            for (int q = 0; q < M_D0; ++q) {
+                au[r][q] = doneU ? T(0) : cu[q][k];
+                av[r][q] = doneV ? T(0) : cv[q][k];
+            }
+        }
         __syncthreads();  // the previous tile's LDS reads are done
-        for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int idx = int(threadIdx.x) + it * kBlock;
+            if (NE % kBlock != 0 && idx >= NE) break;
             const int hi = idx / HW, hj = idx - hi * HW;
             const int i = tl.i0 - 1 + hi, j = tl.j0 - 1 + hj;
-            sU[hi][hj] = doneU ? T(0) : at(srcU, i, j, M, N, P);
-            sV[hi][hj] = doneV ? T(0) : at(srcV, i, j, M, N, P);
+            const bool in = i >= 0 && i <= M && j >= 0 && j <= N;  // zero outside, as at()
+            sU[hi][hj] = in ? lu[it] : T(0);
+            sV[hi][hj] = in ? lv[it] : T(0);
         }
         __syncthreads();
         const int lane = int(threadIdx.x & 63), w = int(threadIdx.x >> 6);
@@ -355,27 +440,21 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(KArgs a) {
             const size_t k = size_t(i) * P + j;
             // U unknowns: i in [1, M-1], j in [1, N-2]; V: i in [1, M-2], j in [1, N-1]
             if (!doneU && i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2) {
-                void* const* A = d.mu;
-                const T ap = G(static_cast<const T*>(A[M_AP]))[k];
+                const T ap = au[r][M_AP];
                 const T dc = sU[row + 1][lane + 1];
-                const T Ax = ap * dc - G(static_cast<const T*>(A[M_AE]))[k] * sU[row + 1][lane + 2] -
-                             G(static_cast<const T*>(A[M_AW]))[k] * sU[row + 1][lane] -
-                             G(static_cast<const T*>(A[M_AN]))[k] * sU[row + 2][lane + 1] -
-                             G(static_cast<const T*>(A[M_AS]))[k] * sU[row][lane + 1];
-                const T res = G(static_cast<const T*>(A[M_R0]))[k] - Ax;
-                G(static_cast<T*>(A[M_D0 + (pu ^ 1)]))[k] = dc + res / ap;
+                const T Ax = ap * dc - au[r][M_AE] * sU[row + 1][lane + 2] - au[r][M_AW] * sU[row + 1][lane] -
+                             au[r][M_AN] * sU[row + 2][lane + 1] - au[r][M_AS] * sU[row][lane + 1];
+                const T res = au[r][M_R0] - Ax;
+                dstU[k] = dc + res / ap;
                 v[0] += double(res) * double(res);
             }
             if (!doneV && i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1) {
-                void* const* A = d.mv;
-                const T ap = G(static_cast<const T*>(A[M_AP]))[k];
+                const T ap = av[r][M_AP];
                 const T dc = sV[row + 1][lane + 1];
-                const T Ax = ap * dc - G(static_cast<const T*>(A[M_AE]))[k] * sV[row + 1][lane + 2] -
-                             G(static_cast<const T*>(A[M_AW]))[k] * sV[row + 1][lane] -
-                             G(static_cast<const T*>(A[M_AN]))[k] * sV[row + 2][lane + 1] -
-                             G(static_cast<const T*>(A[M_AS]))[k] * sV[row][lane + 1];
-                const T res = G(static_cast<const T*>(A[M_R0]))[k] - Ax;
-                G(static_cast<T*>(A[M_D0 + (pv ^ 1)]))[k] = dc + res / ap;
+                const T Ax = ap * dc - av[r][M_AE] * sV[row + 1][lane + 2] - av[r][M_AW] * sV[row + 1][lane] -
+                             av[r][M_AN] * sV[row + 2][lane + 1] - av[r][M_AS] * sV[row][lane + 1];
+                const T res = av[r][M_R0] - Ax;
+                dstV[k] = dc + res / ap;
                 v[1] += double(res) * double(res);
             }
         }
@@ -396,10 +475,29 @@ __global__ __launch_bounds__(kBlock) void k_mom_finish(KArgs a) {
         SlotState& st = a.state[tl.s];
         if (st.status != ST_RUNNING) continue;
         const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
-        const double maxVel = 3.0 * fmax(fabs(st.inletVelocity), 0.1);
         const double Uin = st.inletVelocity;
+        const double maxVel = 3.0 * fmax(fabs(Uin), 0.1);
         const auto dU = G(static_cast<const T*>(d.mu[M_D0 + (st.jiters[0] & 1)]));
         const auto dV = G(static_cast<const T*>(d.mv[M_D0 + (st.jiters[1] & 1)]));
+        const auto u = G(d.u), vv = G(d.v), gam = G(d.gam);
+        const auto us = G(d.us), vs = G(d.vs);
+        // all loads first (clamped addresses), then the branchy selection
+        double uk[R], um[R], vk[R], vm[R], ge[R];
+        T duk[R], dum[R], dvk[R], dvm[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = tl.i + r;
+            const size_t kc = clampIdx(i, j, M, P), kw = clampIdx(i, j - 1, M, P);
+            uk[r] = u[kc];
+            um[r] = u[kw];
+            vk[r] = vv[kc];
+            vm[r] = vv[kw];
+            duk[r] = dU[kc];
+            dum[r] = dU[kw];
+            dvk[r] = dV[kc];
+            dvm[r] = dV[kw];
+            ge[r] = gam[clampIdx(i, j + 1, M, P)];
+        }
         double v[3] = {0.0, 0.0, 0.0};
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -412,14 +510,14 @@ __global__ __launch_bounds__(kBlock) void k_mom_finish(KArgs a) {
                 if (i == 0 || i == M) {
                     val = 0.0;
                 } else if (j == 0) {
-                    val = (G(d.gam)[k + 1] > 0.5) ? Uin : 0.0;  // gamma(i-1, 0) > 0.5
+                    val = (ge[r] > 0.5) ? Uin : 0.0;  // gamma(i-1, 0) > 0.5
                 } else if (j == N - 1) {
-                    val = clampv(G(d.u)[k - 1] + double(dU[k - 1]), maxVel);  // outlet: u(i,N-1) = u(i,N-2)
+                    val = clampv(um[r] + double(dum[r]), maxVel);  // outlet: u(i,N-1) = u(i,N-2)
                 } else {
-                    val = clampv(G(d.u)[k] + double(dU[k]), maxVel);
-                    v[0] = nanmax(v[0], fabs(val - G(d.u)[k]));
+                    val = clampv(uk[r] + double(duk[r]), maxVel);
+                    v[0] = nanmax(v[0], fabs(val - uk[r]));
                 }
-                G(d.us)[k] = val;
+                us[k] = val;
                 if (!(val == val)) v[2] = 1.0;
             }
             // v* : M x (N+1)
@@ -428,12 +526,12 @@ __global__ __launch_bounds__(kBlock) void k_mom_finish(KArgs a) {
                 if (i == 0 || i == M - 1 || j == 0) {
                     val = 0.0;
                 } else if (j == N) {
-                    val = clampv(G(d.v)[k - 1] + double(dV[k - 1]), maxVel);  // outlet: v(i,N) = v(i,N-1)
+                    val = clampv(vm[r] + double(dvm[r]), maxVel);  // outlet: v(i,N) = v(i,N-1)
                 } else {
-                    val = clampv(G(d.v)[k] + double(dV[k]), maxVel);
-                    v[1] = nanmax(v[1], fabs(val - G(d.v)[k]));
+                    val = clampv(vk[r] + double(dvk[r]), maxVel);
+                    v[1] = nanmax(v[1], fabs(val - vk[r]));
                 }
-                G(d.vs)[k] = val;
+                vs[k] = val;
                 if (!(val == val)) v[2] = 1.0;
             }
         }
@@ -455,32 +553,52 @@ __global__ __launch_bounds__(kBlock) void k_pres_assemble(KArgs a) {
         if (st.status != ST_RUNNING) continue;
         const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
         const int jMass = d.massScope ? N - 2 : N - 1;
+        const double rho = d.rho, hx = d.hx, hy = d.hy;
+        const auto us = G(d.us), vs = G(d.vs), dE = G(d.dE), dN = G(d.dN);
+        const auto oCE = G(static_cast<T*>(d.pr[P_CE])), oCN = G(static_cast<T*>(d.pr[P_CN]));
+        const auto oAP = G(static_cast<T*>(d.pr[P_AP])), oR = G(static_cast<T*>(d.pr[P_R]));
+        const auto oX = G(static_cast<T*>(d.pr[P_X]));
+        const auto bm = G(d.bm), pc = G(d.pc);
+        // all loads first (clamped addresses)
+        double uE[R], uW[R], vN[R], vS[R], dEe[R], dEw[R], dNn[R], dNs[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = tl.i + r;
+            const size_t kc = clampIdx(i, j, M, P), kw = clampIdx(i, j - 1, M, P), ks = clampIdx(i - 1, j, M, P);
+            uE[r] = us[kc];
+            uW[r] = us[kw];
+            vN[r] = vs[kc];
+            vS[r] = vs[ks];
+            dEe[r] = dE[kc];
+            dEw[r] = dE[kw];
+            dNn[r] = dN[kc];
+            dNs[r] = dN[ks];
+        }
         double v[2] = {0.0, 0.0};
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int i = tl.i + r;
             if (!(i >= 1 && i <= M - 1 && j >= 1 && j <= N - 1)) continue;
             const size_t k = size_t(i) * P + j;
-            const double rho = d.rho, hx = d.hx, hy = d.hy;
-            const double mE = rho * G(d.us)[k] * hy, mW = rho * G(d.us)[k - 1] * hy;
-            const double mN = rho * G(d.vs)[k] * hx, mS = rho * G(d.vs)[k - P] * hx;
+            const double mE = rho * uE[r] * hy, mW = rho * uW[r] * hy;
+            const double mN = rho * vN[r] * hx, mS = rho * vS[r] * hx;
             const double b = mW - mE + mS - mN;
             if (j <= jMass) v[0] = nanmax(v[0], fabs(b));
             if (j <= N - 2) {
-                const double aE = rho * G(d.dE)[k] * hy;
-                const double aW = (j > 1) ? rho * G(d.dE)[k - 1] * hy : 0.0;
-                const double aN = (i < M - 1) ? rho * G(d.dN)[k] * hx : 0.0;
-                const double aS = (i > 1) ? rho * G(d.dN)[k - P] * hx : 0.0;
+                const double aE = rho * dEe[r] * hy;
+                const double aW = (j > 1) ? rho * dEw[r] * hy : 0.0;
+                const double aN = (i < M - 1) ? rho * dNn[r] * hx : 0.0;
+                const double aS = (i > 1) ? rho * dNs[r] * hx : 0.0;
                 double aP = aE + aW + aN + aS;
                 if (aP < 1e-20) aP = 1.0;
-                G(static_cast<T*>(d.pr[P_CE]))[k] = T(aE);
-                G(static_cast<T*>(d.pr[P_CN]))[k] = T(aN);
-                G(static_cast<T*>(d.pr[P_AP]))[k] = T(aP);
+                oCE[k] = T(aE);
+                oCN[k] = T(aN);
+                oAP[k] = T(aP);
                 const T rr = T(b);
-                G(static_cast<T*>(d.pr[P_R]))[k] = rr;
-                G(static_cast<T*>(d.pr[P_X]))[k] = T(0);
-                G(d.bm)[k] = b;
-                G(d.pc)[k] = 0.0;
+                oR[k] = rr;
+                oX[k] = T(0);
+                bm[k] = b;
+                pc[k] = 0.0;
                 const T z = rr / T(aP);
                 v[1] += double(rr) * double(z);
             }
@@ -508,13 +626,6 @@ __device__ __forceinline__ T pAt(gp<const T> r, gp<const T> aP, gp<const T> Z, g
     const size_t k = size_t(i) * P + j;
     const T z = Z ? Z[k] : r[k] / aP[k];
     return kk == 0 ? z : z + beta * pold[k];
-}
-
-// Row-major index of (i, j) clamped into the (M+1) x P array: lanes outside
-// the unknowns load from a valid address and their values are discarded, so
-// every load of a tile is issued unconditionally, ahead of any use.
-__device__ __forceinline__ size_t clampIdx(int i, int j, int M, int P) {
-    return size_t(min(max(i, 0), M)) * P + min(max(j, 0), P - 1);
 }
 
 template <typename T, bool HasZ, bool First>
@@ -739,7 +850,9 @@ __device__ __forceinline__ MgTile decodeMgTile(const KArgs& a, int t) {
 
 // multigrid work of a slot in the current pressure solve
 __device__ __forceinline__ bool mgActive(const SlotDesc& d, const SlotState& st) {
-    return d.precond && st.status == ST_RUNNING && !st.pdone && !st.stepPhase;
+    // no short-circuit: the four loads overlap
+    const int pre = d.precond, status = st.status, pdone = st.pdone, phase = st.stepPhase;
+    return (pre != 0) & (status == ST_RUNNING) & (pdone == 0) & (phase == 0);
 }
 
 __device__ __forceinline__ size_t mgAt(const MgLevel& L, int I, int J) {
@@ -766,6 +879,8 @@ struct MgView {
           xo(G(static_cast<T*>(L.x))), z(G(static_cast<T*>(L.z))), bo(G(static_cast<T*>(L.b))) {}
     __device__ __forceinline__ bool in(int I, int J) const { return I >= 0 && I < H && J >= 0 && J < W; }
     __device__ __forceinline__ size_t at(int I, int J) const { return size_t(I + 1) * P + (J + 1); }
+    // index of (I, J) clamped into the level (a valid address for any I, J)
+    __device__ __forceinline__ size_t atc(int I, int J) const { return at(min(max(I, 0), H - 1), min(max(J, 0), W - 1)); }
 };
 
 // A x at LDS position (h, w) of a halo tile, terms in the CSR column order
@@ -893,18 +1008,31 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(KArgs a) {
         const MgView<T> L(d.lv[a.level]);
         const MgView<T> C(d.lv[a.level + 1]);
         const T om = T(d.mgOmega);
-        __syncthreads();  // the previous tile's LDS reads are done
-        for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
+        // every staging load first (clamped addresses), then the LDS stores
+        constexpr int NE = HH * HW, NIT = (NE + kBlock - 1) / kBlock;
+        T lb[NIT], la[NIT], le[NIT], ln[NIT];
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int idx = int(threadIdx.x) + it * kBlock;
             const int h = idx / HW, w = idx - h * HW;
-            const int I = tl.I0 - 2 + h, J = tl.J0 - 2 + w;
-            const bool in = L.in(I, J);
-            const size_t k = in ? L.at(I, J) : 0;
-            const T bv = in ? L.b[k] : T(0), av = in ? L.aP[k] : T(0);
-            sb[h][w] = bv;
-            saP[h][w] = av;
-            scE[h][w] = in ? L.cE[k] : T(0);
-            scN[h][w] = in ? L.cN[k] : T(0);
-            sx1[h][w] = in ? om * (bv / av) : T(0);
+            const size_t k = L.atc(tl.I0 - 2 + h, tl.J0 - 2 + w);
+            lb[it] = L.b[k];
+            la[it] = L.aP[k];
+            le[it] = L.cE[k];
+            ln[it] = L.cN[k];
+        }
+        __syncthreads();  // the previous tile's LDS reads are done
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int idx = int(threadIdx.x) + it * kBlock;
+            if (NE % kBlock != 0 && idx >= NE) break;
+            const int h = idx / HW, w = idx - h * HW;
+            const bool in = L.in(tl.I0 - 2 + h, tl.J0 - 2 + w);
+            sb[h][w] = in ? lb[it] : T(0);
+            saP[h][w] = in ? la[it] : T(0);
+            scE[h][w] = in ? le[it] : T(0);
+            scN[h][w] = in ? ln[it] : T(0);
+            sx1[h][w] = in ? om * (lb[it] / la[it]) : T(0);
         }
         __syncthreads();
         for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
@@ -1072,17 +1200,34 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(KArgs a) {
         const MgView<T> L(d.lv[a.level]);
         const MgView<T> C(d.lv[a.level + 1]);
         const T om = T(d.mgOmega), sc = T(d.mgScale);
-        __syncthreads();
-        for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
+        // every staging load first (clamped addresses), then the LDS stores
+        constexpr int NE = HH * HW, NIT = (NE + kBlock - 1) / kBlock;
+        T lb[NIT], la[NIT], le[NIT], ln[NIT], lx[NIT], lz[NIT];
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int idx = int(threadIdx.x) + it * kBlock;
             const int h = idx / HW, w = idx - h * HW;
-            const int I = tl.I0 - 2 + h, J = tl.J0 - 2 + w;
-            const bool in = L.in(I, J);
-            const size_t k = in ? L.at(I, J) : 0;
-            sb[h][w] = in ? L.b[k] : T(0);
-            saP[h][w] = in ? L.aP[k] : T(0);
-            scE[h][w] = in ? L.cE[k] : T(0);
-            scN[h][w] = in ? L.cN[k] : T(0);
-            sx3[h][w] = in ? L.x[k] + sc * C.z[C.at(I >> 1, J >> 1)] : T(0);
+            const int I = min(max(tl.I0 - 2 + h, 0), L.H - 1), J = min(max(tl.J0 - 2 + w, 0), L.W - 1);
+            const size_t k = L.at(I, J);
+            lb[it] = L.b[k];
+            la[it] = L.aP[k];
+            le[it] = L.cE[k];
+            ln[it] = L.cN[k];
+            lx[it] = L.x[k];
+            lz[it] = C.z[C.at(I >> 1, J >> 1)];
+        }
+        __syncthreads();  // the previous tile's LDS reads are done
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int idx = int(threadIdx.x) + it * kBlock;
+            if (NE % kBlock != 0 && idx >= NE) break;
+            const int h = idx / HW, w = idx - h * HW;
+            const bool in = L.in(tl.I0 - 2 + h, tl.J0 - 2 + w);
+            sb[h][w] = in ? lb[it] : T(0);
+            saP[h][w] = in ? la[it] : T(0);
+            scE[h][w] = in ? le[it] : T(0);
+            scN[h][w] = in ? ln[it] : T(0);
+            sx3[h][w] = in ? lx[it] + sc * lz[it] : T(0);
         }
         __syncthreads();
         for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
@@ -1194,47 +1339,65 @@ __global__ __launch_bounds__(kBlock) void k_correct(KArgs a) {
         SlotState& st = a.state[tl.s];
         if (st.status != ST_RUNNING) continue;
         const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
-        const double maxVel = 3.0 * fmax(fabs(st.inletVelocity), 0.1);
         const double Uin = st.inletVelocity;
+        const double maxVel = 3.0 * fmax(fabs(Uin), 0.1);
+        const double pAlpha = d.pAlpha;
+        const auto gp_ = G(d.p), pc = G(d.pc), dE = G(d.dE), dN = G(d.dN);
+        const auto us = G(d.us), vs = G(d.vs), gam = G(d.gam), u = G(d.u), vv = G(d.v);
+        // the u row at the outlet column copies its west neighbour, the v row
+        // at j = N too: load the node each row reads, from clamped addresses
+        const int ju = (j == N - 1) ? j - 1 : j, jv = (j == N) ? j - 1 : j;
+        double pr[R], pcc[R], dEu[R], pcu0[R], pcu1[R], usu[R], ge[R], dNv[R], pcv0[R], pcv1[R], vsv[R];
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+            const int i = tl.i + rr;
+            const size_t kc = clampIdx(i, j, M, P), ku = clampIdx(i, ju, M, P), kv = clampIdx(i, jv, M, P);
+            pr[rr] = gp_[kc];
+            pcc[rr] = pc[kc];
+            dEu[rr] = dE[ku];
+            pcu0[rr] = pc[ku];
+            pcu1[rr] = pc[clampIdx(i, ju + 1, M, P)];
+            usu[rr] = us[ku];
+            ge[rr] = gam[clampIdx(i, j + 1, M, P)];
+            dNv[rr] = dN[kv];
+            pcv0[rr] = pc[kv];
+            pcv1[rr] = pc[clampIdx(i + 1, jv, M, P)];
+            vsv[rr] = vs[kv];
+        }
         double bad = 0.0;
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
             const int i = tl.i + rr;
             if (i > M || j > N) continue;
             const size_t k = size_t(i) * P + j;
-            const auto pc = G(d.pc);
             // pressure update + ghosts; each ghost has exactly one writer
             if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 1) {
-                const double pn = G(d.p)[k] + d.pAlpha * pc[k];
-                G(d.p)[k] = pn;
-                if (i == 1) G(d.p)[k - P] = pn;      // p(0,j) = p(1,j)
-                if (i == M - 1) G(d.p)[k + P] = pn;  // p(M,j) = p(M-1,j)
-                if (j == 1) G(d.p)[k - 1] = pn;      // p(i,0) = p(i,1)
-                if (j == N - 1) G(d.p)[k + 1] = 0.0; // p(i,N) = 0
+                const double pn = pr[rr] + pAlpha * pcc[rr];
+                gp_[k] = pn;
+                if (i == 1) gp_[k - P] = pn;      // p(0,j) = p(1,j)
+                if (i == M - 1) gp_[k + P] = pn;  // p(M,j) = p(M-1,j)
+                if (j == 1) gp_[k - 1] = pn;      // p(i,0) = p(i,1)
+                if (j == N - 1) gp_[k + 1] = 0.0; // p(i,N) = 0
                 if (!(pn == pn)) bad = 1.0;
             }
-            auto uCorr = [&](size_t kk) {
-                const double du = clampv(G(d.dE)[kk] * (pc[kk] - pc[kk + 1]), maxVel);
-                return clampv(G(d.us)[kk] + du, maxVel);
-            };
-            auto vCorr = [&](size_t kk) {
-                const double dv = clampv(G(d.dN)[kk] * (pc[kk] - pc[kk + P]), maxVel);
-                return clampv(G(d.vs)[kk] + dv, maxVel);
-            };
             if (j <= N - 1) {
                 double val;
                 if (i == 0 || i == M) val = 0.0;
-                else if (j == 0) val = (G(d.gam)[k + 1] > 0.5) ? Uin : 0.0;
-                else if (j == N - 1) val = uCorr(k - 1);
-                else val = uCorr(k);
-                G(d.u)[k] = val;
+                else if (j == 0) val = (ge[rr] > 0.5) ? Uin : 0.0;
+                else {  // interior, or the outlet column from its west neighbour
+                    const double du = clampv(dEu[rr] * (pcu0[rr] - pcu1[rr]), maxVel);
+                    val = clampv(usu[rr] + du, maxVel);
+                }
+                u[k] = val;
             }
             if (i <= M - 1) {
                 double val;
                 if (i == 0 || i == M - 1 || j == 0) val = 0.0;
-                else if (j == N) val = vCorr(k - 1);
-                else val = vCorr(k);
-                G(d.v)[k] = val;
+                else {  // interior, or j = N from its west neighbour
+                    const double dv = clampv(dNv[rr] * (pcv0[rr] - pcv1[rr]), maxVel);
+                    val = clampv(vsv[rr] + dv, maxVel);
+                }
+                vv[k] = val;
             }
         }
         double vb[1] = {bad};
