@@ -21,6 +21,10 @@ namespace cpdev {
 constexpr int kTileW = 64;
 constexpr int kTileH = 4;
 constexpr int kBlock = kTileW * kTileH;
+// Multigrid level tiles: the 64 lanes of a wave span kMgTileW columns of the
+// level plus a two-column halo on each side (the two smoothing sweeps and the
+// residual reach two nodes out).
+constexpr int kMgTileW = kTileW - 4;
 constexpr int kNumPartials = 4;  // per-tile reduction partials (doubles)
 constexpr int kNumXCD = 8;       // accelerator complex dies of an MI355X
 constexpr int kMaxLevels = 12;   // aggregation-multigrid levels (incl. the finest)
@@ -169,12 +173,9 @@ hipError_t launchKernel(Kern k, bool f32, int rows, int grid, int nslots, hipStr
 // idempotent kernel).
 hipError_t launchWorkOnly(Kern k, bool f32, int rows, int grid, hipStream_t s, const KArgs& a);
 // Rows per wave of the finest multigrid level's tiles for a batch tile
-// height (coarser levels use 1); R = 2 measured fastest on 1024 x 512 grids
-// (1.60 ms per outer step of 8 slots, against 1.64 with R = 4).
-inline int mgRows(bool f32, int rows) {
-    (void)f32;
-    return rows < 2 ? rows : 2;
-}
+// height (coarser levels use 2). Even, so that a wave's rows hold whole 2 x 2
+// aggregates; 4 only in single precision (register footprint).
+inline int mgRows(bool f32, int rows) { return (f32 && rows >= 4) ? 4 : 2; }
 hipError_t launchPlane(int nreq, hipStream_t s, const SlotDesc* desc, const PlaneReq* req, PlaneOut* out);
 
 }  // namespace cpdev

@@ -286,7 +286,7 @@ int launchMg(cp_batch* b, Kern k, int level) {
     a.tileSlot = b->dMgTileSlot + size_t(level) * b->maxTiles;
     a.nTiles = b->dMgNTiles + level;
     a.level = level;
-    const int rows = level == 0 ? b->mgRowsEff() : 1;
+    const int rows = level == 0 ? b->mgRowsEff() : 2;
     const int grid = perSlot ? b->maxSlots : mgGrid(b, level);
     const hipError_t e = launchKernel(k, f32, rows, grid, b->maxSlots, b->stream, a);
     if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
@@ -462,8 +462,8 @@ void setTiling(SlotDesc& d, int tileRows, int mgRows0) {
     d.ntiles = d.tilesX * tilesY;
     for (int l = 0; l < d.nLevels; ++l) {
         MgLevel& L = d.lv[l];
-        const int rows = kTileH * (l == 0 ? mgRows0 : 1);
-        L.tilesX = (L.W + kTileW - 1) / kTileW;
+        const int rows = kTileH * (l == 0 ? mgRows0 : 2);
+        L.tilesX = (L.W + kMgTileW - 1) / kMgTileW;
         L.ntiles = L.tilesX * ((L.H + rows - 1) / rows);
     }
 }
@@ -1223,8 +1223,8 @@ int cp_serve(cp_batch* b, const cp_request* reqs, int n, int steps_per_poll, cp_
 int cp_batch_set_mg_tiling(cp_batch* b, int rows) {
     if (!b) return setError(nullptr, CP_E_ARG, "null batch");
     const int maxRows = b->precision == CP_PREC_MIXED ? 4 : 2;
-    if (!(rows == 0 || rows == 1 || rows == 2 || rows == 4) || rows > maxRows)
-        return setError(&b->err, CP_E_ARG, "multigrid tile rows %d out of range (0, 1, 2%s)", rows,
+    if (!(rows == 0 || rows == 2 || rows == 4) || rows > maxRows)
+        return setError(&b->err, CP_E_ARG, "multigrid tile rows %d out of range (0, 2%s)", rows,
                         maxRows == 4 ? ", 4" : "");
     b->mgTileRows = rows;
     for (int s = 0; s < b->maxSlots; ++s)
@@ -1244,7 +1244,7 @@ int cp_batch_autotune(cp_batch* b, int steps, int* rows_out, int* mg_rows_out, i
     if (!any) return setError(&err, CP_E_STATE, "autotune needs running slots");
     const bool f32 = b->precision == CP_PREC_MIXED;
     const int rowsC[] = {1, 2, 4, 8};
-    const int mgC[] = {1, 2, 4};
+    const int mgC[] = {2, 4};
     const int wgC[] = {4, 8, 16};
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
@@ -1254,7 +1254,7 @@ int cp_batch_autotune(cp_batch* b, int steps, int* rows_out, int* mg_rows_out, i
     int rc = CP_OK;
     for (int r : rowsC) {
         for (int m : mgC) {
-            if (m > (f32 ? 4 : 2) || m > r) continue;
+            if (m > (f32 ? 4 : 2)) continue;
             for (int w : wgC) {
                 b->tileRows = r;
                 b->mgTileRows = m;

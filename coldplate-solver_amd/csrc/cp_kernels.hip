@@ -844,7 +844,7 @@ __device__ __forceinline__ MgTile decodeMgTile(const KArgs& a, int t) {
     const int ty = lt / L.tilesX;
     const int tx = lt - ty * L.tilesX;
     tl.I0 = ty * (kTileH * R);
-    tl.J0 = tx * kTileW;
+    tl.J0 = tx * kMgTileW;
     return tl;
 }
 
@@ -883,19 +883,27 @@ struct MgView {
     __device__ __forceinline__ size_t atc(int I, int J) const { return at(min(max(I, 0), H - 1), min(max(J, 0), W - 1)); }
 };
 
-// A x at LDS position (h, w) of a halo tile, terms in the CSR column order
-// S, W, P, E, N (absent neighbours hold zero coefficients and values).
-template <typename T, int HH, int HW>
-__device__ __forceinline__ T mgAx(const T (&aP)[HH][HW], const T (&cE)[HH][HW], const T (&cN)[HH][HW],
-                                  const T (&x)[HH][HW], int h, int w) {
+// (A x) at a node from its own and its four neighbours' values, terms in
+// the CSR column order S, W, P, E, N (absent neighbours hold zero
+// coefficients and values): cS = cN of the south node, cW = cE of the west
+// node.
+template <typename T>
+__device__ __forceinline__ T ax5(T cS, T xS, T cW, T xW, T aPc, T xC, T cEc, T xE, T cNc, T xN) {
     T s = T(0);
-    s += -cN[h - 1][w] * x[h - 1][w];
-    s += -cE[h][w - 1] * x[h][w - 1];
-    s += aP[h][w] * x[h][w];
-    s += -cE[h][w] * x[h][w + 1];
-    s += -cN[h][w] * x[h + 1][w];
+    s += -cS * xS;
+    s += -cW * xW;
+    s += aPc * xC;
+    s += -cEc * xE;
+    s += -cNc * xN;
     return s;
 }
+
+// lane l receives lane l-1's value (west) / lane l+1's value (east); lanes 0
+// and 63 receive their own, which only halo lanes consume
+template <typename T>
+__device__ __forceinline__ T fromWest(T x) { return __shfl_up(x, 1, 64); }
+template <typename T>
+__device__ __forceinline__ T fromEast(T x) { return __shfl_down(x, 1, 64); }
 
 // Galerkin coarse operator of level a.level (>= 1) from level a.level - 1:
 // aP_c = sum of the aggregate's aP minus twice its internal couplings,
@@ -911,32 +919,53 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarsen(KArgs a) {
         if (!mgActive(d, st) || a.level >= d.nLevels) continue;
         const MgView<T> C(d.lv[a.level]);
         const MgView<T> F(d.lv[a.level - 1]);
-        auto g = [&](int I, int J) { return F.in(I, J) ? double(F.aP[F.at(I, J)]) : 0.0; };
-        auto eE = [&](int I, int J) { return (F.in(I, J) && J + 1 < F.W) ? double(F.cE[F.at(I, J)]) : 0.0; };
-        auto eN = [&](int I, int J) { return (F.in(I, J) && I + 1 < F.H) ? double(F.cN[F.at(I, J)]) : 0.0; };
         const auto caP = G(static_cast<T*>(d.lv[a.level].aP));
         const auto ccE = G(static_cast<T*>(d.lv[a.level].cE));
         const auto ccN = G(static_cast<T*>(d.lv[a.level].cN));
         const int lane = int(threadIdx.x & 63), w = int(threadIdx.x >> 6);
+        // the aggregate's fine aP, cE and cN, loaded together from clamped
+        // addresses; only couplings between unknowns count
+        T fa[R][4], fe[R][4], fn[R][4];
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
             const int I = tl.I0 + w * R + rr, J = tl.J0 + lane;
-            if (I >= C.H || J >= C.W) continue;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const size_t k = F.atc(2 * I + (q >> 1), 2 * J + (q & 1));
+                fa[rr][q] = F.aP[k];
+                fe[rr][q] = F.cE[k];
+                fn[rr][q] = F.cN[k];
+            }
+        }
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+            const int I = tl.I0 + w * R + rr, J = tl.J0 + lane;
+            if (lane >= kMgTileW || I >= C.H || J >= C.W) continue;
             const int i0 = 2 * I, j0 = 2 * J;
+            // q = 0: (i0, j0), 1: (i0, j0+1), 2: (i0+1, j0), 3: (i0+1, j0+1)
+            auto g = [&](int q) { return F.in(i0 + (q >> 1), j0 + (q & 1)) ? double(fa[rr][q]) : 0.0; };
+            auto eE = [&](int q) {
+                const int fi = i0 + (q >> 1), fj = j0 + (q & 1);
+                return (F.in(fi, fj) && fj + 1 < F.W) ? double(fe[rr][q]) : 0.0;
+            };
+            auto eN = [&](int q) {
+                const int fi = i0 + (q >> 1), fj = j0 + (q & 1);
+                return (F.in(fi, fj) && fi + 1 < F.H) ? double(fn[rr][q]) : 0.0;
+            };
             double acc = 0.0;
-            acc += g(i0, j0);
-            acc += g(i0, j0 + 1);
-            acc += g(i0 + 1, j0);
-            acc += g(i0 + 1, j0 + 1);
+            acc += g(0);
+            acc += g(1);
+            acc += g(2);
+            acc += g(3);
             double internal = 0.0;
-            internal += eE(i0, j0);
-            internal += eE(i0 + 1, j0);
-            internal += eN(i0, j0);
-            internal += eN(i0, j0 + 1);
+            internal += eE(0);
+            internal += eE(2);
+            internal += eN(0);
+            internal += eN(1);
             const size_t k = C.at(I, J);
             caP[k] = T(acc - 2.0 * internal);
-            ccE[k] = T(eE(i0, j0 + 1) + eE(i0 + 1, j0 + 1));
-            ccN[k] = T(eN(i0 + 1, j0) + eN(i0 + 1, j0 + 1));
+            ccE[k] = T(eE(1) + eE(3));
+            ccN[k] = T(eN(2) + eN(3));
         }
     }
 }
@@ -994,11 +1023,13 @@ __global__ __launch_bounds__(kBlock) void k_mg_factor(KArgs a) {
 
 // Down-sweep at level a.level (above the slot's tail): x1 = w b/aP,
 // x2 = x1 + w (b - A x1)/aP, r = b - A x2; stores x2 and the aggregate sums
-// of r as the next level's right-hand side.
+// of r as the next level's right-hand side. Each wave marches over its R
+// rows with all loads issued up front: lanes hold columns J0-2 .. J0+61
+// (outputs at lanes 2..61) and the wave loads rows I-2 .. I+R+1; values
+// outside the level are zero.
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_mg_down(KArgs a) {
-    constexpr int TH = kTileH * R, HH = TH + 4, HW = kTileW + 4;
-    __shared__ T sb[HH][HW], saP[HH][HW], scE[HH][HW], scN[HH][HW], sx1[HH][HW], sx2[HH][HW];
+    constexpr int NR = R + 4;
     const int nT = *a.nTiles;
     XCD_TILE_LOOP(t, nT) {
         const MgTile tl = decodeMgTile<R>(a, t);
@@ -1008,66 +1039,59 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(KArgs a) {
         const MgView<T> L(d.lv[a.level]);
         const MgView<T> C(d.lv[a.level + 1]);
         const T om = T(d.mgOmega);
-        // every staging load first (clamped addresses), then the LDS stores
-        constexpr int NE = HH * HW, NIT = (NE + kBlock - 1) / kBlock;
-        T lb[NIT], la[NIT], le[NIT], ln[NIT];
+        const int lane = int(threadIdx.x & 63);
+        const int J = tl.J0 - 2 + lane, ib = tl.I0 + int(threadIdx.x >> 6) * R;
+        T b[NR], ap[NR], ce[NR], cn[NR];
 #pragma unroll
-        for (int it = 0; it < NIT; ++it) {
-            const int idx = int(threadIdx.x) + it * kBlock;
-            const int h = idx / HW, w = idx - h * HW;
-            const size_t k = L.atc(tl.I0 - 2 + h, tl.J0 - 2 + w);
-            lb[it] = L.b[k];
-            la[it] = L.aP[k];
-            le[it] = L.cE[k];
-            ln[it] = L.cN[k];
+        for (int h = 0; h < NR; ++h) {
+            const size_t k = L.atc(ib - 2 + h, J);
+            b[h] = L.b[k];
+            ap[h] = L.aP[k];
+            ce[h] = L.cE[k];
+            cn[h] = L.cN[k];
         }
-        __syncthreads();  // the previous tile's LDS reads are done
+        T x1[NR];
 #pragma unroll
-        for (int it = 0; it < NIT; ++it) {
-            const int idx = int(threadIdx.x) + it * kBlock;
-            if (NE % kBlock != 0 && idx >= NE) break;
-            const int h = idx / HW, w = idx - h * HW;
-            const bool in = L.in(tl.I0 - 2 + h, tl.J0 - 2 + w);
-            sb[h][w] = in ? lb[it] : T(0);
-            saP[h][w] = in ? la[it] : T(0);
-            scE[h][w] = in ? le[it] : T(0);
-            scN[h][w] = in ? ln[it] : T(0);
-            sx1[h][w] = in ? om * (lb[it] / la[it]) : T(0);
+        for (int h = 0; h < NR; ++h) {
+            const bool in = L.in(ib - 2 + h, J);
+            b[h] = in ? b[h] : T(0);
+            ap[h] = in ? ap[h] : T(0);
+            ce[h] = in ? ce[h] : T(0);
+            cn[h] = in ? cn[h] : T(0);
+            x1[h] = in ? om * (b[h] / ap[h]) : T(0);
         }
-        __syncthreads();
-        for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
-            const int h = idx / HW, w = idx - h * HW;
-            const int I = tl.I0 - 2 + h, J = tl.J0 - 2 + w;
-            T v = T(0);
-            if (h >= 1 && h < HH - 1 && w >= 1 && w < HW - 1 && L.in(I, J))
-                v = sx1[h][w] + om * ((sb[h][w] - mgAx(saP, scE, scN, sx1, h, w)) / saP[h][w]);
-            sx2[h][w] = v;
+        T x2[NR];
+        x2[0] = x2[NR - 1] = T(0);
+#pragma unroll
+        for (int h = 1; h < NR - 1; ++h) {
+            const T s = ax5(cn[h - 1], x1[h - 1], fromWest(ce[h]), fromWest(x1[h]), ap[h], x1[h], ce[h],
+                            fromEast(x1[h]), cn[h], x1[h + 1]);
+            x2[h] = L.in(ib - 2 + h, J) ? x1[h] + om * ((b[h] - s) / ap[h]) : T(0);
         }
-        __syncthreads();
-        const int lane = int(threadIdx.x & 63), wv = int(threadIdx.x >> 6);
+        const bool outLane = lane >= 2 && lane < 2 + kMgTileW;
+        T r[R];
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
-            const int row = wv * R + rr, h = row + 2, w = lane + 2;
-            const int I = tl.I0 + row, J = tl.J0 + lane;
-            T r = T(0);
-            if (I < L.H && J < L.W) {
-                r = sb[h][w] - mgAx(saP, scE, scN, sx2, h, w);
-                L.xo[L.at(I, J)] = sx2[h][w];
-            }
-            sx1[h][w] = r;  // x1 is no longer read: reuse for the residual
+            const int h = rr + 2, I = ib + rr;
+            const T s = ax5(cn[h - 1], x2[h - 1], fromWest(ce[h]), fromWest(x2[h]), ap[h], x2[h], ce[h],
+                            fromEast(x2[h]), cn[h], x2[h + 1]);
+            const bool in = L.in(I, J);
+            r[rr] = in ? b[h] - s : T(0);
+            if (in && outLane) L.xo[L.at(I, J)] = x2[h];
         }
-        __syncthreads();
-        for (int idx = int(threadIdx.x); idx < (TH / 2) * (kTileW / 2); idx += kBlock) {
-            const int ci = idx / (kTileW / 2), cj = idx - ci * (kTileW / 2);
-            const int Ic = tl.I0 / 2 + ci, Jc = tl.J0 / 2 + cj;
-            if (Ic >= C.H || Jc >= C.W) continue;
-            const int h = 2 + 2 * ci, w = 2 + 2 * cj;
-            T acc = T(0);
-            acc += sx1[h][w];
-            acc += sx1[h][w + 1];
-            acc += sx1[h + 1][w];
-            acc += sx1[h + 1][w + 1];
-            C.bo[C.at(Ic, Jc)] = acc;
+        // aggregate (I, J), (I, J+1), (I+1, J), (I+1, J+1) at even lanes
+#pragma unroll
+        for (int m = 0; m < R / 2; ++m) {
+            const T e0 = fromEast(r[2 * m]), e1 = fromEast(r[2 * m + 1]);
+            const int Ic = (ib + 2 * m) >> 1, Jc = (J >> 1);
+            if (outLane && !(lane & 1) && Ic < C.H && Jc < C.W) {
+                T acc = T(0);
+                acc += r[2 * m];
+                acc += e0;
+                acc += r[2 * m + 1];
+                acc += e1;
+                C.bo[C.at(Ic, Jc)] = acc;
+            }
         }
     }
 }
@@ -1186,11 +1210,11 @@ __global__ __launch_bounds__(kTailBlock) void k_mg_tail(KArgs a) {
 
 // Up-sweep at level a.level (above the slot's tail): x3 = x2 + s z_c,
 // x4 = x3 + w (b - A x3)/aP, z = x4 + w (b - A x4)/aP. On the finest level
-// the tile also reduces r.z (b is the PCG residual there).
+// the tile also reduces r.z (b is the PCG residual there). Same wave layout
+// as the down-sweep.
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_mg_up(KArgs a) {
-    constexpr int TH = kTileH * R, HH = TH + 4, HW = kTileW + 4;
-    __shared__ T sb[HH][HW], saP[HH][HW], scE[HH][HW], scN[HH][HW], sx3[HH][HW], sx4[HH][HW];
+    constexpr int NR = R + 4;
     const int nT = *a.nTiles;
     XCD_TILE_LOOP(t, nT) {
         const MgTile tl = decodeMgTile<R>(a, t);
@@ -1200,55 +1224,52 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(KArgs a) {
         const MgView<T> L(d.lv[a.level]);
         const MgView<T> C(d.lv[a.level + 1]);
         const T om = T(d.mgOmega), sc = T(d.mgScale);
-        // every staging load first (clamped addresses), then the LDS stores
-        constexpr int NE = HH * HW, NIT = (NE + kBlock - 1) / kBlock;
-        T lb[NIT], la[NIT], le[NIT], ln[NIT], lx[NIT], lz[NIT];
+        const int lane = int(threadIdx.x & 63);
+        const int J = tl.J0 - 2 + lane, ib = tl.I0 + int(threadIdx.x >> 6) * R;
+        const int Jcl = min(max(J, 0), L.W - 1);
+        T b[NR], ap[NR], ce[NR], cn[NR], x3[NR];
+        {
+            T xv[NR], zc[NR];
 #pragma unroll
-        for (int it = 0; it < NIT; ++it) {
-            const int idx = int(threadIdx.x) + it * kBlock;
-            const int h = idx / HW, w = idx - h * HW;
-            const int I = min(max(tl.I0 - 2 + h, 0), L.H - 1), J = min(max(tl.J0 - 2 + w, 0), L.W - 1);
-            const size_t k = L.at(I, J);
-            lb[it] = L.b[k];
-            la[it] = L.aP[k];
-            le[it] = L.cE[k];
-            ln[it] = L.cN[k];
-            lx[it] = L.x[k];
-            lz[it] = C.z[C.at(I >> 1, J >> 1)];
-        }
-        __syncthreads();  // the previous tile's LDS reads are done
+            for (int h = 0; h < NR; ++h) {
+                const int Icl = min(max(ib - 2 + h, 0), L.H - 1);
+                const size_t k = L.at(Icl, Jcl);
+                b[h] = L.b[k];
+                ap[h] = L.aP[k];
+                ce[h] = L.cE[k];
+                cn[h] = L.cN[k];
+                xv[h] = L.x[k];
+                zc[h] = C.z[C.at(Icl >> 1, Jcl >> 1)];
+            }
 #pragma unroll
-        for (int it = 0; it < NIT; ++it) {
-            const int idx = int(threadIdx.x) + it * kBlock;
-            if (NE % kBlock != 0 && idx >= NE) break;
-            const int h = idx / HW, w = idx - h * HW;
-            const bool in = L.in(tl.I0 - 2 + h, tl.J0 - 2 + w);
-            sb[h][w] = in ? lb[it] : T(0);
-            saP[h][w] = in ? la[it] : T(0);
-            scE[h][w] = in ? le[it] : T(0);
-            scN[h][w] = in ? ln[it] : T(0);
-            sx3[h][w] = in ? lx[it] + sc * lz[it] : T(0);
+            for (int h = 0; h < NR; ++h) {
+                const bool in = L.in(ib - 2 + h, J);
+                b[h] = in ? b[h] : T(0);
+                ap[h] = in ? ap[h] : T(0);
+                ce[h] = in ? ce[h] : T(0);
+                cn[h] = in ? cn[h] : T(0);
+                x3[h] = in ? xv[h] + sc * zc[h] : T(0);
+            }
         }
-        __syncthreads();
-        for (int idx = int(threadIdx.x); idx < HH * HW; idx += kBlock) {
-            const int h = idx / HW, w = idx - h * HW;
-            const int I = tl.I0 - 2 + h, J = tl.J0 - 2 + w;
-            T v = T(0);
-            if (h >= 1 && h < HH - 1 && w >= 1 && w < HW - 1 && L.in(I, J))
-                v = sx3[h][w] + om * ((sb[h][w] - mgAx(saP, scE, scN, sx3, h, w)) / saP[h][w]);
-            sx4[h][w] = v;
+        T x4[NR];
+        x4[0] = x4[NR - 1] = T(0);
+#pragma unroll
+        for (int h = 1; h < NR - 1; ++h) {
+            const T s = ax5(cn[h - 1], x3[h - 1], fromWest(ce[h]), fromWest(x3[h]), ap[h], x3[h], ce[h],
+                            fromEast(x3[h]), cn[h], x3[h + 1]);
+            x4[h] = L.in(ib - 2 + h, J) ? x3[h] + om * ((b[h] - s) / ap[h]) : T(0);
         }
-        __syncthreads();
-        const int lane = int(threadIdx.x & 63), wv = int(threadIdx.x >> 6);
+        const bool outLane = lane >= 2 && lane < 2 + kMgTileW;
         double v[1] = {0.0};
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
-            const int row = wv * R + rr, h = row + 2, w = lane + 2;
-            const int I = tl.I0 + row, J = tl.J0 + lane;
-            if (I < L.H && J < L.W) {
-                const T z = sx4[h][w] + om * ((sb[h][w] - mgAx(saP, scE, scN, sx4, h, w)) / saP[h][w]);
+            const int h = rr + 2, I = ib + rr;
+            const T s = ax5(cn[h - 1], x4[h - 1], fromWest(ce[h]), fromWest(x4[h]), ap[h], x4[h], ce[h],
+                            fromEast(x4[h]), cn[h], x4[h + 1]);
+            if (outLane && L.in(I, J)) {
+                const T z = x4[h] + om * ((b[h] - s) / ap[h]);
                 L.z[L.at(I, J)] = z;
-                v[0] += double(sb[h][w]) * double(z);
+                v[0] += double(b[h]) * double(z);
             }
         }
         if (a.level == 0) blockReduceTo<1, 0u>(v, a.partials + size_t(t) * kNumPartials);
@@ -1738,11 +1759,11 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
 // ---------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------
-// Multigrid halo tiles hold six (4R+4) x 68 arrays in LDS; the host uses
-// R <= 4 (float) or R <= 2 (double) for them, which keeps them under 64 KiB.
+// Multigrid level tiles: R even (whole aggregates per wave), 4 only in
+// single precision (register footprint); the host uses no other.
 template <typename T, int R>
-constexpr bool mgTileFits() {
-    return size_t(6) * (kTileH * R + 4) * (kTileW + 4) * sizeof(T) <= 65536 && R <= (sizeof(T) == 4 ? 4 : 2);
+constexpr bool mgRowsOk() {
+    return R == 2 || (R == 4 && sizeof(T) == 4);
 }
 
 template <typename T, int R>
@@ -1759,12 +1780,14 @@ static void launchTR(Kern k, int grid, hipStream_t s, const KArgs& a) {
         case K_REF_RES: hipLaunchKernelGGL((k_refine_resid<T, R>), g, b, 0, s, a); break;
         case K_CORR:
         case K_CORR_OUTER: hipLaunchKernelGGL((k_correct<R>), g, b, 0, s, a); break;
-        case K_MG_COARSEN: hipLaunchKernelGGL((k_mg_coarsen<T, R>), g, b, 0, s, a); break;
+        case K_MG_COARSEN:
+            if constexpr (mgRowsOk<T, R>()) hipLaunchKernelGGL((k_mg_coarsen<T, R>), g, b, 0, s, a);
+            break;
         case K_MG_DOWN:
-            if constexpr (mgTileFits<T, R>()) hipLaunchKernelGGL((k_mg_down<T, R>), g, b, 0, s, a);
+            if constexpr (mgRowsOk<T, R>()) hipLaunchKernelGGL((k_mg_down<T, R>), g, b, 0, s, a);
             break;
         case K_MG_UP:
-            if constexpr (mgTileFits<T, R>()) hipLaunchKernelGGL((k_mg_up<T, R>), g, b, 0, s, a);
+            if constexpr (mgRowsOk<T, R>()) hipLaunchKernelGGL((k_mg_up<T, R>), g, b, 0, s, a);
             break;
         case K_MG_FACTOR: hipLaunchKernelGGL(k_mg_factor, g, b, 0, s, a); break;
         case K_MG_TAIL: hipLaunchKernelGGL((k_mg_tail<T>), g, dim3(kTailBlock), 0, s, a); break;

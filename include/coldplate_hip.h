@@ -290,14 +290,15 @@ int cp_batch_set_tiling(cp_batch* b, int tile_rows, int wg_per_cu);
    graph replays). */
 int cp_batch_set_graph(cp_batch* b, int use_graph);
 
-/* Rows per wave R of the finest multigrid level's 64 x 4R tiles (no
-   counterpart in the reference): 1, 2 or 4 (4 in mixed precision only);
-   0 derives it from the batch tile height. Every slot is retiled. */
+/* Rows per wave R of the finest multigrid level's 60 x 4R tiles (no
+   counterpart in the reference; coarser levels use R = 2): 2 or 4 (4 in
+   mixed precision only); 0 derives it from the batch tile height. Every
+   slot is retiled. */
 int cp_batch_set_mg_tiling(cp_batch* b, int rows);
 
 /* Autotuner (no counterpart in the reference): for every candidate tiling
-   (rows per wave R in {1, 2, 4, 8}, finest-multigrid-level rows in {1, 2, 4}
-   up to R (2 in FP64), workgroups per CU in {4, 8, 16}) replays `steps`
+   (rows per wave R in {1, 2, 4, 8}, finest-multigrid-level rows in {2, 4}
+   (2 in FP64), workgroups per CU in {4, 8, 16}) replays `steps`
    outer iterations of the admitted slots, timed with HIP events on the batch
    stream, and keeps the fastest tiling for the batch. The replays are real
    work: every candidate advances the running slots by steps + 1 outer
