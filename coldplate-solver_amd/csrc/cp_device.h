@@ -67,6 +67,9 @@ struct MgLevel {
 struct alignas(64) SlotDesc {
     int M, N, pitch;
     int tile0, ntiles, tilesX;
+    // the p' unknowns' own tile list (kernels that touch only p' unknowns):
+    // 64 x 4R tiles from node (1, 0), covering rows 1 .. M-1, columns 0 .. N-2
+    int ptile0, pntiles, ptilesX;
     int tileRows;                // rows per wave R: a tile is 64 x 4R nodes
     int nU, nV;                  // momentum DOF counts (Jacobi stop test)
     int convScheme, pseudo, localPseudo;

@@ -189,6 +189,8 @@ struct cp_batch {
     SlotState* dState = nullptr;
     int* dTileSlot = nullptr;
     int* dNTiles = nullptr;
+    int* dPTileSlot = nullptr;  // p' unknowns' tile list
+    int* dPNTiles = nullptr;
     // multigrid tile lists: level l's list at dMgTileSlot + l * maxTiles
     int* dMgTileSlot = nullptr;
     int* dMgNTiles = nullptr;
@@ -205,13 +207,14 @@ struct cp_batch {
     PlaneReq* hPlaneReq = nullptr;
     PlaneOut* hPlaneOut = nullptr;
     int totalTiles = 0;
+    int totalPTiles = 0;
     // graph mode
     hipGraphExec_t graphExec = nullptr;
     int graphJ = -1, graphP = -1, graphR = -1, graphL = -1;
     // measurement: k_pcg_q timing bursts inside cp_serve (every kProbeEvery
     // polls), as (ms per launch, p' unknowns of the running slots)
     int probeKernel = -1;
-    hipEvent_t burstEv[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> burstEv;  // event pairs of the timing bursts
     std::vector<double> probeMs, probeUnknowns;
     // tile height R (rows per wave) shared by every slot: 1, 2, 4 or 8
     int tileRows = 4;
@@ -263,9 +266,21 @@ KArgs kargs(cp_batch* b) {
     return a;
 }
 
+// kernels that read and write p' unknowns only run over the p' tile list
+bool pOnly(Kern k) { return k == K_PCG_Q || k == K_PCG_U || k == K_REF_ACC || k == K_REF_RES; }
+
+KArgs kargsFor(cp_batch* b, Kern k) {
+    KArgs a = kargs(b);
+    if (pOnly(k)) {
+        a.tileSlot = b->dPTileSlot;
+        a.nTiles = b->dPNTiles;
+    }
+    return a;
+}
+
 int launch(cp_batch* b, Kern k, int grid) {
     const hipError_t e =
-        launchKernel(k, b->precision == CP_PREC_MIXED, b->tileRows, grid, b->maxSlots, b->stream, kargs(b));
+        launchKernel(k, b->precision == CP_PREC_MIXED, b->tileRows, grid, b->maxSlots, b->stream, kargsFor(b, k));
     if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
     return CP_OK;
 }
@@ -341,6 +356,15 @@ int rebuildTiles(cp_batch* b) {
     }
     if (ts.size() > b->maxTiles) return setError(&err, CP_E_CAPACITY, "tile table full");
     b->totalTiles = int(ts.size());
+    std::vector<int> pts;
+    for (int s = 0; s < b->maxSlots; ++s) {
+        if (!b->slots[s].used) continue;
+        SlotDesc& d = b->hDesc[s];
+        d.ptile0 = int(pts.size());
+        for (int k = 0; k < d.pntiles; ++k) pts.push_back(s);
+    }
+    if (pts.size() > b->maxTiles) return setError(&err, CP_E_CAPACITY, "tile table full");
+    b->totalPTiles = int(pts.size());
     for (int s = 0; s < b->maxSlots; ++s)
         if (b->slots[s].used && (b->hDesc[s].dpConv || b->hDesc[s].slopeGate)) b->dpAny = 1;
     int counts[kMaxLevels];
@@ -369,6 +393,9 @@ int rebuildTiles(cp_batch* b) {
     if (!ts.empty())
         HIP_TRY(hipMemcpyAsync(b->dTileSlot, ts.data(), ts.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
     HIP_TRY(hipMemcpyAsync(b->dNTiles, &b->totalTiles, sizeof(int), hipMemcpyHostToDevice, b->stream));
+    if (!pts.empty())
+        HIP_TRY(hipMemcpyAsync(b->dPTileSlot, pts.data(), pts.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(b->dPNTiles, &b->totalPTiles, sizeof(int), hipMemcpyHostToDevice, b->stream));
     HIP_TRY(hipMemcpyAsync(b->dDesc, b->hDesc.data(), sizeof(SlotDesc) * b->maxSlots, hipMemcpyHostToDevice,
                            b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
@@ -460,6 +487,9 @@ void setTiling(SlotDesc& d, int tileRows, int mgRows0) {
     const int rowsPerTile = kTileH * d.tileRows;
     const int tilesY = (d.M + 1 + rowsPerTile - 1) / rowsPerTile;
     d.ntiles = d.tilesX * tilesY;
+    // p' unknowns: rows 1 .. M-1 from row 1, columns 1 .. N-2 from column 0
+    d.ptilesX = (d.N - 1 + kTileW - 1) / kTileW;
+    d.pntiles = d.ptilesX * ((d.M - 1 + rowsPerTile - 1) / rowsPerTile);
     for (int l = 0; l < d.nLevels; ++l) {
         MgLevel& L = d.lv[l];
         const int rows = kTileH * (l == 0 ? mgRows0 : 2);
@@ -652,6 +682,8 @@ int batchCreate(int precision, size_t budget, int maxSlots, cp_batch** out, std:
     HIP_TRY(hipMalloc(&b->dState, sizeof(SlotState) * maxSlots));
     HIP_TRY(hipMalloc(&b->dTileSlot, sizeof(int) * b->maxTiles));
     HIP_TRY(hipMalloc(&b->dNTiles, sizeof(int)));
+    HIP_TRY(hipMalloc(&b->dPTileSlot, sizeof(int) * b->maxTiles));
+    HIP_TRY(hipMalloc(&b->dPNTiles, sizeof(int)));
     HIP_TRY(hipMalloc(&b->dMgTileSlot, sizeof(int) * b->maxTiles * kMaxLevels));
     HIP_TRY(hipMalloc(&b->dMgNTiles, sizeof(int) * kMaxLevels));
     HIP_TRY(hipMalloc(&b->dPartials, sizeof(double) * kNumPartials * b->maxTiles));
@@ -667,6 +699,7 @@ int batchCreate(int precision, size_t budget, int maxSlots, cp_batch** out, std:
     HIP_TRY(hipMemcpyAsync(b->dState, b->hState, sizeof(SlotState) * maxSlots, hipMemcpyHostToDevice, b->stream));
     const int zero = 0;
     HIP_TRY(hipMemcpyAsync(b->dNTiles, &zero, sizeof(int), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(b->dPNTiles, &zero, sizeof(int), hipMemcpyHostToDevice, b->stream));
     HIP_TRY(hipMemsetAsync(b->dMgNTiles, 0, sizeof(int) * kMaxLevels, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
     *out = b;
@@ -684,6 +717,8 @@ void batchDestroy(cp_batch* b) {
     (void)hipFree(b->dState);
     (void)hipFree(b->dTileSlot);
     (void)hipFree(b->dNTiles);
+    (void)hipFree(b->dPTileSlot);
+    (void)hipFree(b->dPNTiles);
     (void)hipFree(b->dMgTileSlot);
     (void)hipFree(b->dMgNTiles);
     (void)hipFree(b->dPartials);
@@ -1334,24 +1369,34 @@ int cp_batch_probe_burst(cp_batch* b, int kernel, int launches, double* ms_per_l
 }
 
 namespace {
+// Each launch is bracketed by its own pair of events, so the average is the
+// kernel's own duration: the gaps between launches (larger under a tracing
+// profiler) stay outside every pair.
 int burst(cp_batch* b, int launches, double* ms_per_launch) {
     std::string& err = b->err;
-    for (hipEvent_t& ev : b->burstEv)
-        if (!ev) HIP_TRY(hipEventCreate(&ev));
-    KArgs a = kargs(b);
+    while (b->burstEv.size() < size_t(2 * launches)) {
+        hipEvent_t ev = nullptr;
+        HIP_TRY(hipEventCreate(&ev));
+        b->burstEv.push_back(ev);
+    }
+    KArgs a = kargsFor(b, K_PCG_Q);
     a.force = 1;
     const int grid = gridFor(b, 1 << 30);
     const bool f32 = b->precision == CP_PREC_MIXED;
-    HIP_TRY(hipEventRecord(b->burstEv[0], b->stream));
     for (int k = 0; k < launches; ++k) {
+        HIP_TRY(hipEventRecord(b->burstEv[2 * k], b->stream));
         const hipError_t e = launchWorkOnly(K_PCG_Q, f32, b->tileRows, grid, b->stream, a);
         if (e != hipSuccess) return setError(&err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+        HIP_TRY(hipEventRecord(b->burstEv[2 * k + 1], b->stream));
     }
-    HIP_TRY(hipEventRecord(b->burstEv[1], b->stream));
-    HIP_TRY(hipEventSynchronize(b->burstEv[1]));
-    float t = 0.f;
-    HIP_TRY(hipEventElapsedTime(&t, b->burstEv[0], b->burstEv[1]));
-    *ms_per_launch = double(t) / launches;
+    HIP_TRY(hipEventSynchronize(b->burstEv[2 * launches - 1]));
+    double tot = 0.0;
+    for (int k = 0; k < launches; ++k) {
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, b->burstEv[2 * k], b->burstEv[2 * k + 1]));
+        tot += double(t);
+    }
+    *ms_per_launch = tot / launches;
     return CP_OK;
 }
 }  // namespace

@@ -77,8 +77,8 @@ __device__ __forceinline__ void blockReduceTo(double (&v)[NQ], double* dst) {
     }
 }
 
-// Sums (or maxes) the partials of one slot's tiles in tile order; the result
-// is valid in every thread of the block. Each thread issues its loads in
+// Sums (or maxes) the partials of one slot's tiles in tile order (ntiles >=
+// 1); the result is valid in every thread of the block. Each thread issues its loads in
 // groups of 8 before combining them, so the latency is paid once per group.
 template <int NQ, unsigned MaxMask>
 __device__ __forceinline__ void slotReduce(const double* partials, int tile0, int ntiles, double (&out)[NQ]) {
@@ -86,18 +86,22 @@ __device__ __forceinline__ void slotReduce(const double* partials, int tile0, in
     double v[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) v[q] = 0.0;
-    constexpr int G = 8;
-    for (int base = 0; base < ntiles; base += kBlock * G) {
-        double x[G][NQ];
+    constexpr int kG = 8;
+    for (int base = 0; base < ntiles; base += kBlock * kG) {
+        double x[kG][NQ];
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
+        for (int g = 0; g < kG; ++g) {
+            // clamped index: every load is issued, the ones past the end discarded
             const int k = base + g * kBlock + int(threadIdx.x);
+            const size_t kc = size_t(tile0 + min(k, ntiles - 1)) * kNumPartials;
 #pragma unroll
-            for (int q = 0; q < NQ; ++q)
-                x[g][q] = (k < ntiles) ? partials[size_t(tile0 + k) * kNumPartials + q] : 0.0;
+            for (int q = 0; q < NQ; ++q) {
+                const double xv = G(partials)[kc + q];
+                x[g][q] = (k < ntiles) ? xv : 0.0;
+            }
         }
 #pragma unroll
-        for (int g = 0; g < G; ++g)
+        for (int g = 0; g < kG; ++g)
 #pragma unroll
             for (int q = 0; q < NQ; ++q) v[q] = ((MaxMask >> q) & 1u) ? nanmax(v[q], x[g][q]) : v[q] + x[g][q];
     }
@@ -123,6 +127,23 @@ struct Tile {
 // A tile is 64 columns x 4R rows (R = rows per wave, a compile-time launch
 // parameter shared by every slot of a batch): wave w of the workgroup owns
 // rows [i0 + w R, i0 + (w+1) R).
+// Tile t of the p' list (a slot's ptile0 .. ptile0 + pntiles - 1): rows from
+// 1, so the tiles hold no row outside the unknowns' range 1 .. M-1.
+template <int R>
+__device__ __forceinline__ Tile decodePTile(const KArgs& a, int t) {
+    Tile tl;
+    tl.s = __builtin_amdgcn_readfirstlane(a.tileSlot[t]);
+    const SlotDesc& d = a.desc[tl.s];
+    const int lt = t - d.ptile0;
+    const int ty = lt / d.ptilesX;
+    const int tx = lt - ty * d.ptilesX;
+    tl.i0 = 1 + ty * (kTileH * R);
+    tl.j0 = tx * kTileW;
+    tl.i = tl.i0 + int(threadIdx.x >> 6) * R;
+    tl.j = tl.j0 + int(threadIdx.x & 63);
+    return tl;
+}
+
 template <int R>
 __device__ __forceinline__ Tile decodeTile(const KArgs& a, int t) {
     Tile tl;
@@ -160,6 +181,13 @@ __device__ __forceinline__ double clampv(double x, double m) { return fmax(-m, f
 __device__ __forceinline__ size_t clampIdx(int i, int j, int M, int P) {
     return size_t(min(max(i, 0), M)) * P + min(max(j, 0), P - 1);
 }
+
+// lane l receives lane l-1's value (west) / lane l+1's value (east); lanes 0
+// and 63 receive their own, which only halo lanes consume
+template <typename T>
+__device__ __forceinline__ T fromWest(T x) { return __shfl_up(x, 1, 64); }
+template <typename T>
+__device__ __forceinline__ T fromEast(T x) { return __shfl_down(x, 1, 64); }
 
 // ---------------------------------------------------------------------------
 // momentum assembly (momentum_solver.cpp:185-298, delta form)
@@ -369,17 +397,56 @@ __device__ __forceinline__ T at(const T* f, int i, int j, int M, int jmax, int P
     return (i >= 0 && i <= M && j >= 0 && j <= jmax) ? f[size_t(i) * P + j] : T(0);
 }
 
-// Jacobi sweep on LDS halo tiles: the workgroup stages delta of U and V for
-// its (4R+2) x 66 halo region (rows i0-1 .. i0+4R, columns j0-1 .. j0+64)
-// with coalesced loads, then each thread updates its R nodes from LDS; the
-// coefficient arrays are read once per node. delta is zero outside each
-// system's unknowns, so the product needs no masks.
+// One Jacobi sweep of one system for a wave's R rows x 64 columns, all loads
+// issued up front: delta for the rows and the rows above and below (lanes
+// 0-31 also at column j0-1, lanes 32-63 at column j0+64, for the lanes at
+// the wave's edges), and the coefficients. delta is zero outside each
+// system's unknowns and is taken as zero outside the node grid, so the
+// product needs no masks.
+template <typename T, int R>
+__device__ __forceinline__ void jacobiWave(const Tile& tl, int M, int N, int P, int iMax, int jMax,
+                                           gp<const T> const (&cf)[M_D0], gp<const T> src, gp<T> dst,
+                                           double& v) {
+    const int lane = int(threadIdx.x & 63);
+    const int j = tl.j, ib = tl.i;
+    const int je = lane < 32 ? tl.j0 - 1 : tl.j0 + kTileW;
+    T dl[R + 2], de[R], c[R][M_D0];
+#pragma unroll
+    for (int h = 0; h < R + 2; ++h) dl[h] = src[clampIdx(ib - 1 + h, j, M, P)];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        de[r] = src[clampIdx(ib + r, je, M, P)];
+        const size_t k = clampIdx(ib + r, j, M, P);
+#pragma unroll
+        for (int q = 0; q < M_D0; ++q) c[r][q] = cf[q][k];
+    }
+    const bool jIn = j >= 0 && j <= N, jeIn = je >= 0 && je <= N;
+#pragma unroll
+    for (int h = 0; h < R + 2; ++h) dl[h] = (jIn && ib - 1 + h >= 0 && ib - 1 + h <= M) ? dl[h] : T(0);
+#pragma unroll
+    for (int r = 0; r < R; ++r) de[r] = (jeIn && ib + r <= M) ? de[r] : T(0);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = ib + r;
+        const T dc = dl[r + 1];
+        T dW = fromWest(dc), dE = fromEast(dc);
+        dW = lane == 0 ? de[r] : dW;
+        dE = lane == 63 ? de[r] : dE;
+        if (i >= 1 && i <= iMax && j >= 1 && j <= jMax) {
+            const T ap = c[r][M_AP];
+            const T Ax = ap * dc - c[r][M_AE] * dE - c[r][M_AW] * dW - c[r][M_AN] * dl[r + 2] - c[r][M_AS] * dl[r];
+            const T res = c[r][M_R0] - Ax;
+            dst[size_t(i) * P + j] = dc + res / ap;
+            v += double(res) * double(res);
+        }
+    }
+}
+
+// Jacobi sweep: U and V in the same launch, each system skipped once its
+// stop test has fired. U unknowns: i in [1, M-1], j in [1, N-2]; V: i in
+// [1, M-2], j in [1, N-1].
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_jacobi(KArgs a) {
-    constexpr int TH = kTileH * R, HH = TH + 2, HW = kTileW + 2;
-    constexpr int NE = HH * HW, NIT = (NE + kBlock - 1) / kBlock;
-    __shared__ T sU[HH][HW];
-    __shared__ T sV[HH][HW];
     const int nT = *a.nTiles;
     XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodeTile<R>(a, t);
@@ -395,69 +462,13 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(KArgs a) {
             cu[q] = G(static_cast<const T*>(d.mu[q]));
             cv[q] = G(static_cast<const T*>(d.mv[q]));
         }
-        const auto srcU = G(static_cast<const T*>(d.mu[M_D0 + pu]));
-        const auto srcV = G(static_cast<const T*>(d.mv[M_D0 + pv]));
-        const auto dstU = G(static_cast<T*>(d.mu[M_D0 + (pu ^ 1)]));
-        const auto dstV = G(static_cast<T*>(d.mv[M_D0 + (pv ^ 1)]));
-        // every load of the tile first, from clamped addresses: the halo of
-        // delta for both systems, then the coefficients of this thread's rows
-        T lu[NIT], lv[NIT];
-#pragma unroll
-        for (int it = 0; it < NIT; ++it) {
-            const int idx = int(threadIdx.x) + it * kBlock;
-            const int hi = idx / HW, hj = idx - hi * HW;
-            const size_t k = clampIdx(tl.i0 - 1 + hi, tl.j0 - 1 + hj, M, P);
-            lu[it] = doneU ? T(0) : srcU[k];
-            lv[it] = doneV ? T(0) : srcV[k];
-        }
-        T au[R][M_D0], av[R][M_D0];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const size_t k = clampIdx(tl.i + r, tl.j, M, P);
-#pragma unroll
-            for (int q = 0; q < M_D0; ++q) {
-                au[r][q] = doneU ? T(0) : cu[q][k];
-                av[r][q] = doneV ? T(0) : cv[q][k];
-            }
-        }
-        __syncthreads();  // the previous tile's LDS reads are done
-#pragma unroll
-        for (int it = 0; it < NIT; ++it) {
-            const int idx = int(threadIdx.x) + it * kBlock;
-            if (NE % kBlock != 0 && idx >= NE) break;
-            const int hi = idx / HW, hj = idx - hi * HW;
-            const int i = tl.i0 - 1 + hi, j = tl.j0 - 1 + hj;
-            const bool in = i >= 0 && i <= M && j >= 0 && j <= N;  // zero outside, as at()
-            sU[hi][hj] = in ? lu[it] : T(0);
-            sV[hi][hj] = in ? lv[it] : T(0);
-        }
-        __syncthreads();
-        const int lane = int(threadIdx.x & 63), w = int(threadIdx.x >> 6);
         double v[2] = {0.0, 0.0};
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int row = w * R + r, i = tl.i0 + row, j = tl.j0 + lane;
-            const size_t k = size_t(i) * P + j;
-            // U unknowns: i in [1, M-1], j in [1, N-2]; V: i in [1, M-2], j in [1, N-1]
-            if (!doneU && i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2) {
-                const T ap = au[r][M_AP];
-                const T dc = sU[row + 1][lane + 1];
-                const T Ax = ap * dc - au[r][M_AE] * sU[row + 1][lane + 2] - au[r][M_AW] * sU[row + 1][lane] -
-                             au[r][M_AN] * sU[row + 2][lane + 1] - au[r][M_AS] * sU[row][lane + 1];
-                const T res = au[r][M_R0] - Ax;
-                dstU[k] = dc + res / ap;
-                v[0] += double(res) * double(res);
-            }
-            if (!doneV && i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1) {
-                const T ap = av[r][M_AP];
-                const T dc = sV[row + 1][lane + 1];
-                const T Ax = ap * dc - av[r][M_AE] * sV[row + 1][lane + 2] - av[r][M_AW] * sV[row + 1][lane] -
-                             av[r][M_AN] * sV[row + 2][lane + 1] - av[r][M_AS] * sV[row][lane + 1];
-                const T res = av[r][M_R0] - Ax;
-                dstV[k] = dc + res / ap;
-                v[1] += double(res) * double(res);
-            }
-        }
+        if (!doneU)
+            jacobiWave<T, R>(tl, M, N, P, M - 1, N - 2, cu, G(static_cast<const T*>(d.mu[M_D0 + pu])),
+                             G(static_cast<T*>(d.mu[M_D0 + (pu ^ 1)])), v[0]);
+        if (!doneV)
+            jacobiWave<T, R>(tl, M, N, P, M - 2, N - 1, cv, G(static_cast<const T*>(d.mv[M_D0 + pv])),
+                             G(static_cast<T*>(d.mv[M_D0 + (pv ^ 1)])), v[1]);
         blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials);
     }
 }
@@ -694,7 +705,7 @@ template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_pcg_q(KArgs a) {
     const int nT = *a.nTiles;
     XCD_TILE_LOOP(t, nT) {
-        const Tile tl = decodeTile<R>(a, t);
+        const Tile tl = decodePTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
         // read the gate fields together and test them without short-circuit
@@ -723,7 +734,7 @@ template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
     const int nT = *a.nTiles;
     XCD_TILE_LOOP(t, nT) {
-        const Tile tl = decodeTile<R>(a, t);
+        const Tile tl = decodePTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
         const int status = st.status, pdone = st.pdone, phase = st.stepPhase, kk = st.pk;
@@ -770,7 +781,7 @@ template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_refine_accum(KArgs a) {
     const int nT = *a.nTiles;
     XCD_TILE_LOOP(t, nT) {
-        const Tile tl = decodeTile<R>(a, t);
+        const Tile tl = decodePTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
         if (st.status != ST_RUNNING || st.stepPhase) continue;
@@ -790,7 +801,7 @@ template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_refine_resid(KArgs a) {
     const int nT = *a.nTiles;
     XCD_TILE_LOOP(t, nT) {
-        const Tile tl = decodeTile<R>(a, t);
+        const Tile tl = decodePTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
         if (st.status != ST_RUNNING || st.stepPhase) continue;
@@ -897,13 +908,6 @@ __device__ __forceinline__ T ax5(T cS, T xS, T cW, T xW, T aPc, T xC, T cEc, T x
     s += -cNc * xN;
     return s;
 }
-
-// lane l receives lane l-1's value (west) / lane l+1's value (east); lanes 0
-// and 63 receive their own, which only halo lanes consume
-template <typename T>
-__device__ __forceinline__ T fromWest(T x) { return __shfl_up(x, 1, 64); }
-template <typename T>
-__device__ __forceinline__ T fromEast(T x) { return __shfl_down(x, 1, 64); }
 
 // Galerkin coarse operator of level a.level (>= 1) from level a.level - 1:
 // aP_c = sum of the aggregate's aP minus twice its internal couplings,
@@ -1598,8 +1602,20 @@ __global__ __launch_bounds__(kBlock) void k_dp_monitor(KArgs a) {
 __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int outer) {
     const int s = blockIdx.x;
     const SlotDesc& d = a.desc[s];
-    SlotState& st = a.state[s];
-    if (d.ntiles == 0 || st.status != ST_RUNNING) return;
+    // The slot's state is read into LDS in one step (a word per thread),
+    // updated there by thread 0 and written back whole, so its fields cost
+    // no dependent global round trips. Nothing else touches the state while
+    // this kernel runs.
+    __shared__ SlotState sst;
+    constexpr int kWords = int(sizeof(SlotState) / sizeof(uint32_t));
+    static_assert(kWords <= kBlock, "one word per thread");
+    const auto gw = G(reinterpret_cast<uint32_t*>(&a.state[s]));
+    uint32_t* lw = reinterpret_cast<uint32_t*>(&sst);
+    if (int(threadIdx.x) < kWords) lw[threadIdx.x] = gw[threadIdx.x];
+    const int ntiles = d.ntiles;
+    __syncthreads();
+    SlotState& st = sst;
+    if (ntiles == 0 || st.status != ST_RUNNING) return;
     switch (phase) {
         case PH_JAC:
             if (st.jdone[0] && st.jdone[1]) return;
@@ -1669,7 +1685,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
         }
         case PH_PCG_Q: {
             double tot[1];
-            slotReduce<1, 0u>(a.partials, d.tile0, d.ntiles, tot);
+            slotReduce<1, 0u>(a.partials, d.ptile0, d.pntiles, tot);
             if (threadIdx.x == 0) {
                 if (fabs(tot[0]) < 1e-20) {  // pressure_solver.cpp:158
                     st.pdone = 1;
@@ -1682,7 +1698,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
         }
         case PH_PCG_U: {
             double tot[2];
-            slotReduce<2, 0u>(a.partials, d.tile0, d.ntiles, tot);
+            slotReduce<2, 0u>(a.partials, d.ptile0, d.pntiles, tot);
             if (threadIdx.x == 0) {
                 st.rr = tot[0];
                 if (sqrt(tot[0]) < d.pTol) {
@@ -1711,7 +1727,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
         }
         case PH_REF_RES: {
             double tot[2];
-            slotReduce<2, 0u>(a.partials, d.tile0, d.ntiles, tot);
+            slotReduce<2, 0u>(a.partials, d.ptile0, d.pntiles, tot);
             if (threadIdx.x == 0) {
                 st.pcycle += 1;
                 if (sqrt(tot[0]) < d.pTol) {
@@ -1754,6 +1770,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
         default:
             break;
     }
+    __syncthreads();
+    if (int(threadIdx.x) < kWords) gw[threadIdx.x] = lw[threadIdx.x];
 }
 
 // ---------------------------------------------------------------------------

@@ -7,7 +7,8 @@ CSV files (--output-format csv):
   * --pmc FETCH_SIZE         -> <fetch>
   * --pmc WRITE_SIZE         -> <write>
 and the bench's own JSON line from the kernel-trace run's log (its live,
-event-timed average launch duration of the same kernel).
+event-timed average launch duration of the same kernel), plus, with --bench,
+the JSON line of the same command run without the profiler.
 
 The byte counters are scaled against a device-to-device copy of known size
 run in the same process (bench.py --calibrate-copy MIB): the copy reads and
@@ -105,6 +106,7 @@ def main() -> int:
     ap.add_argument("--fetch", required=True)
     ap.add_argument("--write", required=True)
     ap.add_argument("--log", default="")
+    ap.add_argument("--bench", default="", help="JSON line of the same command run without the profiler")
     ap.add_argument("--copy-mib", type=int, required=True)
     ap.add_argument("--kernel", default="k_pcg_q")
     ap.add_argument("--workload", default="network_1024x512, 8 geometries, mixed precision")
@@ -128,6 +130,7 @@ def main() -> int:
     traffic = (f_avg * f_factor + w_avg * w_factor) * 1024.0
 
     b = bench_line(a.log)
+    bu = bench_line(a.bench)
     krow = next(r for r in stats if a.kernel in r["Name"])
     rocprof_avg_us = float(krow["AverageNs"]) / 1e3
     live_us = b["roofline"]["avg_launch_us"] if b else None
@@ -152,6 +155,11 @@ def main() -> int:
         lines.append(f"* achieved (bench, events): {b['roofline']['achieved']:.1f} GB/s = "
                      f"{100 * b['roofline']['frac']:.1f} % of 8000 GB/s; "
                      f"with the rocprof average: {alg / (rocprof_avg_us * 1e3):.1f} GB/s")
+    if bu:
+        r = bu["roofline"]
+        lines.append(f"* the same command without the profiler: {r['avg_launch_us']:.2f} µs over "
+                     f"{r['launches_timed']} sampled launches, {r['achieved']:.1f} GB/s = {100 * r['frac']:.1f} % "
+                     f"of 8000 GB/s (tiling {bu['config'].get('tiling')})")
     lines += ["", "## HBM traffic (rocprofv3 --pmc, one counter per run)", ""]
     lines.append(f"* calibration: {a.copy_mib} MiB device-to-device copy reads FETCH_SIZE {f_copy:.0f} KiB "
                  f"and WRITE_SIZE {w_copy:.0f} KiB -> factors FETCH x{f_factor:.3f}, WRITE x{w_factor:.3f}")
@@ -162,7 +170,8 @@ def main() -> int:
     open(a.out + ".md", "w").write("\n".join(lines) + "\n")
     json.dump({"kernel": kname, "workload": a.workload, "fetch_kib": f_avg, "write_kib": w_avg,
                "fetch_factor": f_factor, "write_factor": w_factor, "traffic_bytes": traffic,
-               "rocprof_avg_us": rocprof_avg_us, "bench_avg_us": live_us, "bytes_per_launch": alg},
+               "rocprof_avg_us": rocprof_avg_us, "bench_avg_us": live_us, "bytes_per_launch": alg,
+               "bench_unprofiled_avg_us": bu["roofline"]["avg_launch_us"] if bu else None},
               open(a.out + ".json", "w"), indent=1)
     print("\n".join(lines))
     return 0

