@@ -91,6 +91,9 @@ def parse():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run 3 device-to-device copies of MIB MiB (byte-counter calibration)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tiling (no autotuning pass)")
+    ap.add_argument("--tiling", default="", metavar="R,MG",
+                    help="fixed tiling instead of autotuning: rows per wave R and finest multigrid rows MG "
+                         "(counter runs use the tiling the autotuner chose)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the step's kernels directly instead of replaying the captured graph")
     ap.add_argument("--profile-steps", type=int, default=0,
@@ -344,7 +347,12 @@ def main() -> int:
     if a.no_graph:
         b.set_graph(False)
     tuned = None
-    if not a.no_tune:
+    if a.tiling:
+        r_, m_ = (int(x) for x in a.tiling.split(","))
+        b.set_tiling(r_, 16)
+        b.set_mg_tiling(m_)
+        tuned = {"tile_rows": r_, "mg_rows": m_, "wg_per_cu": 16, "fixed": True}
+    elif not a.no_tune:
         # autotune the tiling on the workload's own geometries, then start
         # every measured stream from an empty batch
         q0 = [served_params(p, a, a.eps) for p, _ in GEOMS]
@@ -450,6 +458,7 @@ def main() -> int:
             "traffic": traffic,
             "avg_launch_us": round(avg_ms * 1e3, 2),
             "bytes_per_launch": int(per_launch),
+            "bytes_per_launch_full_batch": int(PCG_Q_BYTES_FP32 * sum(n_pressure(p.M, p.N) for p, _ in GEOMS[:B])),
             "launches_timed": n_launch,
         },
         "cpu_baseline": cpu,

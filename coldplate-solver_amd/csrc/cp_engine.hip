@@ -1369,12 +1369,12 @@ int cp_batch_probe_burst(cp_batch* b, int kernel, int launches, double* ms_per_l
 }
 
 namespace {
-// Each launch is bracketed by its own pair of events, so the average is the
-// kernel's own duration: the gaps between launches (larger under a tracing
-// profiler) stay outside every pair.
+// One pair of events around `launches` back-to-back launches: the average
+// is the kernel's duration plus the (small) dispatch gap between
+// consecutive launches on one queue, as a timing loop measures it.
 int burst(cp_batch* b, int launches, double* ms_per_launch) {
     std::string& err = b->err;
-    while (b->burstEv.size() < size_t(2 * launches)) {
+    while (b->burstEv.size() < 2) {
         hipEvent_t ev = nullptr;
         HIP_TRY(hipEventCreate(&ev));
         b->burstEv.push_back(ev);
@@ -1383,20 +1383,16 @@ int burst(cp_batch* b, int launches, double* ms_per_launch) {
     a.force = 1;
     const int grid = gridFor(b, 1 << 30);
     const bool f32 = b->precision == CP_PREC_MIXED;
+    HIP_TRY(hipEventRecord(b->burstEv[0], b->stream));
     for (int k = 0; k < launches; ++k) {
-        HIP_TRY(hipEventRecord(b->burstEv[2 * k], b->stream));
         const hipError_t e = launchWorkOnly(K_PCG_Q, f32, b->tileRows, grid, b->stream, a);
         if (e != hipSuccess) return setError(&err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
-        HIP_TRY(hipEventRecord(b->burstEv[2 * k + 1], b->stream));
     }
-    HIP_TRY(hipEventSynchronize(b->burstEv[2 * launches - 1]));
-    double tot = 0.0;
-    for (int k = 0; k < launches; ++k) {
-        float t = 0.f;
-        HIP_TRY(hipEventElapsedTime(&t, b->burstEv[2 * k], b->burstEv[2 * k + 1]));
-        tot += double(t);
-    }
-    *ms_per_launch = tot / launches;
+    HIP_TRY(hipEventRecord(b->burstEv[1], b->stream));
+    HIP_TRY(hipEventSynchronize(b->burstEv[1]));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, b->burstEv[0], b->burstEv[1]));
+    *ms_per_launch = double(t) / launches;
     return CP_OK;
 }
 }  // namespace

@@ -33,6 +33,19 @@ __device__ __forceinline__ gp<T> G(T* p) {
     return (gp<T>)p;
 }
 
+// Element k of a field array, with the byte offset formed in 32 bits and
+// added to the (workgroup-uniform) base: this compiles to a global access
+// with a scalar base and a 32-bit vector offset, without 64-bit address
+// arithmetic per lane. Every field array is far below 4 GiB.
+template <typename T>
+__device__ __forceinline__ T ld32(gp<const T> base, uint32_t k) {
+    return *reinterpret_cast<gp<const T>>(reinterpret_cast<gp<const char>>(base) + k * uint32_t(sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void st32(gp<T> base, uint32_t k, T v) {
+    *reinterpret_cast<gp<T>>(reinterpret_cast<gp<char>>(base) + k * uint32_t(sizeof(T))) = v;
+}
+
 // ---------------------------------------------------------------------------
 // reductions
 // ---------------------------------------------------------------------------
@@ -180,6 +193,9 @@ __device__ __forceinline__ double clampv(double x, double m) { return fmax(-m, f
 // every load of a tile is issued unconditionally, ahead of any use.
 __device__ __forceinline__ size_t clampIdx(int i, int j, int M, int P) {
     return size_t(min(max(i, 0), M)) * P + min(max(j, 0), P - 1);
+}
+__device__ __forceinline__ uint32_t clampIdx32(int i, int j, int M, int P) {
+    return uint32_t(min(max(i, 0), M)) * uint32_t(P) + uint32_t(min(max(j, 0), P - 1));
 }
 
 // lane l receives lane l-1's value (west) / lane l+1's value (east); lanes 0
@@ -640,10 +656,10 @@ __device__ __forceinline__ T pAt(gp<const T> r, gp<const T> aP, gp<const T> Z, g
 }
 
 template <typename T, bool HasZ, bool First>
-__device__ __forceinline__ T pLoad(gp<const T> r, gp<const T> aP, gp<const T> Z, gp<const T> pold, size_t k,
+__device__ __forceinline__ T pLoad(gp<const T> r, gp<const T> aP, gp<const T> Z, gp<const T> pold, uint32_t k,
                                    T beta) {
-    const T z = HasZ ? Z[k] : r[k] / aP[k];
-    return First ? z : z + beta * pold[k];
+    const T z = HasZ ? ld32(Z, k) : ld32(r, k) / ld32(aP, k);
+    return First ? z : z + beta * ld32(pold, k);
 }
 
 // q = A p_k for one wave's R rows x 64 columns. The wave loads p_k for its
@@ -668,20 +684,20 @@ __device__ __forceinline__ void pcgQWave(const SlotDesc& d, const Tile& tl, int 
 #pragma unroll
     for (int h = 0; h < R + 2; ++h) {
         const int i = ib - 1 + h;
-        const T pc = pLoad<T, HasZ, First>(r, aP, Z, pold, clampIdx(i, j, M, P), beta);
-        const T px = pLoad<T, HasZ, First>(r, aP, Z, pold, clampIdx(i, je, M, P), beta);
+        const T pc = pLoad<T, HasZ, First>(r, aP, Z, pold, clampIdx32(i, j, M, P), beta);
+        const T px = pLoad<T, HasZ, First>(r, aP, Z, pold, clampIdx32(i, je, M, P), beta);
         p[h] = isPDof(i, j, M, N) ? pc : T(0);
         pe[h] = isPDof(i, je, M, N) ? px : T(0);
     }
 #pragma unroll
     for (int rr = 0; rr < R; ++rr) {
-        const size_t k = clampIdx(ib + rr, j, M, P);
-        ap[rr] = aP[k];
-        ce[rr] = cE[k];
-        cee[rr] = cE[clampIdx(ib + rr, je, M, P)];
+        const uint32_t k = clampIdx32(ib + rr, j, M, P);
+        ap[rr] = ld32(aP, k);
+        ce[rr] = ld32(cE, k);
+        cee[rr] = ld32(cE, clampIdx32(ib + rr, je, M, P));
     }
 #pragma unroll
-    for (int h = 0; h < R + 1; ++h) cn[h] = cN[clampIdx(ib - 1 + h, j, M, P)];
+    for (int h = 0; h < R + 1; ++h) cn[h] = ld32(cN, clampIdx32(ib - 1 + h, j, M, P));
 #pragma unroll
     for (int rr = 0; rr < R; ++rr) {
         const int i = ib + rr;
@@ -693,9 +709,9 @@ __device__ __forceinline__ void pcgQWave(const SlotDesc& d, const Tile& tl, int 
         // CSR column order S, W, P, E, N (pressure_solver.cpp:108-118)
         const T q = -cn[rr] * p[rr] - cEw * pW + ap[rr] * pC - ce[rr] * pE - cn[rr + 1] * p[rr + 2];
         if (isPDof(i, j, M, N)) {
-            const size_t k = size_t(i) * P + j;
-            pnew[k] = pC;
-            Q[k] = q;
+            const uint32_t k = uint32_t(i) * uint32_t(P) + uint32_t(j);
+            st32(pnew, k, pC);
+            st32(Q, k, q);
             v += double(pC) * double(q);
         }
     }
@@ -751,21 +767,21 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
         T xs[R], ps[R], rs[R], qs[R], as[R];
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
-            const size_t k = clampIdx(tl.i + rr, j, M, P);
-            xs[rr] = x[k];
-            ps[rr] = pk[k];
-            rs[rr] = r[k];
-            qs[rr] = Q[k];
-            as[rr] = aP[k];
+            const uint32_t k = clampIdx32(tl.i + rr, j, M, P);
+            xs[rr] = ld32<T>(x, k);
+            ps[rr] = ld32(pk, k);
+            rs[rr] = ld32<T>(r, k);
+            qs[rr] = ld32(Q, k);
+            as[rr] = ld32(aP, k);
         }
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
             const int i = tl.i + rr;
             if (!isPDof(i, j, M, N)) continue;
-            const size_t k = size_t(i) * P + j;
-            x[k] = xs[rr] + alpha * ps[rr];
+            const uint32_t k = uint32_t(i) * uint32_t(P) + uint32_t(j);
+            st32(x, k, xs[rr] + alpha * ps[rr]);
             const T rn = rs[rr] - alpha * qs[rr];
-            r[k] = rn;
+            st32(r, k, rn);
             v[0] += double(rn) * double(rn);
             v[1] += double(rn) * double(rn / as[rr]);
         }

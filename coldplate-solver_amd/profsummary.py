@@ -84,10 +84,12 @@ def counter_per_dispatch(d: str, counter: str) -> dict[str, list[float]]:
 
 
 def pick(d: dict[str, list[float]], sub: str) -> tuple[str, list[float]]:
-    for k, v in d.items():
-        if sub in k:
-            return k, v
-    raise SystemExit(f"no kernel matching {sub}")
+    """The kernel matching `sub` with the most dispatches (the tiling the
+    run used; an autotuning pass also launches the other instantiations)."""
+    hits = [(k, v) for k, v in d.items() if sub in k]
+    if not hits:
+        raise SystemExit(f"no kernel matching {sub}")
+    return max(hits, key=lambda kv: len(kv[1]))
 
 
 def bench_line(log: str) -> dict | None:
@@ -131,10 +133,14 @@ def main() -> int:
 
     b = bench_line(a.log)
     bu = bench_line(a.bench)
-    krow = next(r for r in stats if a.kernel in r["Name"])
+    krow = max((r for r in stats if a.kernel in r["Name"]), key=lambda r: int(r["Calls"]))
     rocprof_avg_us = float(krow["AverageNs"]) / 1e3
     live_us = b["roofline"]["avg_launch_us"] if b else None
     alg = b["roofline"]["bytes_per_launch"] if b else None
+    # the counter runs keep every slot busy: compare their traffic with the
+    # algorithmic bytes of a launch over the whole batch
+    ref = bu or b
+    alg_full = ref["roofline"].get("bytes_per_launch_full_batch") if ref else None
 
     lines = [f"# {a.kernel} — {a.workload}", ""]
     if a.commands and os.path.exists(a.commands):
@@ -149,8 +155,9 @@ def main() -> int:
     lines += ["", "## Live timing vs rocprof", ""]
     lines.append(f"* rocprof average of `{krow['Name']}`: {rocprof_avg_us:.2f} µs over {krow['Calls']} launches")
     if b:
-        lines.append(f"* bench.py HIP-event average of the same kernel in the same run: {live_us:.2f} µs "
-                     f"over {b['roofline']['launches_timed']} sampled launches")
+        lines.append(f"* bench.py HIP-event average of the same kernel in the traced run: {live_us:.2f} µs "
+                     f"over {b['roofline']['launches_timed']} sampled launches (back-to-back bursts, so it "
+                     f"also holds the tracer's gaps between dispatches)")
         lines.append(f"* algorithmic bytes per launch: {alg} (28 B per p' unknown)")
         lines.append(f"* achieved (bench, events): {b['roofline']['achieved']:.1f} GB/s = "
                      f"{100 * b['roofline']['frac']:.1f} % of 8000 GB/s; "
@@ -166,7 +173,8 @@ def main() -> int:
     lines.append(f"* `{kname}`: FETCH_SIZE {f_avg:.0f} KiB/dispatch ({len(kf)} dispatches), "
                  f"WRITE_SIZE {w_avg:.0f} KiB/dispatch ({len(kw)} dispatches)")
     lines.append(f"* calibrated traffic per launch: {traffic / 1e6:.1f} MB"
-                 + (f" = {traffic / alg:.2f} x the algorithmic bytes" if alg else ""))
+                 + (f" = {traffic / alg_full:.2f} x the algorithmic bytes of a launch over the full batch "
+                    f"({alg_full} B)" if alg_full else ""))
     open(a.out + ".md", "w").write("\n".join(lines) + "\n")
     json.dump({"kernel": kname, "workload": a.workload, "fetch_kib": f_avg, "write_kib": w_avg,
                "fetch_factor": f_factor, "write_factor": w_factor, "traffic_bytes": traffic,
