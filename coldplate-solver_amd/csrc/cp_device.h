@@ -25,6 +25,8 @@ constexpr int kBlock = kTileW * kTileH;
 // level plus a two-column halo on each side (the two smoothing sweeps and the
 // residual reach two nodes out).
 constexpr int kMgTileW = kTileW - 4;
+// p' unknowns' tiles: each lane holds two adjacent columns (paired loads).
+constexpr int kPTileW = 2 * kTileW;
 constexpr int kNumPartials = 4;  // per-tile reduction partials (doubles)
 constexpr int kNumXCD = 8;       // accelerator complex dies of an MI355X
 constexpr int kMaxLevels = 12;   // aggregation-multigrid levels (incl. the finest)
@@ -68,7 +70,8 @@ struct alignas(64) SlotDesc {
     int M, N, pitch;
     int tile0, ntiles, tilesX;
     // the p' unknowns' own tile list (kernels that touch only p' unknowns):
-    // 64 x 4R tiles from node (1, 0), covering rows 1 .. M-1, columns 0 .. N-2
+    // 128 x 4Rp tiles from node (1, 0), covering rows 1 .. M-1, columns
+    // 0 .. N-2, with Rp = pRows(mixed, tileRows)
     int ptile0, pntiles, ptilesX;
     int tileRows;                // rows per wave R: a tile is 64 x 4R nodes
     int nU, nV;                  // momentum DOF counts (Jacobi stop test)
@@ -179,6 +182,10 @@ hipError_t launchWorkOnly(Kern k, bool f32, int rows, int grid, hipStream_t s, c
 // height (coarser levels use 2). Even, so that a wave's rows hold whole 2 x 2
 // aggregates; 4 only in single precision (register footprint).
 inline int mgRows(bool f32, int rows) { return (f32 && rows >= 4) ? 4 : 2; }
+// Rows per wave of the p' list's tiles for a batch tile height: half (two
+// columns per lane keep the nodes per lane), one in double precision
+// (register footprint of the paired loads).
+inline int pRows(bool f32, int rows) { return f32 ? (rows >= 2 ? rows / 2 : 1) : 1; }
 hipError_t launchPlane(int nreq, hipStream_t s, const SlotDesc* desc, const PlaneReq* req, PlaneOut* out);
 
 }  // namespace cpdev

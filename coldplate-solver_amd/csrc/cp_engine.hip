@@ -279,8 +279,9 @@ KArgs kargsFor(cp_batch* b, Kern k) {
 }
 
 int launch(cp_batch* b, Kern k, int grid) {
-    const hipError_t e =
-        launchKernel(k, b->precision == CP_PREC_MIXED, b->tileRows, grid, b->maxSlots, b->stream, kargsFor(b, k));
+    const bool f32 = b->precision == CP_PREC_MIXED;
+    const int rows = pOnly(k) ? pRows(f32, b->tileRows) : b->tileRows;
+    const hipError_t e = launchKernel(k, f32, rows, grid, b->maxSlots, b->stream, kargsFor(b, k));
     if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
     return CP_OK;
 }
@@ -311,9 +312,12 @@ int launchMg(cp_batch* b, Kern k, int level) {
 // Coarse operators of every level and the coarsest inverse (after assembly).
 int mgSetup(cp_batch* b) {
     int rc;
-    for (int l = 1; l < b->mgLevels; ++l)
+    // grid-wide launches down to the batch's first tail level; the levels
+    // below it are coarsened in LDS by the per-slot factor launch
+    const int last = std::min(b->mgTail, b->mgLevels - 1);
+    for (int l = 1; l <= last; ++l)
         if ((rc = launchMg(b, K_MG_COARSEN, l))) return rc;
-    return b->mgLevels > 0 ? launchMg(b, K_MG_FACTOR, 0) : CP_OK;
+    return b->mgLevels > 0 ? launchMg(b, K_MG_FACTOR, last + 1) : CP_OK;
 }
 
 // z = V-cycle(r) for every multigrid slot whose CG is running, then r.z:
@@ -488,8 +492,9 @@ void setTiling(SlotDesc& d, int tileRows, int mgRows0) {
     const int tilesY = (d.M + 1 + rowsPerTile - 1) / rowsPerTile;
     d.ntiles = d.tilesX * tilesY;
     // p' unknowns: rows 1 .. M-1 from row 1, columns 1 .. N-2 from column 0
-    d.ptilesX = (d.N - 1 + kTileW - 1) / kTileW;
-    d.pntiles = d.ptilesX * ((d.M - 1 + rowsPerTile - 1) / rowsPerTile);
+    const int pRowsPerTile = kTileH * pRows(d.mixed != 0, d.tileRows);
+    d.ptilesX = (d.N - 1 + kPTileW - 1) / kPTileW;
+    d.pntiles = d.ptilesX * ((d.M - 1 + pRowsPerTile - 1) / pRowsPerTile);
     for (int l = 0; l < d.nLevels; ++l) {
         MgLevel& L = d.lv[l];
         const int rows = kTileH * (l == 0 ? mgRows0 : 2);
@@ -1385,7 +1390,7 @@ int burst(cp_batch* b, int launches, double* ms_per_launch) {
     const bool f32 = b->precision == CP_PREC_MIXED;
     HIP_TRY(hipEventRecord(b->burstEv[0], b->stream));
     for (int k = 0; k < launches; ++k) {
-        const hipError_t e = launchWorkOnly(K_PCG_Q, f32, b->tileRows, grid, b->stream, a);
+        const hipError_t e = launchWorkOnly(K_PCG_Q, f32, pRows(f32, b->tileRows), grid, b->stream, a);
         if (e != hipSuccess) return setError(&err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
     }
     HIP_TRY(hipEventRecord(b->burstEv[1], b->stream));

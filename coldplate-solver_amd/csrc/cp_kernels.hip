@@ -45,6 +45,27 @@ template <typename T>
 __device__ __forceinline__ void st32(gp<T> base, uint32_t k, T v) {
     *reinterpret_cast<gp<T>>(reinterpret_cast<gp<char>>(base) + k * uint32_t(sizeof(T))) = v;
 }
+// Two adjacent elements k, k+1 (k even: the pair is aligned) in one access.
+template <typename T>
+using Pair = T __attribute__((ext_vector_type(2)));
+template <typename T>
+__device__ __forceinline__ Pair<T> ld2(gp<const T> base, uint32_t k) {
+    return *reinterpret_cast<gp<const Pair<T>>>(reinterpret_cast<gp<const char>>(base) + k * uint32_t(sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void st2(gp<T> base, uint32_t k, Pair<T> v) {
+    *reinterpret_cast<gp<Pair<T>>>(reinterpret_cast<gp<char>>(base) + k * uint32_t(sizeof(T))) = v;
+}
+// Stores the pair's members whose mask is set (both at once when both are).
+template <typename T>
+__device__ __forceinline__ void stMasked(gp<T> base, uint32_t k, Pair<T> v, bool m0, bool m1) {
+    if (m0 && m1) {
+        st2(base, k, v);
+    } else {
+        if (m0) st32(base, k, v.x);
+        if (m1) st32(base, k + 1, v.y);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // reductions
@@ -141,7 +162,8 @@ struct Tile {
 // parameter shared by every slot of a batch): wave w of the workgroup owns
 // rows [i0 + w R, i0 + (w+1) R).
 // Tile t of the p' list (a slot's ptile0 .. ptile0 + pntiles - 1): rows from
-// 1, so the tiles hold no row outside the unknowns' range 1 .. M-1.
+// 1, so the tiles hold no row outside the unknowns' range 1 .. M-1; 128
+// columns, lane l holding columns j0 + 2l and j0 + 2l + 1 (tl.j is the first).
 template <int R>
 __device__ __forceinline__ Tile decodePTile(const KArgs& a, int t) {
     Tile tl;
@@ -151,9 +173,9 @@ __device__ __forceinline__ Tile decodePTile(const KArgs& a, int t) {
     const int ty = lt / d.ptilesX;
     const int tx = lt - ty * d.ptilesX;
     tl.i0 = 1 + ty * (kTileH * R);
-    tl.j0 = tx * kTileW;
+    tl.j0 = tx * kPTileW;
     tl.i = tl.i0 + int(threadIdx.x >> 6) * R;
-    tl.j = tl.j0 + int(threadIdx.x & 63);
+    tl.j = tl.j0 + 2 * int(threadIdx.x & 63);
     return tl;
 }
 
@@ -188,6 +210,21 @@ __device__ __forceinline__ Tile decodeTile(const KArgs& a, int t) {
 
 __device__ __forceinline__ double clampv(double x, double m) { return fmax(-m, fmin(m, x)); }
 
+// Copies a slot's descriptor into LDS, every thread of the block taking a
+// share of its 8-byte words, and synchronises the block. Per-slot kernels
+// read the descriptor from there: read through the global pointer, each
+// field would be reloaded (one dependent round trip) after every store the
+// compiler cannot separate from it.
+__device__ __forceinline__ const SlotDesc& descToLds(SlotDesc* dst, const SlotDesc* src) {
+    constexpr int kWords = int(sizeof(SlotDesc) / sizeof(uint64_t));
+    static_assert(sizeof(SlotDesc) % sizeof(uint64_t) == 0, "descriptor size");
+    const auto gs = G(reinterpret_cast<const uint64_t*>(src));
+    uint64_t* ld = reinterpret_cast<uint64_t*>(dst);
+    for (int w = int(threadIdx.x); w < kWords; w += int(blockDim.x)) ld[w] = gs[w];
+    __syncthreads();
+    return *dst;
+}
+
 // Row-major index of (i, j) clamped into the (M+1) x P array: lanes outside
 // the unknowns load from a valid address and their values are discarded, so
 // every load of a tile is issued unconditionally, ahead of any use.
@@ -196,6 +233,10 @@ __device__ __forceinline__ size_t clampIdx(int i, int j, int M, int P) {
 }
 __device__ __forceinline__ uint32_t clampIdx32(int i, int j, int M, int P) {
     return uint32_t(min(max(i, 0), M)) * uint32_t(P) + uint32_t(min(max(j, 0), P - 1));
+}
+// the same for an even column j of a pair (P is even: the pair stays aligned)
+__device__ __forceinline__ uint32_t clampPair32(int i, int j, int M, int P) {
+    return uint32_t(min(max(i, 0), M)) * uint32_t(P) + uint32_t(min(max(j, 0), P - 2));
 }
 
 // lane l receives lane l-1's value (west) / lane l+1's value (east); lanes 0
@@ -662,10 +703,11 @@ __device__ __forceinline__ T pLoad(gp<const T> r, gp<const T> aP, gp<const T> Z,
     return First ? z : z + beta * ld32(pold, k);
 }
 
-// q = A p_k for one wave's R rows x 64 columns. The wave loads p_k for its
-// rows and for the rows above and below them (lanes 0-31 also at column
-// j0-1, lanes 32-63 at column j0+64), then marches down the rows taking the
-// west and east neighbours from the adjacent lanes.
+// q = A p_k for one wave's R rows x 128 columns, two adjacent columns per
+// lane. The wave loads p_k for its rows and for the rows above and below
+// them (lanes 0-31 also at column j0-1, lanes 32-63 at column j0+128), then
+// marches down the rows taking the neighbours across lanes from the
+// adjacent lanes.
 template <typename T, int R, bool HasZ, bool First>
 __device__ __forceinline__ void pcgQWave(const SlotDesc& d, const Tile& tl, int kk, T beta, double& v) {
     const int M = d.M, N = d.N, P = d.pitch;
@@ -679,41 +721,58 @@ __device__ __forceinline__ void pcgQWave(const SlotDesc& d, const Tile& tl, int 
     const auto Q = G(static_cast<T*>(d.pr[P_Q]));
     const int lane = int(threadIdx.x & 63);
     const int j = tl.j, ib = tl.i;
-    const int je = lane < 32 ? tl.j0 - 1 : tl.j0 + kTileW;
-    T p[R + 2], pe[R + 2], ap[R], ce[R], cee[R], cn[R + 1];
+    const int je = lane < 32 ? tl.j0 - 1 : tl.j0 + kPTileW;
+    Pair<T> p[R + 2], ap[R], ce[R], cn[R + 1];
+    T pe[R + 2], cee[R];
 #pragma unroll
     for (int h = 0; h < R + 2; ++h) {
         const int i = ib - 1 + h;
-        const T pc = pLoad<T, HasZ, First>(r, aP, Z, pold, clampIdx32(i, j, M, P), beta);
+        const uint32_t k2 = clampPair32(i, j, M, P);
+        Pair<T> z;
+        if (HasZ) {
+            z = ld2(Z, k2);
+        } else {
+            const Pair<T> rv = ld2(r, k2), av = ld2(aP, k2);
+            z.x = rv.x / av.x;
+            z.y = rv.y / av.y;
+        }
+        if (!First) {
+            const Pair<T> po = ld2(pold, k2);
+            z.x = z.x + beta * po.x;
+            z.y = z.y + beta * po.y;
+        }
         const T px = pLoad<T, HasZ, First>(r, aP, Z, pold, clampIdx32(i, je, M, P), beta);
-        p[h] = isPDof(i, j, M, N) ? pc : T(0);
+        p[h].x = isPDof(i, j, M, N) ? z.x : T(0);
+        p[h].y = isPDof(i, j + 1, M, N) ? z.y : T(0);
         pe[h] = isPDof(i, je, M, N) ? px : T(0);
     }
 #pragma unroll
     for (int rr = 0; rr < R; ++rr) {
-        const uint32_t k = clampIdx32(ib + rr, j, M, P);
-        ap[rr] = ld32(aP, k);
-        ce[rr] = ld32(cE, k);
+        const uint32_t k2 = clampPair32(ib + rr, j, M, P);
+        ap[rr] = ld2(aP, k2);
+        ce[rr] = ld2(cE, k2);
         cee[rr] = ld32(cE, clampIdx32(ib + rr, je, M, P));
     }
 #pragma unroll
-    for (int h = 0; h < R + 1; ++h) cn[h] = ld32(cN, clampIdx32(ib - 1 + h, j, M, P));
+    for (int h = 0; h < R + 1; ++h) cn[h] = ld2(cN, clampPair32(ib - 1 + h, j, M, P));
 #pragma unroll
     for (int rr = 0; rr < R; ++rr) {
         const int i = ib + rr;
-        const T pC = p[rr + 1];
-        T pW = __shfl_up(pC, 1, 64), pE = __shfl_down(pC, 1, 64), cEw = __shfl_up(ce[rr], 1, 64);
+        const Pair<T> pC = p[rr + 1];
+        T pW = fromWest(pC.y), pE = fromEast(pC.x), cEw = fromWest(ce[rr].y);
         pW = lane == 0 ? pe[rr + 1] : pW;
         cEw = lane == 0 ? cee[rr] : cEw;
         pE = lane == 63 ? pe[rr + 1] : pE;
         // CSR column order S, W, P, E, N (pressure_solver.cpp:108-118)
-        const T q = -cn[rr] * p[rr] - cEw * pW + ap[rr] * pC - ce[rr] * pE - cn[rr + 1] * p[rr + 2];
-        if (isPDof(i, j, M, N)) {
-            const uint32_t k = uint32_t(i) * uint32_t(P) + uint32_t(j);
-            st32(pnew, k, pC);
-            st32(Q, k, q);
-            v += double(pC) * double(q);
-        }
+        Pair<T> q;
+        q.x = -cn[rr].x * p[rr].x - cEw * pW + ap[rr].x * pC.x - ce[rr].x * pC.y - cn[rr + 1].x * p[rr + 2].x;
+        q.y = -cn[rr].y * p[rr].y - ce[rr].x * pC.x + ap[rr].y * pC.y - ce[rr].y * pE - cn[rr + 1].y * p[rr + 2].y;
+        const bool d0 = isPDof(i, j, M, N), d1 = isPDof(i, j + 1, M, N);
+        const uint32_t k = uint32_t(max(i, 0)) * uint32_t(P) + uint32_t(j);
+        stMasked(pnew, k, pC, d0, d1);
+        stMasked(Q, k, q, d0, d1);
+        if (d0) v += double(pC.x) * double(q.x);
+        if (d1) v += double(pC.y) * double(q.y);
     }
 }
 
@@ -763,27 +822,38 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
         const auto Q = G(static_cast<const T*>(d.pr[P_Q]));
         const auto aP = G(static_cast<const T*>(d.pr[P_AP]));
         double v[2] = {0.0, 0.0};
-        // all loads first, from clamped addresses; stores only on unknowns
-        T xs[R], ps[R], rs[R], qs[R], as[R];
+        // all loads first (column pairs, clamped addresses); stores only on
+        // unknowns
+        Pair<T> xs[R], ps[R], rs[R], qs[R], as[R];
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
-            const uint32_t k = clampIdx32(tl.i + rr, j, M, P);
-            xs[rr] = ld32<T>(x, k);
-            ps[rr] = ld32(pk, k);
-            rs[rr] = ld32<T>(r, k);
-            qs[rr] = ld32(Q, k);
-            as[rr] = ld32(aP, k);
+            const uint32_t k = clampPair32(tl.i + rr, j, M, P);
+            xs[rr] = ld2<T>(x, k);
+            ps[rr] = ld2(pk, k);
+            rs[rr] = ld2<T>(r, k);
+            qs[rr] = ld2(Q, k);
+            as[rr] = ld2(aP, k);
         }
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
             const int i = tl.i + rr;
-            if (!isPDof(i, j, M, N)) continue;
-            const uint32_t k = uint32_t(i) * uint32_t(P) + uint32_t(j);
-            st32(x, k, xs[rr] + alpha * ps[rr]);
-            const T rn = rs[rr] - alpha * qs[rr];
-            st32(r, k, rn);
-            v[0] += double(rn) * double(rn);
-            v[1] += double(rn) * double(rn / as[rr]);
+            const bool d0 = isPDof(i, j, M, N), d1 = isPDof(i, j + 1, M, N);
+            const uint32_t k = uint32_t(max(i, 0)) * uint32_t(P) + uint32_t(j);
+            Pair<T> xn, rn;
+            xn.x = xs[rr].x + alpha * ps[rr].x;
+            xn.y = xs[rr].y + alpha * ps[rr].y;
+            rn.x = rs[rr].x - alpha * qs[rr].x;
+            rn.y = rs[rr].y - alpha * qs[rr].y;
+            stMasked(x, k, xn, d0, d1);
+            stMasked(r, k, rn, d0, d1);
+            if (d0) {
+                v[0] += double(rn.x) * double(rn.x);
+                v[1] += double(rn.x) * double(rn.x / as[rr].x);
+            }
+            if (d1) {
+                v[0] += double(rn.y) * double(rn.y);
+                v[1] += double(rn.y) * double(rn.y / as[rr].y);
+            }
         }
         blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials);
     }
@@ -803,12 +873,23 @@ __global__ __launch_bounds__(kBlock) void k_refine_accum(KArgs a) {
         if (st.status != ST_RUNNING || st.stepPhase) continue;
         const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
         const auto x = G(static_cast<const T*>(d.pr[P_X]));
+        const auto pc = G(d.pc);
+        // all loads first (column pairs, clamped addresses)
+        Pair<T> xs[R];
+        Pair<double> ps[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t k = clampPair32(tl.i + r, j, M, P);
+            xs[r] = ld2(x, k);
+            ps[r] = ld2<double>(pc, k);
+        }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int i = tl.i + r;
-            if (!isPDof(i, j, M, N)) continue;
-            const size_t k = size_t(i) * P + j;
-            G(d.pc)[k] += double(x[k]);
+            Pair<double> pn;
+            pn.x = ps[r].x + double(xs[r].x);
+            pn.y = ps[r].y + double(xs[r].y);
+            stMasked(pc, uint32_t(i) * uint32_t(P) + uint32_t(j), pn, isPDof(i, j, M, N), isPDof(i, j + 1, M, N));
         }
     }
 }
@@ -821,15 +902,15 @@ __global__ __launch_bounds__(kBlock) void k_refine_resid(KArgs a) {
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
         if (st.status != ST_RUNNING || st.stepPhase) continue;
-        const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
+        const int M = d.M, N = d.N, P = d.pitch;
         const auto aP = G(static_cast<const T*>(d.pr[P_AP]));
         const auto cE = G(static_cast<const T*>(d.pr[P_CE]));
         const auto cN = G(static_cast<const T*>(d.pr[P_CN]));
         const auto pc = G(d.pc);
         double v[2] = {0.0, 0.0};
 #pragma unroll
-        for (int rr = 0; rr < R; ++rr) {
-            const int i = tl.i + rr;
+        for (int n = 0; n < 2 * R; ++n) {
+            const int i = tl.i + (n >> 1), j = tl.j + (n & 1);
             if (!isPDof(i, j, M, N)) continue;
             const size_t k = size_t(i) * P + j;
             auto pAt64 = [&](int ii, int jj, size_t kn) -> double { return isPDof(ii, jj, M, N) ? pc[kn] : 0.0; };
@@ -990,55 +1071,158 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarsen(KArgs a) {
     }
 }
 
-// Dense inverse of the coarsest operator (n = H W <= kCoarseMax unknowns) by
-// Gauss-Jordan elimination in double, one workgroup per slot.
-__global__ __launch_bounds__(kBlock) void k_mg_factor(KArgs a) {
-    const int s = blockIdx.x;
-    const SlotDesc& d = a.desc[s];
-    const SlotState& st = a.state[s];
-    if (d.ntiles == 0 || !mgActive(d, st)) return;
-    __shared__ double A[kCoarseMax][kCoarseMax + 1];
-    __shared__ double f[kCoarseMax];
-    __shared__ double ipS;
-    const MgLevel& L = d.lv[d.nLevels - 1];
-    const int W = L.W, n = L.H * L.W;
-    const bool f32 = d.mixed != 0;
-    auto ld = [&](void* p, int r) {
-        const size_t k = mgAt(L, r / W, r % W);
-        return f32 ? double(G(static_cast<const float*>(p))[k]) : G(static_cast<const double*>(p))[k];
-    };
-    for (int idx = int(threadIdx.x); idx < n * n; idx += kBlock) {
-        const int r = idx / n, c = idx - r * n;
-        double v = 0.0;
-        if (c == r) v = ld(L.aP, r);
-        else if (c == r + 1 && (r % W) + 1 < W) v = -ld(L.cE, r);
-        else if (c + 1 == r && (c % W) + 1 < W) v = -ld(L.cE, c);
-        else if (c == r + W) v = -ld(L.cN, r);
-        else if (c + W == r) v = -ld(L.cN, c);
-        A[r][c] = v;
+constexpr int kGjCols = kCoarseMax / (kBlock / kCoarseMax);  // columns per thread (16)
+
+// In-LDS coarsening of the levels a.level .. nLevels-1 and the dense
+// inverse of the coarsest level for one slot (see k_mg_factor).
+template <typename T>
+__device__ void mgFactorSlot(const KArgs& a, const SlotDesc& d) {
+    constexpr int kCap = sizeof(T) == 4 ? kTailCapF32 : kTailCapF64;
+    __shared__ T sAP[kCap], sCE[kCap], sCN[kCap];
+        const int tid = int(threadIdx.x);
+    const int L = d.nLevels;
+    const int src = min(max(a.level, 1) - 1, L - 1);  // first level read from global memory
+    __shared__ int off[kMaxLevels + 1];  // compact offsets of levels src .. L-1 (read after the first barrier)
+    if (tid == 0) {
+        off[src] = 0;
+        for (int l = src; l < L; ++l) off[l + 1] = off[l] + d.lv[l].H * d.lv[l].W;
+    }
+    {
+        const MgView<T> V(d.lv[src]);
+        const int n = V.H * V.W;
+        constexpr int kIt = (kCap + kBlock - 1) / kBlock;
+        T la[kIt], le[kIt], ln[kIt];
+#pragma unroll
+        for (int it = 0; it < kIt; ++it) {  // loads first, then the LDS stores
+            const int idx = min(tid + it * kBlock, n - 1);
+            const int I = idx / V.W, J = idx - I * V.W;
+            const size_t k = V.at(I, J);
+            la[it] = V.aP[k];
+            le[it] = V.cE[k];
+            ln[it] = V.cN[k];
+        }
+#pragma unroll
+        for (int it = 0; it < kIt; ++it) {
+            const int idx = tid + it * kBlock;
+            if (idx < n) {
+                sAP[idx] = la[it];
+                sCE[idx] = le[it];
+                sCN[idx] = ln[it];
+            }
+        }
     }
     __syncthreads();
-    for (int k = 0; k < n; ++k) {
-        if (threadIdx.x == 0) {
-            double piv = A[k][k];
-            if (!(piv > 0.0)) piv = 1.0;
-            ipS = 1.0 / piv;
-        }
-        __syncthreads();
-        const double ip = ipS;
-        const int t = int(threadIdx.x);
-        if (t < n) {
-            f[t] = (t == k) ? 0.0 : A[t][k];
-            A[k][t] = ((t == k) ? 1.0 : A[k][t]) * ip;
-        }
-        __syncthreads();
-        for (int idx = t; idx < n * n; idx += kBlock) {
-            const int i = idx / n, j = idx - i * n;
-            if (i != k) A[i][j] = ((j == k) ? 0.0 : A[i][j]) - f[i] * A[k][j];
+    for (int l = src + 1; l < L; ++l) {
+        const int Hf = d.lv[l - 1].H, Wf = d.lv[l - 1].W, of = off[l - 1];
+        const MgView<T> C(d.lv[l]);
+        const auto caP = G(static_cast<T*>(d.lv[l].aP));
+        const auto ccE = G(static_cast<T*>(d.lv[l].cE));
+        const auto ccN = G(static_cast<T*>(d.lv[l].cN));
+        for (int idx = tid; idx < C.H * C.W; idx += kBlock) {
+            const int I = idx / C.W, J = idx - I * C.W;
+            const int i0 = 2 * I, j0 = 2 * J;
+            // q = 0: (i0, j0), 1: (i0, j0+1), 2: (i0+1, j0), 3: (i0+1, j0+1)
+            auto in = [&](int q) { return i0 + (q >> 1) < Hf && j0 + (q & 1) < Wf; };
+            auto at = [&](int q) { return of + (i0 + (q >> 1)) * Wf + j0 + (q & 1); };
+            auto g = [&](int q) { return in(q) ? double(sAP[at(q)]) : 0.0; };
+            auto eE = [&](int q) { return (in(q) && j0 + (q & 1) + 1 < Wf) ? double(sCE[at(q)]) : 0.0; };
+            auto eN = [&](int q) { return (in(q) && i0 + (q >> 1) + 1 < Hf) ? double(sCN[at(q)]) : 0.0; };
+            double acc = 0.0;
+            acc += g(0);
+            acc += g(1);
+            acc += g(2);
+            acc += g(3);
+            double internal = 0.0;
+            internal += eE(0);
+            internal += eE(2);
+            internal += eN(0);
+            internal += eN(1);
+            const T vP = T(acc - 2.0 * internal), vE = T(eE(1) + eE(3)), vN = T(eN(2) + eN(3));
+            sAP[off[l] + idx] = vP;
+            sCE[off[l] + idx] = vE;
+            sCN[off[l] + idx] = vN;
+            const size_t k = C.at(I, J);
+            caP[k] = vP;
+            ccE[k] = vE;
+            ccN[k] = vN;
         }
         __syncthreads();
     }
-    for (int idx = int(threadIdx.x); idx < n * n; idx += kBlock) G(d.coarseInv)[idx] = A[idx / n][idx % n];
+    // Gauss-Jordan: thread (row i = tid % 64, column group cg = tid / 64,
+    // uniform over a wave) keeps A[i][cg + 4m], m = 0 .. 15, in registers.
+    // Per pivot k the owners publish column k and row k through
+    // double-buffered LDS (one barrier per pivot); every thread then updates
+    // its entries. Columns past n are skipped per wave.
+    const int W = d.lv[L - 1].W, n = d.lv[L - 1].H * W, oc = off[L - 1];
+    const int i = tid & (kCoarseMax - 1), cg = tid / kCoarseMax;
+    constexpr int kGroups = kBlock / kCoarseMax;  // 4
+    __shared__ double fbuf[2][kCoarseMax], rbuf[2][kCoarseMax];
+    double av[kGjCols];
+#pragma unroll
+    for (int m = 0; m < kGjCols; ++m) {
+        const int r = i, col = cg + kGroups * m;
+        double v = 0.0;
+        if (r < n && col < n) {
+            if (col == r) v = double(sAP[oc + r]);
+            else if (col == r + 1 && (r % W) + 1 < W) v = -double(sCE[oc + r]);
+            else if (col + 1 == r && (col % W) + 1 < W) v = -double(sCE[oc + col]);
+            else if (col == r + W) v = -double(sCN[oc + r]);
+            else if (col + W == r) v = -double(sCN[oc + col]);
+        }
+        av[m] = v;
+    }
+    const int mEnd = (n - cg + kGroups - 1) / kGroups;  // columns cg + 4m < n
+    for (int k = 0; k < n; ++k) {
+        const int b = k & 1;
+        if (cg == (k & (kGroups - 1))) {  // this wave holds column k
+            double ck = 0.0;
+#pragma unroll
+            for (int m = 0; m < kGjCols; ++m) ck = (cg + kGroups * m == k) ? av[m] : ck;
+            fbuf[b][i] = ck;
+        }
+        if (i == k) {
+#pragma unroll
+            for (int m = 0; m < kGjCols; ++m) rbuf[b][cg + kGroups * m] = av[m];
+        }
+        __syncthreads();
+        double piv = fbuf[b][k];
+        if (!(piv > 0.0)) piv = 1.0;
+        const double ip = 1.0 / piv;
+        const double fi = (i == k) ? 0.0 : fbuf[b][i];
+#pragma unroll
+        for (int m = 0; m < kGjCols; ++m) {
+            if (m < mEnd) {  // uniform over the wave
+                const int j = cg + kGroups * m;
+                const double rk = ((j == k) ? 1.0 : rbuf[b][j]) * ip;
+                av[m] = (i == k) ? rk : ((j == k) ? 0.0 : av[m]) - fi * rk;
+            }
+        }
+    }
+    if (i < n) {
+#pragma unroll
+        for (int m = 0; m < kGjCols; ++m) {
+            const int j = cg + kGroups * m;
+            if (j < n) G(d.coarseInv)[size_t(i) * n + j] = av[m];
+        }
+    }
+}
+
+
+// The coarse end of the hierarchy's setup, one workgroup per slot: the
+// Galerkin operators of levels a.level .. nLevels-1 (coarse enough that the
+// levels from a.level - 1 down fit in LDS), then the dense inverse of the
+// coarsest operator (n = H W <= kCoarseMax unknowns) by Gauss-Jordan
+// elimination in double. The arithmetic is k_mg_coarsen's and the
+// elimination's, point for point. Levels above a.level are coarsened by
+// the grid-wide launches before this one.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_factor(KArgs a) {
+    const int s = blockIdx.x;
+    __shared__ SlotDesc sd;
+    const SlotDesc& d = descToLds(&sd, &a.desc[s]);
+    const SlotState& st = a.state[s];
+    if (d.ntiles == 0 || !mgActive(d, st)) return;
+    mgFactorSlot<T>(a, d);
 }
 
 // Down-sweep at level a.level (above the slot's tail): x1 = w b/aP,
@@ -1141,7 +1325,8 @@ __global__ __launch_bounds__(kTailBlock) void k_mg_tail(KArgs a) {
     __shared__ T sAP[kCap], sCE[kCap], sCN[kCap], sB[kCap], sX[kCap], sZ[kCap], sT[kCap];
     __shared__ double sc64[kCoarseMax];
     const int s = blockIdx.x;
-    const SlotDesc& d = a.desc[s];
+    __shared__ SlotDesc sd;
+    const SlotDesc& d = descToLds(&sd, &a.desc[s]);
     const SlotState& st = a.state[s];
     if (d.ntiles == 0 || !mgActive(d, st)) return;
     const int L0 = d.mgTail, L = d.nLevels;
@@ -1153,16 +1338,37 @@ __global__ __launch_bounds__(kTailBlock) void k_mg_tail(KArgs a) {
         for (int l = L0; l < L; ++l) off[l + 1] = off[l] + d.lv[l].H * d.lv[l].W;
     }
     __syncthreads();
-    for (int l = L0; l < L; ++l) {
-        const MgView<T> V(d.lv[l]);
-        const int n = V.H * V.W, o = off[l];
-        for (int idx = tid; idx < n; idx += kTailBlock) {
-            const int I = idx / V.W, J = idx - I * V.W;
-            const size_t k = V.at(I, J);
-            sAP[o + idx] = V.aP[k];
-            sCE[o + idx] = V.cE[k];
-            sCN[o + idx] = V.cN[k];
-            if (l == L0) sB[o + idx] = V.b[k];
+    {
+        // every tail level's operator (and the first level's right-hand
+        // side) with all loads issued before the LDS stores: element g of
+        // the concatenated levels belongs to the level whose range holds it
+        constexpr int kIt = (kCap + kTailBlock - 1) / kTailBlock;
+        const int total = off[L];
+        T la[kIt], le[kIt], ln[kIt], lb[kIt];
+        int lvOf[kIt];
+#pragma unroll
+        for (int it = 0; it < kIt; ++it) {
+            const int g = min(tid + it * kTailBlock, total - 1);
+            int l = L0;
+            while (l + 1 < L && off[l + 1] <= g) ++l;
+            const MgLevel& V = d.lv[l];
+            const int idx = g - off[l], I = idx / V.W, J = idx - I * V.W;
+            const size_t k = mgAt(V, I, J);
+            la[it] = G(static_cast<const T*>(V.aP))[k];
+            le[it] = G(static_cast<const T*>(V.cE))[k];
+            ln[it] = G(static_cast<const T*>(V.cN))[k];
+            lb[it] = G(static_cast<const T*>(V.b))[k];
+            lvOf[it] = l;
+        }
+#pragma unroll
+        for (int it = 0; it < kIt; ++it) {
+            const int g = tid + it * kTailBlock;
+            if (g < total) {
+                sAP[g] = la[it];
+                sCE[g] = le[it];
+                sCN[g] = ln[it];
+                if (lvOf[it] == L0) sB[g] = lb[it];
+            }
         }
     }
     __syncthreads();
@@ -1534,7 +1740,8 @@ __global__ __launch_bounds__(kBlock) void k_plane(const SlotDesc* desc, const Pl
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_dp_monitor(KArgs a) {
     const int s = blockIdx.x;
-    const SlotDesc& d = a.desc[s];
+    __shared__ SlotDesc sd;
+    const SlotDesc& d = descToLds(&sd, &a.desc[s]);
     SlotState& st = a.state[s];
     const bool gate = d.slopeGate && d.slopeWindow > 0;
     if (d.ntiles == 0 || st.status != ST_RUNNING || (!d.dpConv && !gate)) return;
@@ -1617,7 +1824,8 @@ __global__ __launch_bounds__(kBlock) void k_dp_monitor(KArgs a) {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int outer) {
     const int s = blockIdx.x;
-    const SlotDesc& d = a.desc[s];
+    __shared__ SlotDesc sd;
+    const SlotDesc& d = descToLds(&sd, &a.desc[s]);
     // The slot's state is read into LDS in one step (a word per thread),
     // updated there by thread 0 and written back whole, so its fields cost
     // no dependent global round trips. Nothing else touches the state while
@@ -1823,7 +2031,7 @@ static void launchTR(Kern k, int grid, hipStream_t s, const KArgs& a) {
         case K_MG_UP:
             if constexpr (mgRowsOk<T, R>()) hipLaunchKernelGGL((k_mg_up<T, R>), g, b, 0, s, a);
             break;
-        case K_MG_FACTOR: hipLaunchKernelGGL(k_mg_factor, g, b, 0, s, a); break;
+        case K_MG_FACTOR: hipLaunchKernelGGL((k_mg_factor<T>), g, b, 0, s, a); break;
         case K_MG_TAIL: hipLaunchKernelGGL((k_mg_tail<T>), g, dim3(kTailBlock), 0, s, a); break;
         case K_DP_MONITOR: hipLaunchKernelGGL(k_dp_monitor, g, b, 0, s, a); break;
     }
