@@ -70,6 +70,8 @@ class Params(C.Structure):
         ("usePressureDropSlopeGate", C.c_int32), ("dpSlopeWindowIters", C.c_int32),
         ("dpSlopeMaxDelta", C.c_double),
         ("xPlaneCoreInlet", C.c_double), ("xPlaneCoreOutlet", C.c_double),
+        ("pressureSolverType", C.c_int32), ("momentumSolverType", C.c_int32),
+        ("sorOmega", C.c_double),
     ]
 
     def copy(self) -> "Params":

@@ -83,6 +83,8 @@ struct alignas(64) SlotDesc {
     int dpConv, dpWindow;        // pressure-drop window exit and its length
     int slopeGate, slopeWindow;  // slope gate of the window exit and its length
     double dpPercent, slopeMaxDelta, xCoreIn, xCoreOut;
+    int presType, momType;       // 0: red-black SOR / explicit momentum; 1: CG / implicit + Jacobi
+    double sorOmega;             // SOR over-relaxation (resolved on the host)
     int precond, nLevels;        // CP_PRECOND_*; multigrid levels (>= 2 when precond == 1)
     int mgTail;                  // first level handled by the one-workgroup tail kernel
     double mgOmega, mgScale;
@@ -120,6 +122,7 @@ struct alignas(256) SlotState {
     double inletVelocity, pseudoCFL, serResidPrev, lsAlpha;
     int lsBacktrack;
     int dpIdx, dpFilled, dpConverged, slopeHead, slopeCount;
+    int sorDone;  // SOR slots: the pressure sweeps of this outer iteration have stopped
     double dpLast;  // core-plane total-pressure drop after the last iteration
 };
 
@@ -143,10 +146,15 @@ struct KArgs {
     double* partials;
     int force;  // k_pcg_q only: also run slots whose PCG has stopped (timing bursts)
     int level;  // multigrid kernels: hierarchy level of the tile list
+    int color;  // SOR sweep: 0 = cells with i + j even, 1 = odd
 };
 
 // Finalisation phases (k_finalize), one per reducing work kernel.
-enum Phase { PH_JAC = 0, PH_MOM_FIN, PH_PRES_ASM, PH_PCG_Q, PH_PCG_U, PH_REF_ACC, PH_REF_RES, PH_CORR, PH_MG_Z };
+enum Phase {
+    PH_JAC = 0, PH_MOM_FIN, PH_PRES_ASM, PH_PCG_Q, PH_PCG_U, PH_REF_ACC, PH_REF_RES, PH_CORR, PH_MG_Z,
+    PH_MOM_EXPL,  // explicit momentum: residuals
+    PH_SOR,       // SOR: stop test after a red + black sweep pair
+};
 
 // Kernels of one SIMPLE step, in launch order.
 enum Kern {
@@ -166,6 +174,9 @@ enum Kern {
     K_MG_TAIL,      // multigrid: levels mgTail.. down, coarsest solve, up (one block per slot)
     K_DP_MONITOR,   // pressure-drop window exit after the controller (one block per slot)
     K_MG_UP,        // multigrid: prolongation + post-smoothing at level a.level
+    K_MOM_EXPL,     // explicit momentum: u*, v*, dE, dN of the unknowns
+    K_MOM_EXPL_BC,  // explicit momentum: velocity BCs on (u*, v*)
+    K_SOR,          // one SOR colour sweep on p' (a.color)
 };
 
 // Host-side launchers (defined next to the kernels). launchKernel enqueues

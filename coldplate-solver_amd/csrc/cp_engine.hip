@@ -134,6 +134,16 @@ constexpr int kMgArrays = 6;      // aP cE cN b x z of each coarse multigrid lev
 // 0 is the (M-1) x (N-2) p' grid, each next level aggregates 2x2 blocks, and
 // coarsening stops once a level has at most kCoarseMax unknowns (at least one
 // coarse level). Returns the number of levels.
+// Over-relaxation of the SOR backend (pressure_solver.cpp:810-816): the
+// caller's in (0, 2), else 2 / (1 + sqrt(1 - cos^2(pi / max(M, N))))
+// clamped to [1, 1.95].
+double sorOmegaFor(double w, int M, int N) {
+    if (w > 0.0 && w < 2.0) return w;
+    const double rs = std::cos(M_PI / double(std::max(M, N)));
+    const double om = 2.0 / (1.0 + std::sqrt(1.0 - rs * rs));
+    return std::max(1.0, std::min(1.95, om));
+}
+
 int mgShapes(int M, int N, int* H, int* W) {
     H[0] = M - 1;
     W[0] = N - 2;
@@ -197,6 +207,9 @@ struct cp_batch {
     int mgLevels = 0;  // deepest hierarchy of any multigrid slot admitted so far
     int mgTail = 0;    // deepest first tail level of any multigrid slot admitted so far
     int dpAny = 0;     // some slot admitted so far uses the pressure-drop exit
+    // backend families some slot admitted so far uses (sticky, like the
+    // multigrid shape: the captured step graph keeps its launches)
+    int anyExplicit = 0, anyImplicit = 0, anySOR = 0, anyCG = 0;
     double* dPartials = nullptr;
     PlaneReq* dPlaneReq = nullptr;
     PlaneOut* dPlaneOut = nullptr;
@@ -263,6 +276,7 @@ KArgs kargs(cp_batch* b) {
     a.partials = b->dPartials;
     a.force = 0;
     a.level = 0;
+    a.color = 0;
     return a;
 }
 
@@ -371,6 +385,12 @@ int rebuildTiles(cp_batch* b) {
     b->totalPTiles = int(pts.size());
     for (int s = 0; s < b->maxSlots; ++s)
         if (b->slots[s].used && (b->hDesc[s].dpConv || b->hDesc[s].slopeGate)) b->dpAny = 1;
+    for (int s = 0; s < b->maxSlots; ++s) {
+        if (!b->slots[s].used) continue;
+        const SlotDesc& d = b->hDesc[s];
+        (d.momType == 0 ? b->anyExplicit : b->anyImplicit) = 1;
+        (d.presType == 0 ? b->anySOR : b->anyCG) = 1;
+    }
     int counts[kMaxLevels];
     std::vector<std::vector<int>> lists(kMaxLevels);  // alive until the stream is synchronised
     // mgLevels never decreases: the captured step graph keeps its shape as
@@ -459,14 +479,19 @@ int validate(const cp_params* p, std::string* err) {
         return setError(err, CP_E_ARG, "unknown precision %d", p->precision);
     if (p->pressurePrecond != CP_PRECOND_JACOBI && p->pressurePrecond != CP_PRECOND_AMG)
         return setError(err, CP_E_ARG, "unknown pressurePrecond %d", p->pressurePrecond);
-    if (p->pressurePrecond == CP_PRECOND_AMG &&
+    if (p->pressureSolverType < 0 || p->pressureSolverType > 4)
+        return setError(err, CP_E_ARG, "unknown pressureSolverType %d (0 = SOR, 1-4 = CG)", p->pressureSolverType);
+    if (p->momentumSolverType != 0 && p->momentumSolverType != 1)
+        return setError(err, CP_E_ARG, "unknown momentumSolverType %d (0 = explicit, 1 = implicit)",
+                        p->momentumSolverType);
+    if (p->pressurePrecond == CP_PRECOND_AMG && p->pressureSolverType != 0 &&
         (!(p->mgOmega > 0) || !(p->mgOmega <= 1) || !(p->mgScale > 0) || !(p->mgScale <= 4)))
         return setError(err, CP_E_ARG, "mgOmega must be in (0, 1] and mgScale in (0, 4]");
     if (p->usePressureDropConvergence && !(p->dpConvergenceWindow >= 1 && p->dpConvergenceWindow <= CP_DP_WINDOW_MAX))
         return setError(err, CP_E_ARG, "dpConvergenceWindow must be in [1, %d]", CP_DP_WINDOW_MAX);
     if (p->usePressureDropSlopeGate && !(p->dpSlopeWindowIters >= 0 && p->dpSlopeWindowIters <= CP_DP_WINDOW_MAX))
         return setError(err, CP_E_ARG, "dpSlopeWindowIters must be in [0, %d]", CP_DP_WINDOW_MAX);
-    if (p->pressurePrecond == CP_PRECOND_AMG) {
+    if (p->pressurePrecond == CP_PRECOND_AMG && p->pressureSolverType != 0) {
         int H[kMaxLevels], W[kMaxLevels];
         const int n = mgShapes(p->M, p->N, H, W);
         if (long(H[n - 1]) * W[n - 1] > kCoarseMax)
@@ -518,7 +543,10 @@ SlotDesc makeDesc(const cp_params& p, int precision, int tileRows, int mgRows0) 
     d.slopeMaxDelta = p.dpSlopeMaxDelta;
     d.xCoreIn = p.xPlaneCoreInlet;
     d.xCoreOut = p.xPlaneCoreOutlet;
-    d.precond = p.pressurePrecond;
+    d.presType = p.pressureSolverType == 0 ? 0 : 1;
+    d.momType = p.momentumSolverType == 0 ? 0 : 1;
+    d.sorOmega = sorOmegaFor(p.sorOmega, p.M, p.N);
+    d.precond = d.presType == 0 ? CP_PRECOND_JACOBI : p.pressurePrecond;  // SOR slots build no hierarchy
     d.mgOmega = p.mgOmega;
     d.mgScale = p.mgScale;
     if (d.precond == CP_PRECOND_AMG) {
@@ -816,8 +844,12 @@ int maxOverSlots(cp_batch* b, int SlotDesc::*field) {
 
 int polledMomentum(cp_batch* b) {
     const int grid = gridFor(b, b->totalTiles);
-    int rc = launch(b, K_MOM_ASM, grid);
-    if (rc) return rc;
+    int rc;
+    if (b->anyExplicit) {
+        if ((rc = launch(b, K_MOM_EXPL, grid)) || (rc = launch(b, K_MOM_EXPL_BC, grid))) return rc;
+    }
+    if (!b->anyImplicit) return CP_OK;
+    if ((rc = launch(b, K_MOM_ASM, grid))) return rc;
     const int cap = maxOverSlots(b, &SlotDesc::maxMomIter);
     int launched = 0, chunk = 2;
     while (launched < cap) {
@@ -837,10 +869,45 @@ int polledMomentum(cp_batch* b) {
     return launch(b, K_MOM_FIN, grid);
 }
 
+// One SOR sweep pair (red, then black with the stop test).
+int sorPair(cp_batch* b, int grid) {
+    for (int color = 0; color < 2; ++color) {
+        KArgs a = kargs(b);
+        a.color = color;
+        const hipError_t e =
+            launchKernel(K_SOR, b->precision == CP_PREC_MIXED, b->tileRows, grid, b->maxSlots, b->stream, a);
+        if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+    }
+    return CP_OK;
+}
+
+int polledSOR(cp_batch* b, int grid) {
+    const int cap = maxOverSlots(b, &SlotDesc::maxPresIter);
+    int launched = 0, chunk = 4, rc;
+    while (launched < cap) {
+        const int n = std::min(chunk, cap - launched);
+        for (int k = 0; k < n; ++k)
+            if ((rc = sorPair(b, grid))) return rc;
+        launched += n;
+        if (launched >= cap) break;
+        if ((rc = pollState(b))) return rc;
+        bool pending = false;
+        for (int s = 0; s < b->maxSlots; ++s)
+            if (b->slots[s].used && b->hState[s].status == ST_RUNNING && b->hDesc[s].presType == 0 &&
+                !b->hState[s].sorDone)
+                pending = true;
+        if (!pending) break;
+        chunk = std::min(chunk * 2, 64);
+    }
+    return CP_OK;
+}
+
 int polledPressure(cp_batch* b) {
     const int grid = gridFor(b, b->totalTiles);
     int rc = launch(b, K_PRES_ASM, grid);
     if (rc) return rc;
+    if (b->anySOR && (rc = polledSOR(b, grid))) return rc;
+    if (!b->anyCG) return CP_OK;
     if ((rc = mgSetup(b)) || (rc = mgVcycle(b))) return rc;
     const int cap = maxOverSlots(b, &SlotDesc::maxPresIter);
     const int cycles = maxOverSlots(b, &SlotDesc::refineCycles);
@@ -899,11 +966,23 @@ int polledStep(cp_batch* b, bool outer) {
 // ---- graph mode (fixed inner budgets) -------------------------------------
 int enqueueFixedStep(cp_batch* b, int J, int P, int R, int grid) {
     int rc;
-    if ((rc = launch(b, K_MOM_ASM, grid))) return rc;
-    for (int k = 0; k < J; ++k)
-        if ((rc = launch(b, K_JAC, grid))) return rc;
-    if ((rc = launch(b, K_MOM_FIN, grid))) return rc;
+    if (b->anyExplicit) {
+        if ((rc = launch(b, K_MOM_EXPL, grid)) || (rc = launch(b, K_MOM_EXPL_BC, grid))) return rc;
+    }
+    if (b->anyImplicit) {
+        if ((rc = launch(b, K_MOM_ASM, grid))) return rc;
+        for (int k = 0; k < J; ++k)
+            if ((rc = launch(b, K_JAC, grid))) return rc;
+        if ((rc = launch(b, K_MOM_FIN, grid))) return rc;
+    }
     if ((rc = launch(b, K_PRES_ASM, grid))) return rc;
+    if (b->anySOR)
+        for (int k = 0; k < P; ++k)
+            if ((rc = sorPair(b, grid))) return rc;
+    if (!b->anyCG) {
+        if ((rc = launch(b, K_CORR_OUTER, grid))) return rc;
+        return dpMonitor(b);
+    }
     if ((rc = mgSetup(b)) || (rc = mgVcycle(b))) return rc;
     for (int c = 0; c < R; ++c) {
         for (int k = 0; k < P; ++k) {
@@ -938,7 +1017,8 @@ int graphRun(cp_batch* b, int steps) {
         }
         return CP_OK;
     }
-    const int shapeL = (b->mgLevels * (kMaxLevels + 1) + b->mgTail) * 2 + b->dpAny;
+    const int families = b->anyExplicit | b->anyImplicit << 1 | b->anySOR << 2 | b->anyCG << 3;
+    const int shapeL = ((b->mgLevels * (kMaxLevels + 1) + b->mgTail) * 2 + b->dpAny) * 16 + families;
     if (!b->graphExec || J != b->graphJ || P != b->graphP || R != b->graphR || shapeL != b->graphL) {
         if (b->graphExec) {
             HIP_TRY(hipGraphExecDestroy(b->graphExec));

@@ -95,6 +95,7 @@ __device__ __forceinline__ void blockReduceTo(double (&v)[NQ], double* dst) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
         v[q] = ((MaxMask >> q) & 1u) ? waveReduce<true>(v[q]) : waveReduce<false>(v[q]);
+    __syncthreads();  // thread 0 has read sh[] of the previous call
     if (lane == 0) {
 #pragma unroll
         for (int q = 0; q < NQ; ++q) sh[w][q] = v[q];
@@ -379,7 +380,7 @@ __global__ __launch_bounds__(kBlock) void k_mom_assemble(KArgs a) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
-        if (st.status != ST_RUNNING) continue;
+        if ((st.status != ST_RUNNING) | (d.momType != 1)) continue;
         const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
         const double maxVel = 3.0 * fmax(fabs(st.inletVelocity), 0.1);
         const double cfl = st.pseudoCFL;
@@ -437,6 +438,174 @@ __global__ __launch_bounds__(kBlock) void k_mom_assemble(KArgs a) {
             st.jk[0] = st.jk[1] = 0;
             st.jiters[0] = st.jiters[1] = 0;
             st.jdone[0] = st.jdone[1] = (cap <= 0) ? 1 : 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Explicit momentum (iterations.cpp:176-441, momentumSolverType 0): every U
+// and V unknown updated once from the previous iterate, with the implicit
+// path's coefficients, no linear solve.
+// ---------------------------------------------------------------------------
+// One explicit point update at node k (LDS index ks, row pitch P): writes
+// dE or dN and returns the clamped new velocity.
+template <typename T, bool IsU>
+__device__ __forceinline__ double momRowExplicit(const MomConst<T>& d, const double* u, const double* v,
+                                                 const double* pp, size_t ks, int P, size_t k, int i, int j,
+                                                 double maxVel, double cfl, double alphaLocal) {
+    double ue, uw, vn, vs;
+    if (IsU) {
+        ue = clampv(0.5 * (u[ks] + u[ks + 1]), maxVel);
+        uw = clampv(0.5 * (u[ks - 1] + u[ks]), maxVel);
+        vn = clampv(0.5 * (v[ks] + v[ks + 1]), maxVel);
+        vs = clampv(0.5 * (v[ks - P] + v[ks - P + 1]), maxVel);
+    } else {
+        ue = clampv(0.5 * (u[ks] + u[ks + P]), maxVel);
+        uw = clampv(0.5 * (u[ks - 1] + u[ks + P - 1]), maxVel);
+        vn = clampv(0.5 * (v[ks] + v[ks + P]), maxVel);
+        vs = clampv(0.5 * (v[ks - P] + v[ks]), maxVel);
+    }
+    const double cs = d.convScale;
+    const double Fe = cs * d.rho * d.hy * ue, Fw = cs * d.rho * d.hy * uw;
+    const double Fn = cs * d.rho * d.hx * vn, Fs = cs * d.rho * d.hx * vs;
+    const double aE = d.De + fmax(0.0, -Fe);
+    const double aW = d.De + fmax(0.0, Fw);
+    const double aN = d.Dn + fmax(0.0, -Fn);
+    const double aS = d.Dn + fmax(0.0, Fs);
+    const auto f = IsU ? u : v;
+    const double cur = f[ks];
+    const double Sdc = d.convScheme == 1 ? souCorrection<IsU>(f, ks, i, j, P, d.M, d.N, Fe, Fw, Fn, Fs) : 0.0;
+    const double sumA = aE + aW + aN + aS;
+    // computeLocalDt (iterations.cpp:210-216): no pseudo-time is an infinite step
+    double trans = 0.0;
+    if (d.pseudo) {
+        double dt = d.timeStep;
+        if (d.localPseudo) dt = fmin(d.timeStep, cfl * d.hChar / fmax(fabs(cur), d.minSpeed));
+        trans = d.rho * d.vol / dt;
+    }
+    double aP0 = sumA + (Fe - Fw + Fn - Fs) + trans + d.sinkDiag + alphaLocal * d.vol;
+    aP0 = fmax(aP0, 1e-10);
+    const double dDenom = fmax(aP0, 1e-12);
+    if (IsU) d.dE[k] = d.hy / dDenom;
+    else d.dN[k] = d.hx / dDenom;
+    double Sp = IsU ? (pp[ks] - pp[ks + 1]) * d.hy : (pp[ks] - pp[ks + P]) * d.hx;
+    Sp += trans * cur;  // uOld == u here
+    Sp += Sdc;
+    const double aP = aP0 / d.uvAlpha;
+    Sp += (1.0 - d.uvAlpha) / d.uvAlpha * aP0 * cur;
+    const double nv = (aE * f[ks + 1] + aW * f[ks - 1] + aN * f[ks + P] + aS * f[ks - P] + Sp) / aP;
+    return clampv(nv, maxVel);
+}
+
+// u* and v* of the unknowns, dE, dN, and per tile max |new - old| of U and V
+// (NaN ignored, as the reference's comparisons do) and a NaN flag. The
+// workgroup holding the slot's first tile marks the Jacobi state done.
+template <typename T, int R>
+__global__ __launch_bounds__(kBlock) void k_mom_explicit(KArgs a) {
+    constexpr int TH = kTileH * R, HH = TH + 4, HW = kTileW + 4;
+    __shared__ double su[HH * HW], sv[HH * HW], sp[HH * HW];
+    const int nT = *a.nTiles;
+    XCD_TILE_LOOP(t, nT) {
+        const Tile tl = decodeTile<R>(a, t);
+        const SlotDesc& d = a.desc[tl.s];
+        SlotState& st = a.state[tl.s];
+        if ((st.status != ST_RUNNING) | (d.momType != 0)) continue;
+        const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
+        const double maxVel = 3.0 * fmax(fabs(st.inletVelocity), 0.1);
+        const double cfl = st.pseudoCFL;
+        const auto gu = G(d.u), gv = G(d.v), gpp = G(d.p);
+        const auto us = G(d.us), vs = G(d.vs);
+        const MomConst<T> c(d);
+        constexpr int NE = HH * HW, NIT = (NE + kBlock - 1) / kBlock;
+        double lu[NIT], lv[NIT], lp[NIT];
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int idx = int(threadIdx.x) + it * kBlock;
+            const int h = idx / HW, w = idx - h * HW;
+            const size_t k = clampIdx(tl.i0 - 2 + h, tl.j0 - 2 + w, M, P);
+            lu[it] = gu[k];
+            lv[it] = gv[k];
+            lp[it] = gpp[k];
+        }
+        double aUr[R], aVr[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const size_t k = clampIdx(tl.i + r, j, M, P);
+            aUr[r] = G(d.aU)[k];
+            aVr[r] = G(d.aV)[k];
+        }
+        __syncthreads();  // the previous tile's LDS reads are done
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int idx = int(threadIdx.x) + it * kBlock;
+            if (NE % kBlock != 0 && idx >= NE) break;
+            const int h = idx / HW, w = idx - h * HW;
+            const int i = tl.i0 - 2 + h, jj = tl.j0 - 2 + w;
+            const bool in = i >= 0 && i <= M && jj >= 0 && jj <= N;
+            su[idx] = in ? lu[it] : 0.0;
+            sv[idx] = in ? lv[it] : 0.0;
+            sp[idx] = in ? lp[it] : 0.0;
+        }
+        __syncthreads();
+        const int lane = int(threadIdx.x & 63), wv = int(threadIdx.x >> 6);
+        double v[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = tl.i + r;
+            if (i > M || j > N) continue;
+            const size_t k = size_t(i) * P + j;
+            const size_t ks = size_t(wv * R + r + 2) * HW + (lane + 2);
+            if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2) {
+                const double nv = momRowExplicit<T, true>(c, su, sv, sp, ks, HW, k, i, j, maxVel, cfl, aUr[r]);
+                const double df = fabs(nv - su[ks]);
+                if (df > v[0]) v[0] = df;
+                if (!(nv == nv)) v[2] = 1.0;
+                us[k] = nv;
+            }
+            if (i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1) {
+                const double nv = momRowExplicit<T, false>(c, su, sv, sp, ks, HW, k, i, j, maxVel, cfl, aVr[r]);
+                const double df = fabs(nv - sv[ks]);
+                if (df > v[1]) v[1] = df;
+                if (!(nv == nv)) v[2] = 1.0;
+                vs[k] = nv;
+            }
+        }
+        blockReduceTo<3, 7u>(v, a.partials + size_t(t) * kNumPartials);
+        if (t == d.tile0 && threadIdx.x == 0) {  // no tile of this launch reads it
+            st.jk[0] = st.jk[1] = 0;
+            st.jiters[0] = st.jiters[1] = 0;
+            st.jdone[0] = st.jdone[1] = 1;
+        }
+    }
+}
+
+// Velocity BCs on (u*, v*) of explicit slots (boundaries.cpp:48-106): walls,
+// inlet, and the zero-gradient outlet from the unknowns' new values.
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_mom_expl_bc(KArgs a) {
+    const int nT = *a.nTiles;
+    XCD_TILE_LOOP(t, nT) {
+        const Tile tl = decodeTile<R>(a, t);
+        const SlotDesc& d = a.desc[tl.s];
+        const SlotState& st = a.state[tl.s];
+        if ((st.status != ST_RUNNING) | (d.momType != 0)) continue;
+        const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
+        const double Uin = st.inletVelocity;
+        const auto us = G(d.us), vs = G(d.vs), gam = G(d.gam);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = tl.i + r;
+            if (i > M || j > N) continue;
+            const size_t k = size_t(i) * P + j;
+            if (j <= N - 1) {  // u* : (M+1) x N
+                if (i == 0 || i == M) us[k] = 0.0;
+                else if (j == 0) us[k] = (gam[k + 1] > 0.5) ? Uin : 0.0;  // gamma(i-1, 0) > 0.5
+                else if (j == N - 1) us[k] = us[k - 1];
+            }
+            if (i <= M - 1) {  // v* : M x (N+1)
+                if (i == 0 || i == M - 1 || j == 0) vs[k] = 0.0;
+                else if (j == N) vs[k] = vs[k - 1];
+            }
         }
     }
 }
@@ -541,7 +710,7 @@ __global__ __launch_bounds__(kBlock) void k_mom_finish(KArgs a) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
-        if (st.status != ST_RUNNING) continue;
+        if ((st.status != ST_RUNNING) | (d.momType != 1)) continue;
         const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
         const double Uin = st.inletVelocity;
         const double maxVel = 3.0 * fmax(fabs(Uin), 0.1);
@@ -925,6 +1094,66 @@ __global__ __launch_bounds__(kBlock) void k_refine_resid(KArgs a) {
             v[1] += double(r) * double(r / aP[k]);
         }
         blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Red-black SOR on p' (pressure_solver.cpp:809-909, pressureSolverType 0),
+// in FP64 on the p' field itself: one colour per launch; the cells of a
+// colour couple only to the other colour, so each sweep is a parallel update
+// of the reference's in-place one. The partial of a tile is max |change|
+// over its cells of both colours (the second colour adds to the first's),
+// NaN changes ignored as the reference's comparison does.
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_sor_sweep(KArgs a) {
+    const int nT = *a.nTiles;
+    __shared__ double red[1];
+    XCD_TILE_LOOP(t, nT) {
+        const Tile tl = decodeTile<R>(a, t);
+        const SlotDesc& d = a.desc[tl.s];
+        const SlotState& st = a.state[tl.s];
+        const int status = st.status, done = st.sorDone;
+        if ((status != ST_RUNNING) | (d.presType != 0) | (done != 0)) continue;
+        const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
+        const double rho = d.rho, hx = d.hx, hy = d.hy, om = d.sorOmega;
+        const auto dE = G(d.dE), dN = G(d.dN), bm = G(d.bm), pc = G(d.pc);
+        double mc = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = tl.i + r;
+            if (!(i >= 1 && i <= M - 1 && j >= 1 && j <= N - 1) || ((i + j) & 1) != a.color) continue;
+            const size_t k = size_t(i) * P + j;
+            const double aE = (j < N - 1) ? rho * dE[k] * hy : 0.0;
+            const double aW = (j > 1) ? rho * dE[k - 1] * hy : 0.0;
+            const double aN = (i < M - 1) ? rho * dN[k] * hx : 0.0;
+            const double aS = (i > 1) ? rho * dN[k - P] * hx : 0.0;
+            const double aP = aE + aW + aN + aS;
+            if (j == N - 1 || aP < 1e-20) {
+                pc[k] = 0.0;
+                continue;
+            }
+            const double b = bm[k];  // mass imbalance of the cell (k_pres_assemble)
+            const double pE = (j < N - 1) ? pc[k + 1] : 0.0;
+            const double pW = (j > 1) ? pc[k - 1] : 0.0;
+            const double pN = (i < M - 1) ? pc[k + P] : 0.0;
+            const double pS = (i > 1) ? pc[k - P] : 0.0;
+            const double pGS = (aE * pE + aW * pW + aN * pN + aS * pS + b) / aP;
+            const double cur = pc[k];
+            double pNew = cur + om * (pGS - cur);
+            pNew = fmax(-10000.0, fmin(10000.0, pNew));
+            const double ch = fabs(pNew - cur);
+            if (ch > mc) mc = ch;
+            pc[k] = pNew;
+        }
+        double v[1] = {mc};
+        blockReduceTo<1, 1u>(v, red);
+        if (threadIdx.x == 0) {
+            double* part = a.partials + size_t(t) * kNumPartials;
+            double m = red[0];
+            if (a.color == 1 && part[0] > m) m = part[0];
+            part[0] = m;
+        }
     }
 }
 
@@ -1879,7 +2108,9 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
             }
             break;
         }
-        case PH_MOM_FIN: {
+        case PH_MOM_FIN:
+        case PH_MOM_EXPL: {
+            if (d.momType != (phase == PH_MOM_EXPL ? 0 : 1)) return;
             double tot[3];
             slotReduce<3, 7u>(a.partials, d.tile0, d.ntiles, tot);
             if (threadIdx.x == 0) {
@@ -1888,6 +2119,18 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
                 if (tot[2] != 0.0) st.health = 1;
                 st.itU = st.jiters[0];
                 st.itV = st.jiters[1];
+            }
+            break;
+        }
+        case PH_SOR: {
+            if (d.presType != 0 || st.sorDone) return;
+            double tot[1];
+            slotReduce<1, 1u>(a.partials, d.tile0, d.ntiles, tot);
+            if (threadIdx.x == 0) {
+                st.piters += 1;
+                st.pitTotal = st.piters;
+                // pressure_solver.cpp:906 (a NaN maximum never stops the sweeps)
+                if (tot[0] < d.pTol || st.piters >= d.maxPresIter) st.sorDone = 1;
             }
             break;
         }
@@ -1902,8 +2145,15 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
                 st.piters = 0;
                 st.pitTotal = 0;
                 st.pcycle = 0;
-                st.pdone = (d.maxPresIter <= 0) ? 1 : 0;
-                st.stepPhase = 0;  // refinement not finished
+                if (d.presType == 0) {
+                    // SOR slot: the CG, multigrid and refinement kernels skip it
+                    st.pdone = 1;
+                    st.stepPhase = 1;
+                    st.sorDone = (d.maxPresIter <= 0) ? 1 : 0;
+                } else {
+                    st.pdone = (d.maxPresIter <= 0) ? 1 : 0;
+                    st.stepPhase = 0;  // refinement not finished
+                }
             }
             break;
         }
@@ -1986,7 +2236,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
             slotReduce<1, 1u>(a.partials, d.tile0, d.ntiles, tot);
             if (threadIdx.x == 0) {
                 if (tot[0] != 0.0) st.health = 1;
-                st.itP = d.mixed ? st.pitTotal : st.piters;
+                st.itP = (d.mixed && d.presType == 1) ? st.pitTotal : st.piters;
                 if (outer) controller(d, st);
             }
             break;
@@ -2034,6 +2284,9 @@ static void launchTR(Kern k, int grid, hipStream_t s, const KArgs& a) {
         case K_MG_FACTOR: hipLaunchKernelGGL((k_mg_factor<T>), g, b, 0, s, a); break;
         case K_MG_TAIL: hipLaunchKernelGGL((k_mg_tail<T>), g, dim3(kTailBlock), 0, s, a); break;
         case K_DP_MONITOR: hipLaunchKernelGGL(k_dp_monitor, g, b, 0, s, a); break;
+        case K_MOM_EXPL: hipLaunchKernelGGL((k_mom_explicit<T, R>), g, b, 0, s, a); break;
+        case K_MOM_EXPL_BC: hipLaunchKernelGGL((k_mom_expl_bc<R>), g, b, 0, s, a); break;
+        case K_SOR: hipLaunchKernelGGL((k_sor_sweep<R>), g, b, 0, s, a); break;
     }
 }
 
@@ -2060,6 +2313,8 @@ static int phaseOf(Kern k) {
         case K_CORR:
         case K_CORR_OUTER: return PH_CORR;
         case K_MG_UP: return PH_MG_Z;  // the finest level's up-sweep only
+        case K_MOM_EXPL_BC: return PH_MOM_EXPL;
+        case K_SOR: return PH_SOR;     // after the second colour only
         default: return -1;
     }
 }
@@ -2072,7 +2327,7 @@ hipError_t launchKernel(Kern k, bool f32, int rows, int grid, int nslots, hipStr
     if (f32) launchT<float>(k, rows, grid, s, a);
     else launchT<double>(k, rows, grid, s, a);
     if (evAfter) (void)hipEventRecordWithFlags(evAfter, s, hipEventRecordExternal);
-    const int ph = (k == K_MG_UP && a.level != 0) ? -1 : phaseOf(k);
+    const int ph = ((k == K_MG_UP && a.level != 0) || (k == K_SOR && a.color == 0)) ? -1 : phaseOf(k);
     if (ph >= 0)
         hipLaunchKernelGGL(k_finalize, dim3(nslots), dim3(kBlock), 0, s, a, ph, k == K_CORR_OUTER ? 1 : 0);
     return hipGetLastError();

@@ -44,6 +44,9 @@ static inline void cp_fill_reference_defaults(cp_params* p) {
     p->usePressureDropSlopeGate = 0; p->dpSlopeWindowIters = 1000;      /* SIMPLE.h:101-102 */
     p->dpSlopeMaxDelta = 10.0;                                           /* SIMPLE.h:103 */
     p->xPlaneCoreInlet = 0.01; p->xPlaneCoreOutlet = 0.050;             /* SIMPLE.h:88-89 */
+    p->pressureSolverType = 1;  /* the CG backend (the reference's default 4 is AMG-CG on a GPU) */
+    p->momentumSolverType = 1;                                           /* SIMPLE.h:172 */
+    p->sorOmega = 0.0;                                                   /* SIMPLE.h:156 */
 }
 
 #endif /* COLDPLATE_DEFAULTS_H */

@@ -141,6 +141,20 @@ typedef struct cp_params {
     int32_t usePressureDropSlopeGate, dpSlopeWindowIters;
     double  dpSlopeMaxDelta;
     double  xPlaneCoreInlet, xPlaneCoreOutlet;
+    /* The reference's backend seams (SIMPLE.h:145, 156, 172).
+       pressureSolverType 0 = red-black SOR on p' (pressure_solver.cpp:
+       809-909): at most maxPressureIter red+black sweeps, stopping once a
+       sweep pair changes no p' by pTol or more; sorOmega outside (0, 2)
+       selects 2 / (1 + sqrt(1 - cos^2(pi / max(M, N)))) clamped to
+       [1, 1.95] (pressure_solver.cpp:811-815). pressureSolverType 1-4 =
+       the CG backend (the reference's 1 = Jacobi-preconditioned CG, 2-4 =
+       its other solvers of the same system; pressurePrecond picks the
+       preconditioner here).
+       momentumSolverType 0 = one explicit point update of u* and v* from
+       the previous iterate (iterations.cpp:176-441); 1 = implicit assembly
+       and Jacobi (momentum_solver.cpp:164-339). */
+    int32_t pressureSolverType, momentumSolverType;
+    double  sorOmega;
 } cp_params;
 
 #define CP_DP_WINDOW_MAX 4096

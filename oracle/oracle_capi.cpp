@@ -146,13 +146,20 @@ int or_run(void* hp, int max_steps, cp_step_stats* st, double* wall_s) {
 
 int or_momentum(void* hp, cp_step_stats* st) {
     Handle* h = static_cast<Handle*>(hp);
-    DISPATCH(h, { s.momentum(); fillStats(s, st); });
+    DISPATCH(h, {
+        if (s.prm.momentumSolverType == 0) s.momentumExplicit();
+        else s.momentum();
+        fillStats(s, st);
+    });
     return 0;
 }
 
 int or_pressure(void* hp, cp_step_stats* st) {
     Handle* h = static_cast<Handle*>(hp);
-    DISPATCH(h, { s.lastItP = s.solvePressure(); fillStats(s, st); });
+    DISPATCH(h, {
+        s.lastItP = s.prm.pressureSolverType == 0 ? s.solvePressureSOR() : s.solvePressure();
+        fillStats(s, st);
+    });
     return 0;
 }
 

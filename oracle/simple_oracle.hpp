@@ -757,6 +757,147 @@ struct Simple {
         velocityBC(u, v);
     }
 
+    // Explicit momentum sweep of one component (iterations.cpp:255-417,
+    // momentumSolverType 0): every unknown updated once from the previous
+    // iterate; returns max |new - old| over the unknowns.
+    template <bool IsU>
+    T explicitMomentum() {
+        const T vol = hx * hy;
+        const T maxVel = T(3) * std::max(std::abs(inletVelocity), T(0.1));
+        const T sinkCoeff = (prm.enableTwoPointFiveD && Ht > T(0))
+                                ? sinkMult * (T(5) * eta / (T(2) * Ht * Ht)) : T(0);
+        const T sinkDiag = sinkCoeff * vol;
+        const T hChar = std::min(hx, hy);
+        const T De = eta * hy / hx, Dn = eta * hx / hy;
+        const T convScale = prm.enableConvectionScaling ? (T(6) / T(7)) : T(1);
+        const int iEnd = IsU ? M : M - 1, jEnd = IsU ? N - 1 : N;
+        T r = T(0);
+#pragma omp parallel for reduction(max : r) schedule(static)
+        for (int i = 1; i < iEnd; ++i)
+            for (int j = 1; j < jEnd; ++j) {
+                const T alphaLocal = IsU ? alphaAtU(i, j) : alphaAtV(i, j);
+                T ue, uw, vn, vs, cur, east, west, north, south;
+                if (IsU) {
+                    ue = clampv(T(0.5) * (U(u, i, j) + U(u, i, j + 1)), -maxVel, maxVel);
+                    uw = clampv(T(0.5) * (U(u, i, j - 1) + U(u, i, j)), -maxVel, maxVel);
+                    vn = clampv(T(0.5) * (V(v, i, j) + V(v, i, j + 1)), -maxVel, maxVel);
+                    vs = clampv(T(0.5) * (V(v, i - 1, j) + V(v, i - 1, j + 1)), -maxVel, maxVel);
+                    cur = U(u, i, j);
+                    east = U(u, i, j + 1); west = U(u, i, j - 1); north = U(u, i + 1, j); south = U(u, i - 1, j);
+                } else {
+                    ue = clampv(T(0.5) * (U(u, i, j) + U(u, i + 1, j)), -maxVel, maxVel);
+                    uw = clampv(T(0.5) * (U(u, i, j - 1) + U(u, i + 1, j - 1)), -maxVel, maxVel);
+                    vn = clampv(T(0.5) * (V(v, i, j) + V(v, i + 1, j)), -maxVel, maxVel);
+                    vs = clampv(T(0.5) * (V(v, i - 1, j) + V(v, i, j)), -maxVel, maxVel);
+                    cur = V(v, i, j);
+                    east = V(v, i, j + 1); west = V(v, i, j - 1); north = V(v, i + 1, j); south = V(v, i - 1, j);
+                }
+                const T Fe = convScale * rho * hy * ue, Fw = convScale * rho * hy * uw;
+                const T Fn = convScale * rho * hx * vn, Fs = convScale * rho * hx * vs;
+                const T aE = De + std::max(T(0), -Fe);
+                const T aW = De + std::max(T(0), Fw);
+                const T aN = Dn + std::max(T(0), -Fn);
+                const T aS = Dn + std::max(T(0), Fs);
+                const T Sdc = prm.convectionScheme == 1 ? (IsU ? souU(i, j, Fe, Fw, Fn, Fs) : souV(i, j, Fe, Fw, Fn, Fs))
+                                                        : T(0);
+                const T sumA = aE + aW + aN + aS;
+                // computeLocalDt (iterations.cpp:210-216): no pseudo-time -> infinite step
+                T transCoeff = T(0);
+                if (prm.enablePseudoTime) {
+                    T dtLocal = timeStep;
+                    if (prm.useLocalPseudoTime) {
+                        const T speed = std::max(std::abs(cur), minPseudoSpeed);
+                        dtLocal = std::min(timeStep, pseudoCFL * hChar / speed);
+                    }
+                    transCoeff = rho * vol / dtLocal;
+                }
+                const T brinkmanDrag = alphaLocal * vol;
+                T aP0 = sumA + (Fe - Fw + Fn - Fs) + transCoeff + sinkDiag + brinkmanDrag;
+                aP0 = std::max(aP0, T(1e-10));
+                const T dDenom = std::max(aP0, T(1e-12));
+                T Sp;
+                if (IsU) {
+                    U(dE, i, j) = hy / dDenom;
+                    Sp = (P(p, i, j) - P(p, i, j + 1)) * hy;
+                } else {
+                    V(dN, i, j) = hx / dDenom;
+                    Sp = (P(p, i, j) - P(p, i + 1, j)) * hx;
+                }
+                Sp += transCoeff * cur;  // uOld == u here
+                Sp += Sdc;
+                const T aP = aP0 / uvAlpha;
+                Sp += (T(1) - uvAlpha) / uvAlpha * aP0 * cur;
+                T nv = (aE * east + aW * west + aN * north + aS * south + Sp) / aP;
+                nv = clampv(nv, -maxVel, maxVel);
+                r = std::max(r, std::abs(nv - cur));
+                if (IsU) U(uS, i, j) = nv;
+                else V(vS, i, j) = nv;
+            }
+        return r;
+    }
+
+    // Explicit momentum half of calculateStep (iterations.cpp:222-441).
+    void momentumExplicit() {
+        uOld = u;
+        vOld = v;
+        residU = explicitMomentum<true>();
+        residV = explicitMomentum<false>();
+        lastItU = lastItV = 0;
+        velocityBC(uS, vS);
+    }
+
+    // Over-relaxation factor of the SOR backend (pressure_solver.cpp:810-816).
+    static double sorOmegaFor(double w, int M, int N) {
+        if (w > 0.0 && w < 2.0) return w;
+        const double rs = std::cos(M_PI / double(std::max(M, N)));
+        const double om = 2.0 / (1.0 + std::sqrt(1.0 - rs * rs));
+        return std::max(1.0, std::min(1.95, om));
+    }
+
+    // Mass residual + red-black SOR on p' (pressure_solver.cpp:462-486,
+    // 809-909, pressureSolverType 0). Returns the iteration count.
+    int solvePressureSOR() {
+        std::fill(pS.begin(), pS.end(), T(0));
+        T rm = T(0);
+        const int jEnd = prm.massResidualScope ? N - 1 : N;
+        for (int i = 1; i < M; ++i)
+            for (int j = 1; j < jEnd; ++j) rm = std::max(rm, std::abs(massImbalance(i, j)));
+        residMass = rm;
+        const T omega = T(sorOmegaFor(prm.sorOmega, M, N));
+        int it = 0;
+        for (int pIter = 0; pIter < prm.maxPressureIter; ++pIter) {
+            T maxChange = T(0);
+            it = pIter + 1;
+            for (int color = 0; color < 2; ++color)
+                for (int i = 1; i < M; ++i)
+                    for (int j = 1; j < N; ++j) {
+                        if ((i + j) % 2 != color) continue;
+                        const T aE = (j < N - 1) ? rho * U(dE, i, j) * hy : T(0);
+                        const T aW = (j > 1) ? rho * U(dE, i, j - 1) * hy : T(0);
+                        const T aN = (i < M - 1) ? rho * V(dN, i, j) * hx : T(0);
+                        const T aS = (i > 1) ? rho * V(dN, i - 1, j) * hx : T(0);
+                        const T aP = aE + aW + aN + aS;
+                        if (j == N - 1 || aP < T(1e-20)) {
+                            P(pS, i, j) = T(0);
+                            continue;
+                        }
+                        const T b = massImbalance(i, j);
+                        const T pE = (j < N - 1) ? P(pS, i, j + 1) : T(0);
+                        const T pW = (j > 1) ? P(pS, i, j - 1) : T(0);
+                        const T pN = (i < M - 1) ? P(pS, i + 1, j) : T(0);
+                        const T pSo = (i > 1) ? P(pS, i - 1, j) : T(0);
+                        const T pGS = (aE * pE + aW * pW + aN * pN + aS * pSo + b) / aP;
+                        const T pc = P(pS, i, j);
+                        T pNew = pc + omega * (pGS - pc);
+                        pNew = std::max(T(-10000), std::min(T(10000), pNew));
+                        maxChange = std::max(maxChange, std::abs(pNew - pc));
+                        P(pS, i, j) = pNew;
+                    }
+            if (maxChange < pTol) break;
+        }
+        return it;
+    }
+
     // Momentum half of calculateStep (iterations.cpp:118-135).
     void momentum() {
         uOld = u;
@@ -769,10 +910,12 @@ struct Simple {
         velocityBC(uS, vS);
     }
 
-    // calculateStep (iterations.cpp:82-526), implicit-momentum branch.
+    // calculateStep (iterations.cpp:82-526): momentum (implicit or explicit),
+    // pressure correction (CG or SOR), correction.
     T calculateStep() {
-        momentum();
-        lastItP = solvePressure();
+        if (prm.momentumSolverType == 0) momentumExplicit();
+        else momentum();
+        lastItP = prm.pressureSolverType == 0 ? solvePressureSOR() : solvePressure();
         correct();
         return std::max({residU, residV, residMass});
     }
