@@ -87,29 +87,43 @@ __device__ __forceinline__ double waveReduce(double v) {
 }
 
 // Reduces NQ values over the 256-thread block; MaxMask bit q selects max for
-// value q (else sum). Thread 0 writes the results to dst[0..NQ).
+// value q (else sum). Thread 0 writes the results to dst[0..NQ). `calls`
+// counts the block's calls (every wave makes the same calls, in order):
+// consecutive calls alternate between two LDS buffers, so a call's writes
+// never meet thread 0's reads of the call before it — between two calls on
+// one buffer lies the barrier of the call in between, which thread 0 reaches
+// only after its reads.
 template <int NQ, unsigned MaxMask>
-__device__ __forceinline__ void blockReduceTo(double (&v)[NQ], double* dst) {
-    __shared__ double sh[kBlock / 64][4];
+__device__ __forceinline__ void blockReduceTo(double (&v)[NQ], double* dst, int& calls) {
+    __shared__ double sh[2][kBlock / 64][4];
+    const int b = calls & 1;
+    ++calls;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
         v[q] = ((MaxMask >> q) & 1u) ? waveReduce<true>(v[q]) : waveReduce<false>(v[q]);
-    __syncthreads();  // thread 0 has read sh[] of the previous call
     if (lane == 0) {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) sh[w][q] = v[q];
+        for (int q = 0; q < NQ; ++q) sh[b][w][q] = v[q];
     }
     __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            double r = sh[0][q];
+            double r = sh[b][0][q];
             for (int k = 1; k < kBlock / 64; ++k)
-                r = ((MaxMask >> q) & 1u) ? nanmax(r, sh[k][q]) : r + sh[k][q];
+                r = ((MaxMask >> q) & 1u) ? nanmax(r, sh[b][k][q]) : r + sh[b][k][q];
             dst[q] = r;
         }
     }
+}
+
+// One reduction of a block that makes no other call of this form between
+// barriers (the per-slot kernels).
+template <int NQ, unsigned MaxMask>
+__device__ __forceinline__ void blockReduceTo(double (&v)[NQ], double* dst) {
+    int calls = 0;
+    blockReduceTo<NQ, MaxMask>(v, dst, calls);
 }
 
 // Sums (or maxes) the partials of one slot's tiles in tile order (ntiles >=
@@ -505,6 +519,7 @@ __global__ __launch_bounds__(kBlock) void k_mom_explicit(KArgs a) {
     constexpr int TH = kTileH * R, HH = TH + 4, HW = kTileW + 4;
     __shared__ double su[HH * HW], sv[HH * HW], sp[HH * HW];
     const int nT = *a.nTiles;
+    int nred = 0;  // block reductions made so far (alternating buffers)
     XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
@@ -570,7 +585,7 @@ __global__ __launch_bounds__(kBlock) void k_mom_explicit(KArgs a) {
                 vs[k] = nv;
             }
         }
-        blockReduceTo<3, 7u>(v, a.partials + size_t(t) * kNumPartials);
+        blockReduceTo<3, 7u>(v, a.partials + size_t(t) * kNumPartials, nred);
         if (t == d.tile0 && threadIdx.x == 0) {  // no tile of this launch reads it
             st.jk[0] = st.jk[1] = 0;
             st.jiters[0] = st.jiters[1] = 0;
@@ -674,6 +689,7 @@ __device__ __forceinline__ void jacobiWave(const Tile& tl, int M, int N, int P, 
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_jacobi(KArgs a) {
     const int nT = *a.nTiles;
+    int nred = 0;  // block reductions made so far (alternating buffers)
     XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
@@ -695,7 +711,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(KArgs a) {
         if (!doneV)
             jacobiWave<T, R>(tl, M, N, P, M - 2, N - 1, cv, G(static_cast<const T*>(d.mv[M_D0 + pv])),
                              G(static_cast<T*>(d.mv[M_D0 + (pv ^ 1)])), v[1]);
-        blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials);
+        blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials, nred);
     }
 }
 
@@ -706,6 +722,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(KArgs a) {
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_mom_finish(KArgs a) {
     const int nT = *a.nTiles;
+    int nred = 0;  // block reductions made so far (alternating buffers)
     XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
@@ -772,7 +789,7 @@ __global__ __launch_bounds__(kBlock) void k_mom_finish(KArgs a) {
                 if (!(val == val)) v[2] = 1.0;
             }
         }
-        blockReduceTo<3, 7u>(v, a.partials + size_t(t) * kNumPartials);
+        blockReduceTo<3, 7u>(v, a.partials + size_t(t) * kNumPartials, nred);
     }
 }
 
@@ -783,6 +800,7 @@ __global__ __launch_bounds__(kBlock) void k_mom_finish(KArgs a) {
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_pres_assemble(KArgs a) {
     const int nT = *a.nTiles;
+    int nred = 0;  // block reductions made so far (alternating buffers)
     XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
@@ -840,7 +858,7 @@ __global__ __launch_bounds__(kBlock) void k_pres_assemble(KArgs a) {
                 v[1] += double(rr) * double(z);
             }
         }
-        blockReduceTo<2, 1u>(v, a.partials + size_t(t) * kNumPartials);
+        blockReduceTo<2, 1u>(v, a.partials + size_t(t) * kNumPartials, nred);
     }
 }
 
@@ -948,6 +966,7 @@ __device__ __forceinline__ void pcgQWave(const SlotDesc& d, const Tile& tl, int 
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_pcg_q(KArgs a) {
     const int nT = *a.nTiles;
+    int nred = 0;  // block reductions made so far (alternating buffers)
     XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodePTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
@@ -966,7 +985,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_q(KArgs a) {
             if (kk == 0) pcgQWave<T, R, false, true>(d, tl, kk, beta, v[0]);
             else pcgQWave<T, R, false, false>(d, tl, kk, beta, v[0]);
         }
-        blockReduceTo<1, 0u>(v, a.partials + size_t(t) * kNumPartials);
+        blockReduceTo<1, 0u>(v, a.partials + size_t(t) * kNumPartials, nred);
     }
 }
 
@@ -977,6 +996,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_q(KArgs a) {
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
     const int nT = *a.nTiles;
+    int nred = 0;  // block reductions made so far (alternating buffers)
     XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodePTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
@@ -1024,7 +1044,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
                 v[1] += double(rn.y) * double(rn.y / as[rr].y);
             }
         }
-        blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials);
+        blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials, nred);
     }
 }
 
@@ -1066,6 +1086,7 @@ __global__ __launch_bounds__(kBlock) void k_refine_accum(KArgs a) {
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_refine_resid(KArgs a) {
     const int nT = *a.nTiles;
+    int nred = 0;  // block reductions made so far (alternating buffers)
     XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodePTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
@@ -1093,7 +1114,7 @@ __global__ __launch_bounds__(kBlock) void k_refine_resid(KArgs a) {
             v[0] += r64 * r64;
             v[1] += double(r) * double(r / aP[k]);
         }
-        blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials);
+        blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials, nred);
     }
 }
 
@@ -1109,6 +1130,7 @@ template <int R>
 __global__ __launch_bounds__(kBlock) void k_sor_sweep(KArgs a) {
     const int nT = *a.nTiles;
     __shared__ double red[1];
+    int nred = 0;  // block reductions made so far (alternating buffers)
     XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
@@ -1147,7 +1169,7 @@ __global__ __launch_bounds__(kBlock) void k_sor_sweep(KArgs a) {
             pc[k] = pNew;
         }
         double v[1] = {mc};
-        blockReduceTo<1, 1u>(v, red);
+        blockReduceTo<1, 1u>(v, red, nred);
         if (threadIdx.x == 0) {
             double* part = a.partials + size_t(t) * kNumPartials;
             double m = red[0];
@@ -1671,6 +1693,7 @@ template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_mg_up(KArgs a) {
     constexpr int NR = R + 4;
     const int nT = *a.nTiles;
+    int nred = 0;  // block reductions made so far (alternating buffers)
     XCD_TILE_LOOP(t, nT) {
         const MgTile tl = decodeMgTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
@@ -1727,7 +1750,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(KArgs a) {
                 v[0] += double(b[h]) * double(z);
             }
         }
-        if (a.level == 0) blockReduceTo<1, 0u>(v, a.partials + size_t(t) * kNumPartials);
+        if (a.level == 0) blockReduceTo<1, 0u>(v, a.partials + size_t(t) * kNumPartials, nred);
     }
 }
 
@@ -1809,6 +1832,7 @@ __device__ void controller(const SlotDesc& d, SlotState& st) {
 template <int R>
 __global__ __launch_bounds__(kBlock) void k_correct(KArgs a) {
     const int nT = *a.nTiles;
+    int nred = 0;  // block reductions made so far (alternating buffers)
     XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
@@ -1877,7 +1901,7 @@ __global__ __launch_bounds__(kBlock) void k_correct(KArgs a) {
             }
         }
         double vb[1] = {bad};
-        blockReduceTo<1, 1u>(vb, a.partials + size_t(t) * kNumPartials);
+        blockReduceTo<1, 1u>(vb, a.partials + size_t(t) * kNumPartials, nred);
     }
 }
 
