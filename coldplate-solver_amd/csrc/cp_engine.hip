@@ -849,9 +849,9 @@ int polledMomentum(cp_batch* b) {
         if ((rc = launch(b, K_MOM_EXPL, grid)) || (rc = launch(b, K_MOM_EXPL_BC, grid))) return rc;
     }
     if (!b->anyImplicit) return CP_OK;
-    if ((rc = launch(b, K_MOM_ASM, grid))) return rc;
+    if ((rc = launch(b, K_MOM_ASM, grid))) return rc;  // with the first sweep
     const int cap = maxOverSlots(b, &SlotDesc::maxMomIter);
-    int launched = 0, chunk = 2;
+    int launched = 1, chunk = 2;
     while (launched < cap) {
         const int n = std::min(chunk, cap - launched);
         for (int k = 0; k < n; ++k)
@@ -970,8 +970,8 @@ int enqueueFixedStep(cp_batch* b, int J, int P, int R, int grid) {
         if ((rc = launch(b, K_MOM_EXPL, grid)) || (rc = launch(b, K_MOM_EXPL_BC, grid))) return rc;
     }
     if (b->anyImplicit) {
-        if ((rc = launch(b, K_MOM_ASM, grid))) return rc;
-        for (int k = 0; k < J; ++k)
+        if ((rc = launch(b, K_MOM_ASM, grid))) return rc;  // with the first sweep
+        for (int k = 1; k < J; ++k)
             if ((rc = launch(b, K_JAC, grid))) return rc;
         if ((rc = launch(b, K_MOM_FIN, grid))) return rc;
     }

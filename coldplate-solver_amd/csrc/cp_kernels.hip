@@ -311,24 +311,29 @@ struct MomConst {
     int M, N, pseudo, localPseudo, convScheme;
     double convScale, rho, hx, hy, De, Dn, timeStep, minSpeed, hChar, sinkDiag, vol, uvAlpha;
     gp<double> dE, dN;
-    gp<T> mu[M_D1], mv[M_D1];
+    gp<T> mu[M_COUNT], mv[M_COUNT];
     __device__ __forceinline__ explicit MomConst(const SlotDesc& d)
         : M(d.M), N(d.N), pseudo(d.pseudo), localPseudo(d.localPseudo), convScheme(d.convScheme),
           convScale(d.convScale), rho(d.rho), hx(d.hx), hy(d.hy), De(d.De), Dn(d.Dn), timeStep(d.timeStep),
           minSpeed(d.minSpeed), hChar(d.hChar), sinkDiag(d.sinkDiag), vol(d.vol), uvAlpha(d.uvAlpha), dE(G(d.dE)),
           dN(G(d.dN)) {
 #pragma unroll
-        for (int q = 0; q < M_D1; ++q) {
+        for (int q = 0; q < M_COUNT; ++q) {
             mu[q] = G(static_cast<T*>(d.mu[q]));
             mv[q] = G(static_cast<T*>(d.mv[q]));
         }
     }
 };
 
+// One momentum row; returns the stored residual r0 = b - A u. With `sweep`
+// it also makes the first Jacobi sweep: from delta = 0 the sweep's residual
+// is r0 and its iterate 0 + r0 / aP, in the arithmetic of k_jacobi (T, the
+// stored aP), written to the second delta buffer. Without it (no sweeps)
+// the first buffer, which the momentum finish then reads, is zeroed.
 template <typename T, bool IsU>
-__device__ __forceinline__ void momRow(const MomConst<T>& d, const double* u, const double* v, const double* pp,
-                                       size_t ks, int P, size_t k, int i, int j, double maxVel, double cfl,
-                                       double alphaLocal) {
+__device__ __forceinline__ T momRow(const MomConst<T>& d, const double* u, const double* v, const double* pp,
+                                    size_t ks, int P, size_t k, int i, int j, double maxVel, double cfl,
+                                    double alphaLocal, bool sweep) {
     double ue, uw, vn, vs;
     if (IsU) {
         ue = clampv(0.5 * (u[ks] + u[ks + 1]), maxVel);
@@ -377,25 +382,41 @@ __device__ __forceinline__ void momRow(const MomConst<T>& d, const double* u, co
     A[M_AW][k] = T(aW);
     A[M_AN][k] = T(aN);
     A[M_AS][k] = T(aS);
-    A[M_R0][k] = T(r0);
-    A[M_D0][k] = T(0);
+    const T r0t = T(r0), aPt = T(aP);
+    A[M_R0][k] = r0t;
+    if (sweep) A[M_D1][k] = T(0) + r0t / aPt;
+    else A[M_D0][k] = T(0);
+    return r0t;
 }
+
+// Occupancy hint of the momentum row kernels: without it the compiler keeps
+// 197 VGPRs (two waves per SIMD); at three waves it fits 168 without spills,
+// and the assembly of 8 x 1024x512 runs in 118 instead of 134 us. Eight-row
+// tiles need more registers than three waves allow and keep one.
+constexpr int kMomWavesPerSimd = 3;
+template <int R>
+constexpr int momWaves() { return R >= 8 ? 1 : kMomWavesPerSimd; }
 
 // The workgroup stages u, v and p of its tile with a two-deep halo (the SOU
 // stencil reaches two nodes out) in LDS, so every neighbour read of the
 // momentum rows is an LDS read; nodes outside the grid hold 0, and no row
-// reads them.
+// reads them. The kernel also makes the first Jacobi sweep of both systems
+// (momRow) and leaves its per-tile residual partials, summed per lane in row
+// order and block-reduced as k_jacobi does, so the finalisation after it is
+// the first sweep's stop test.
 template <typename T, int R>
-__global__ __launch_bounds__(kBlock) void k_mom_assemble(KArgs a) {
+__global__ __launch_bounds__(kBlock, momWaves<R>()) void k_mom_assemble(KArgs a) {
     constexpr int TH = kTileH * R, HH = TH + 4, HW = kTileW + 4;
     __shared__ double su[HH * HW], sv[HH * HW], sp[HH * HW];
     const int nT = *a.nTiles;
+    int nred = 0;  // block reductions made so far (alternating buffers)
     XCD_TILE_LOOP(t, nT) {
         const Tile tl = decodeTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
         if ((st.status != ST_RUNNING) | (d.momType != 1)) continue;
         const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
+        const bool sweep = d.maxMomIter > 0;
         const double maxVel = 3.0 * fmax(fabs(st.inletVelocity), 0.1);
         const double cfl = st.pseudoCFL;
         const auto gu = G(d.u), gv = G(d.v), gpp = G(d.p);
@@ -434,17 +455,23 @@ __global__ __launch_bounds__(kBlock) void k_mom_assemble(KArgs a) {
         }
         __syncthreads();
         const int lane = int(threadIdx.x & 63), wv = int(threadIdx.x >> 6);
+        double v[2] = {0.0, 0.0};
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int i = tl.i + r;
             if (i > M || j > N) continue;
             const size_t k = size_t(i) * P + j;
             const size_t ks = size_t(wv * R + r + 2) * HW + (lane + 2);
-            if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2)
-                momRow<T, true>(c, su, sv, sp, ks, HW, k, i, j, maxVel, cfl, aUr[r]);
-            if (i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1)
-                momRow<T, false>(c, su, sv, sp, ks, HW, k, i, j, maxVel, cfl, aVr[r]);
+            if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2) {
+                const T res = momRow<T, true>(c, su, sv, sp, ks, HW, k, i, j, maxVel, cfl, aUr[r], sweep);
+                v[0] += double(res) * double(res);
+            }
+            if (i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1) {
+                const T res = momRow<T, false>(c, su, sv, sp, ks, HW, k, i, j, maxVel, cfl, aVr[r], sweep);
+                v[1] += double(res) * double(res);
+            }
         }
+        blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials, nred);
         // the workgroup holding the slot's first tile resets the Jacobi state
         // (no tile of this kernel reads it)
         if (t == d.tile0 && threadIdx.x == 0) {
@@ -515,7 +542,7 @@ __device__ __forceinline__ double momRowExplicit(const MomConst<T>& d, const dou
 // (NaN ignored, as the reference's comparisons do) and a NaN flag. The
 // workgroup holding the slot's first tile marks the Jacobi state done.
 template <typename T, int R>
-__global__ __launch_bounds__(kBlock) void k_mom_explicit(KArgs a) {
+__global__ __launch_bounds__(kBlock, momWaves<R>()) void k_mom_explicit(KArgs a) {
     constexpr int TH = kTileH * R, HH = TH + 4, HW = kTileW + 4;
     __shared__ double su[HH * HW], sv[HH * HW], sp[HH * HW];
     const int nT = *a.nTiles;
@@ -2327,6 +2354,7 @@ static void launchT(Kern k, int rows, int grid, hipStream_t s, const KArgs& a) {
 // finalisation phase of each work kernel (-1: none)
 static int phaseOf(Kern k) {
     switch (k) {
+        case K_MOM_ASM:  // the assembly makes the first sweep
         case K_JAC: return PH_JAC;
         case K_MOM_FIN: return PH_MOM_FIN;
         case K_PRES_ASM: return PH_PRES_ASM;
