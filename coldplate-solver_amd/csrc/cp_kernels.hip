@@ -777,7 +777,7 @@ __global__ __launch_bounds__(kBlock) void k_mom_finish(KArgs a) {
             dum[r] = dU[kw];
             dvk[r] = dV[kc];
             dvm[r] = dV[kw];
-            ge[r] = gam[clampIdx(i, j + 1, M, P)];
+            ge[r] = gam[clampIdx(i, 1, M, P)];  // read at j = 0 only: one address per row
         }
         double v[3] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -1885,7 +1885,7 @@ __global__ __launch_bounds__(kBlock) void k_correct(KArgs a) {
             pcu0[rr] = pc[ku];
             pcu1[rr] = pc[clampIdx(i, ju + 1, M, P)];
             usu[rr] = us[ku];
-            ge[rr] = gam[clampIdx(i, j + 1, M, P)];
+            ge[rr] = gam[clampIdx(i, 1, M, P)];  // read at j = 0 only: one address per row
             dNv[rr] = dN[kv];
             pcv0[rr] = pc[kv];
             pcv1[rr] = pc[clampIdx(i + 1, jv, M, P)];
