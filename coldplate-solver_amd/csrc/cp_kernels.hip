@@ -1349,7 +1349,74 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarsen(KArgs a) {
     }
 }
 
-constexpr int kGjCols = kCoarseMax / (kBlock / kCoarseMax);  // columns per thread (16)
+// Dense inverse of the coarsest operator (n <= kRows unknowns, 5-point on a
+// grid of width W, compact in LDS at offset oc) by Gauss-Jordan elimination
+// in double, by the whole block. Thread (row i = tid % kRows, column group
+// cg = tid / kRows) keeps A[i][cg + kGroups m], m < kCols, in registers, so
+// the block holds exactly kRows x kRows entries. Per pivot k the owners
+// publish column k and row k through double-buffered LDS (one barrier per
+// pivot); every thread then updates its entries. Entries outside n x n
+// start at 0 and stay 0.
+template <typename T, int kRows>
+__device__ void gjInverse(const T* sAP, const T* sCE, const T* sCN, int oc, int n, int W, gp<double> out) {
+    constexpr int kGroups = kBlock / kRows;
+    constexpr int kCols = kRows / kGroups;
+    static_assert(kRows * kRows == kBlock * kCols && kRows <= kCoarseMax, "kRows x kRows entries per block");
+    const int tid = int(threadIdx.x);
+    const int i = tid % kRows, cg = tid / kRows;
+    __shared__ double fbuf[2][kCoarseMax], rbuf[2][kCoarseMax];
+    double av[kCols];
+#pragma unroll
+    for (int m = 0; m < kCols; ++m) {
+        const int r = i, col = cg + kGroups * m;
+        double v = 0.0;
+        if (r < n && col < n) {
+            if (col == r) v = double(sAP[oc + r]);
+            else if (col == r + 1 && (r % W) + 1 < W) v = -double(sCE[oc + r]);
+            else if (col + 1 == r && (col % W) + 1 < W) v = -double(sCE[oc + col]);
+            else if (col == r + W) v = -double(sCN[oc + r]);
+            else if (col + W == r) v = -double(sCN[oc + col]);
+        }
+        av[m] = v;
+    }
+    for (int k = 0; k < n; ++k) {
+        const int b = k & 1;
+        if (cg == k % kGroups) {  // this column group holds column k
+            double ck = 0.0;
+#pragma unroll
+            for (int m = 0; m < kCols; ++m) ck = (cg + kGroups * m == k) ? av[m] : ck;
+            fbuf[b][i] = ck;
+        }
+        if (i == k) {
+#pragma unroll
+            for (int m = 0; m < kCols; ++m) rbuf[b][cg + kGroups * m] = av[m];
+        }
+        __syncthreads();
+        // every LDS read of the pivot step issued before any use: a read
+        // under the selects below would wait out its latency per column
+        double piv = fbuf[b][k];
+        const double fk = fbuf[b][i];
+        double rb[kCols];
+#pragma unroll
+        for (int m = 0; m < kCols; ++m) rb[m] = rbuf[b][cg + kGroups * m];
+        if (!(piv > 0.0)) piv = 1.0;
+        const double ip = 1.0 / piv;
+        const double fi = (i == k) ? 0.0 : fk;
+#pragma unroll
+        for (int m = 0; m < kCols; ++m) {
+            const int j = cg + kGroups * m;
+            const double rk = ((j == k) ? 1.0 : rb[m]) * ip;
+            av[m] = (i == k) ? rk : ((j == k) ? 0.0 : av[m]) - fi * rk;
+        }
+    }
+    if (i < n) {
+#pragma unroll
+        for (int m = 0; m < kCols; ++m) {
+            const int j = cg + kGroups * m;
+            if (j < n) out[size_t(i) * n + j] = av[m];
+        }
+    }
+}
 
 // In-LDS coarsening of the levels a.level .. nLevels-1 and the dense
 // inverse of the coarsest level for one slot (see k_mg_factor).
@@ -1357,7 +1424,7 @@ template <typename T>
 __device__ void mgFactorSlot(const KArgs& a, const SlotDesc& d) {
     constexpr int kCap = sizeof(T) == 4 ? kTailCapF32 : kTailCapF64;
     __shared__ T sAP[kCap], sCE[kCap], sCN[kCap];
-        const int tid = int(threadIdx.x);
+    const int tid = int(threadIdx.x);
     const int L = d.nLevels;
     const int src = min(max(a.level, 1) - 1, L - 1);  // first level read from global memory
     __shared__ int off[kMaxLevels + 1];  // compact offsets of levels src .. L-1 (read after the first barrier)
@@ -1426,63 +1493,9 @@ __device__ void mgFactorSlot(const KArgs& a, const SlotDesc& d) {
         }
         __syncthreads();
     }
-    // Gauss-Jordan: thread (row i = tid % 64, column group cg = tid / 64,
-    // uniform over a wave) keeps A[i][cg + 4m], m = 0 .. 15, in registers.
-    // Per pivot k the owners publish column k and row k through
-    // double-buffered LDS (one barrier per pivot); every thread then updates
-    // its entries. Columns past n are skipped per wave.
     const int W = d.lv[L - 1].W, n = d.lv[L - 1].H * W, oc = off[L - 1];
-    const int i = tid & (kCoarseMax - 1), cg = tid / kCoarseMax;
-    constexpr int kGroups = kBlock / kCoarseMax;  // 4
-    __shared__ double fbuf[2][kCoarseMax], rbuf[2][kCoarseMax];
-    double av[kGjCols];
-#pragma unroll
-    for (int m = 0; m < kGjCols; ++m) {
-        const int r = i, col = cg + kGroups * m;
-        double v = 0.0;
-        if (r < n && col < n) {
-            if (col == r) v = double(sAP[oc + r]);
-            else if (col == r + 1 && (r % W) + 1 < W) v = -double(sCE[oc + r]);
-            else if (col + 1 == r && (col % W) + 1 < W) v = -double(sCE[oc + col]);
-            else if (col == r + W) v = -double(sCN[oc + r]);
-            else if (col + W == r) v = -double(sCN[oc + col]);
-        }
-        av[m] = v;
-    }
-    const int mEnd = (n - cg + kGroups - 1) / kGroups;  // columns cg + 4m < n
-    for (int k = 0; k < n; ++k) {
-        const int b = k & 1;
-        if (cg == (k & (kGroups - 1))) {  // this wave holds column k
-            double ck = 0.0;
-#pragma unroll
-            for (int m = 0; m < kGjCols; ++m) ck = (cg + kGroups * m == k) ? av[m] : ck;
-            fbuf[b][i] = ck;
-        }
-        if (i == k) {
-#pragma unroll
-            for (int m = 0; m < kGjCols; ++m) rbuf[b][cg + kGroups * m] = av[m];
-        }
-        __syncthreads();
-        double piv = fbuf[b][k];
-        if (!(piv > 0.0)) piv = 1.0;
-        const double ip = 1.0 / piv;
-        const double fi = (i == k) ? 0.0 : fbuf[b][i];
-#pragma unroll
-        for (int m = 0; m < kGjCols; ++m) {
-            if (m < mEnd) {  // uniform over the wave
-                const int j = cg + kGroups * m;
-                const double rk = ((j == k) ? 1.0 : rbuf[b][j]) * ip;
-                av[m] = (i == k) ? rk : ((j == k) ? 0.0 : av[m]) - fi * rk;
-            }
-        }
-    }
-    if (i < n) {
-#pragma unroll
-        for (int m = 0; m < kGjCols; ++m) {
-            const int j = cg + kGroups * m;
-            if (j < n) G(d.coarseInv)[size_t(i) * n + j] = av[m];
-        }
-    }
+    if (n <= kCoarseMax / 2) gjInverse<T, kCoarseMax / 2>(sAP, sCE, sCN, oc, n, W, G(d.coarseInv));
+    else gjInverse<T, kCoarseMax>(sAP, sCE, sCN, oc, n, W, G(d.coarseInv));
 }
 
 
@@ -1602,6 +1615,7 @@ __global__ __launch_bounds__(kTailBlock) void k_mg_tail(KArgs a) {
     constexpr int kCap = sizeof(T) == 4 ? kTailCapF32 : kTailCapF64;
     __shared__ T sAP[kCap], sCE[kCap], sCN[kCap], sB[kCap], sX[kCap], sZ[kCap], sT[kCap];
     __shared__ double sc64[kCoarseMax];
+    __shared__ double sInvT[kCoarseMax * kCoarseMax];  // coarsest inverse, transposed
     const int s = blockIdx.x;
     __shared__ SlotDesc sd;
     const SlotDesc& d = descToLds(&sd, &a.desc[s]);
@@ -1621,9 +1635,15 @@ __global__ __launch_bounds__(kTailBlock) void k_mg_tail(KArgs a) {
         // side) with all loads issued before the LDS stores: element g of
         // the concatenated levels belongs to the level whose range holds it
         constexpr int kIt = (kCap + kTailBlock - 1) / kTailBlock;
+        constexpr int kInvIt = kCoarseMax * kCoarseMax / kTailBlock;
+        static_assert(kCoarseMax * kCoarseMax % kTailBlock == 0, "whole inverse loads per thread");
         const int total = off[L];
+        const int nc = d.lv[L - 1].H * d.lv[L - 1].W, nc2 = nc * nc;
         T la[kIt], le[kIt], ln[kIt], lb[kIt];
         int lvOf[kIt];
+        double li[kInvIt];  // the coarsest inverse, with the same batch of loads
+#pragma unroll
+        for (int it = 0; it < kInvIt; ++it) li[it] = G(d.coarseInv)[min(tid + it * kTailBlock, nc2 - 1)];
 #pragma unroll
         for (int it = 0; it < kIt; ++it) {
             const int g = min(tid + it * kTailBlock, total - 1);
@@ -1646,6 +1666,14 @@ __global__ __launch_bounds__(kTailBlock) void k_mg_tail(KArgs a) {
                 sCE[g] = le[it];
                 sCN[g] = ln[it];
                 if (lvOf[it] == L0) sB[g] = lb[it];
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < kInvIt; ++it) {
+            const int g = tid + it * kTailBlock;
+            if (g < nc2) {
+                const int r = g / nc, c = g - r * nc;
+                sInvT[c * nc + r] = li[it];
             }
         }
     }
@@ -1679,7 +1707,7 @@ __global__ __launch_bounds__(kTailBlock) void k_mg_tail(KArgs a) {
         __syncthreads();
         if (tid < n) {
             double acc = 0.0;
-            for (int k = 0; k < n; ++k) acc += G(d.coarseInv)[size_t(tid) * n + k] * sc64[k];
+            for (int k = 0; k < n; ++k) acc += sInvT[k * n + tid] * sc64[k];
             sZ[o + tid] = T(acc);
         }
         __syncthreads();
