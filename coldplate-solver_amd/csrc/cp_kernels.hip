@@ -1241,6 +1241,16 @@ __device__ __forceinline__ bool mgActive(const SlotDesc& d, const SlotState& st)
     return (pre != 0) & (status == ST_RUNNING) & (pdone == 0) & (phase == 0);
 }
 
+// The state fields mgActive tests, for the per-slot kernels: loaded before
+// the descriptor is staged, so both arrive in one round trip.
+struct MgGate {
+    int status, pdone, phase;
+};
+__device__ __forceinline__ MgGate mgGate(const SlotState& st) { return {st.status, st.pdone, st.stepPhase}; }
+__device__ __forceinline__ bool mgActive(const SlotDesc& d, const MgGate& g) {
+    return (d.precond != 0) & (g.status == ST_RUNNING) & (g.pdone == 0) & (g.phase == 0);
+}
+
 __device__ __forceinline__ size_t mgAt(const MgLevel& L, int I, int J) {
     return size_t(I + 1) * L.pitch + (J + 1);
 }
@@ -1510,9 +1520,9 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg_factor(KArgs a) {
     const int s = blockIdx.x;
     __shared__ SlotDesc sd;
+    const MgGate gate = mgGate(a.state[s]);
     const SlotDesc& d = descToLds(&sd, &a.desc[s]);
-    const SlotState& st = a.state[s];
-    if (d.ntiles == 0 || !mgActive(d, st)) return;
+    if (d.ntiles == 0 || !mgActive(d, gate)) return;
     mgFactorSlot<T>(a, d);
 }
 
@@ -1618,9 +1628,9 @@ __global__ __launch_bounds__(kTailBlock) void k_mg_tail(KArgs a) {
     __shared__ double sInvT[kCoarseMax * kCoarseMax];  // coarsest inverse, transposed
     const int s = blockIdx.x;
     __shared__ SlotDesc sd;
+    const MgGate gate = mgGate(a.state[s]);
     const SlotDesc& d = descToLds(&sd, &a.desc[s]);
-    const SlotState& st = a.state[s];
-    if (d.ntiles == 0 || !mgActive(d, st)) return;
+    if (d.ntiles == 0 || !mgActive(d, gate)) return;
     const int L0 = d.mgTail, L = d.nLevels;
     const T om = T(d.mgOmega), sc = T(d.mgScale);
     const int tid = int(threadIdx.x);
@@ -2132,20 +2142,36 @@ __global__ __launch_bounds__(kBlock) void k_dp_monitor(KArgs a) {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int outer) {
     const int s = blockIdx.x;
+    // The slot's descriptor and state are read into LDS in one round trip
+    // (every load issued before the LDS stores); the state is updated there
+    // by thread 0 and written back whole, so its fields cost no dependent
+    // global round trips. Nothing else touches the state while this kernel
+    // runs.
     __shared__ SlotDesc sd;
-    const SlotDesc& d = descToLds(&sd, &a.desc[s]);
-    // The slot's state is read into LDS in one step (a word per thread),
-    // updated there by thread 0 and written back whole, so its fields cost
-    // no dependent global round trips. Nothing else touches the state while
-    // this kernel runs.
     __shared__ SlotState sst;
     constexpr int kWords = int(sizeof(SlotState) / sizeof(uint32_t));
+    constexpr int kDescWords = int(sizeof(SlotDesc) / sizeof(uint64_t));
+    constexpr int kDescIt = (kDescWords + kBlock - 1) / kBlock;
     static_assert(kWords <= kBlock, "one word per thread");
+    static_assert(sizeof(SlotDesc) % sizeof(uint64_t) == 0, "descriptor size");
     const auto gw = G(reinterpret_cast<uint32_t*>(&a.state[s]));
     uint32_t* lw = reinterpret_cast<uint32_t*>(&sst);
-    if (int(threadIdx.x) < kWords) lw[threadIdx.x] = gw[threadIdx.x];
-    const int ntiles = d.ntiles;
+    {
+        const int tid = int(threadIdx.x);
+        const auto gs = G(reinterpret_cast<const uint64_t*>(&a.desc[s]));
+        uint64_t* ld = reinterpret_cast<uint64_t*>(&sd);
+        uint64_t dv[kDescIt];
+#pragma unroll
+        for (int it = 0; it < kDescIt; ++it) dv[it] = gs[min(tid + it * kBlock, kDescWords - 1)];
+        const uint32_t sv = gw[min(tid, kWords - 1)];
+#pragma unroll
+        for (int it = 0; it < kDescIt; ++it)
+            if (tid + it * kBlock < kDescWords) ld[tid + it * kBlock] = dv[it];
+        if (tid < kWords) lw[tid] = sv;
+    }
     __syncthreads();
+    const SlotDesc& d = sd;
+    const int ntiles = d.ntiles;
     SlotState& st = sst;
     if (ntiles == 0 || st.status != ST_RUNNING) return;
     switch (phase) {
