@@ -297,12 +297,6 @@ __device__ __forceinline__ double souCorrection(Ptr f, size_t k, int i, int j, i
     return S;
 }
 
-// Assembles one U (IsU) or V row at node k. Writes the relaxed diagonal, the
-// four neighbour coefficients, dE/dN, the initial residual r0 = b - A u of
-// the Jacobi system (so the sweeps iterate on the correction delta = x - u
-// from delta = 0, which is the reference's x0 = u), and delta = 0.
-// u, v, p are read from the workgroup's LDS tile (index ks, row pitch P);
-// k indexes the node's global arrays.
 // Descriptor fields of the momentum rows, copied to registers once per tile
 // (read through the descriptor, they would be reloaded after every store,
 // since the compiler cannot rule out aliasing).
@@ -325,11 +319,16 @@ struct MomConst {
     }
 };
 
-// One momentum row; returns the stored residual r0 = b - A u. With `sweep`
-// it also makes the first Jacobi sweep: from delta = 0 the sweep's residual
-// is r0 and its iterate 0 + r0 / aP, in the arithmetic of k_jacobi (T, the
-// stored aP), written to the second delta buffer. Without it (no sweeps)
-// the first buffer, which the momentum finish then reads, is zeroed.
+// Assembles one U (IsU) or V row at node k: writes the relaxed diagonal, the
+// four neighbour coefficients, dE/dN and the initial residual r0 = b - A u
+// of the Jacobi system (the sweeps iterate on the correction delta = x - u
+// from delta = 0, which is the reference's x0 = u), and returns r0 as
+// stored. With `sweep` it also makes the first Jacobi sweep: from delta = 0
+// the sweep's residual is r0 and its iterate 0 + r0 / aP, in the arithmetic
+// of k_jacobi (T, the stored aP), written to the second delta buffer.
+// Without it (no sweeps) the first buffer, which the momentum finish then
+// reads, is zeroed. u, v, p are read from the workgroup's LDS tile (index
+// ks, row pitch P); k indexes the node's global arrays.
 template <typename T, bool IsU>
 __device__ __forceinline__ T momRow(const MomConst<T>& d, const double* u, const double* v, const double* pp,
                                     size_t ks, int P, size_t k, int i, int j, double maxVel, double cfl,
