@@ -154,7 +154,11 @@ enum Phase {
     PH_JAC = 0, PH_MOM_FIN, PH_PRES_ASM, PH_PCG_Q, PH_PCG_U, PH_REF_ACC, PH_REF_RES, PH_CORR, PH_MG_Z,
     PH_MOM_EXPL,  // explicit momentum: residuals
     PH_SOR,       // SOR: stop test after a red + black sweep pair
+    PH_PCGU_MGZ,  // PH_PCG_U's stop test, then PH_MG_Z, after the V-cycle that follows the update
 };
+// Partial entry of the finest up-sweep's r.z: apart from the PCG update's
+// entries 0 and 1, which PH_PCGU_MGZ reads after the V-cycle.
+constexpr int kMgZPartial = 2;
 
 // Kernels of one SIMPLE step, in launch order.
 enum Kern {
@@ -189,6 +193,8 @@ hipError_t launchKernel(Kern k, bool f32, int rows, int grid, int nslots, hipStr
 // The work kernel alone, without its finalisation (timing bursts of an
 // idempotent kernel).
 hipError_t launchWorkOnly(Kern k, bool f32, int rows, int grid, hipStream_t s, const KArgs& a);
+// The finalisation kernel alone (one workgroup per slot) for `phase`.
+hipError_t launchFinalize(int phase, int nslots, hipStream_t s, const KArgs& a);
 // Rows per wave of the finest multigrid level's tiles for a batch tile
 // height (coarser levels use 2). Even, so that a wave's rows hold whole 2 x 2
 // aggregates; 4 only in single precision (register footprint).

@@ -309,7 +309,9 @@ int mgGrid(const cp_batch* b, int level) {
     return gridFor(b, 1 << 30);
 }
 
-int launchMg(cp_batch* b, Kern k, int level) {
+// finalPhase >= 0: launch the work kernel alone, then the finalisation of
+// that phase instead of the kernel's own.
+int launchMg(cp_batch* b, Kern k, int level, int finalPhase = -1) {
     const bool f32 = b->precision == CP_PREC_MIXED;
     const bool perSlot = k == K_MG_FACTOR || k == K_MG_TAIL;
     KArgs a = kargs(b);
@@ -318,7 +320,13 @@ int launchMg(cp_batch* b, Kern k, int level) {
     a.level = level;
     const int rows = level == 0 ? b->mgRowsEff() : 2;
     const int grid = perSlot ? b->maxSlots : mgGrid(b, level);
-    const hipError_t e = launchKernel(k, f32, rows, grid, b->maxSlots, b->stream, a);
+    hipError_t e;
+    if (finalPhase < 0) {
+        e = launchKernel(k, f32, rows, grid, b->maxSlots, b->stream, a);
+    } else {
+        e = launchWorkOnly(k, f32, rows, grid, b->stream, a);
+        if (e == hipSuccess) e = launchFinalize(finalPhase, b->maxSlots, b->stream, kargs(b));
+    }
     if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
     return CP_OK;
 }
@@ -337,7 +345,10 @@ int mgSetup(cp_batch* b) {
 // z = V-cycle(r) for every multigrid slot whose CG is running, then r.z:
 // per-level launches above the tails, one tail launch, per-level launches
 // back up.
-int mgVcycle(cp_batch* b) {
+// afterUpdate: the V-cycle follows a PCG update launched without its
+// finalisation; the finest up-sweep is then finalised by PH_PCGU_MGZ, which
+// applies the update's stop test first (one launch fewer per iteration).
+int mgVcycle(cp_batch* b, bool afterUpdate = false) {
     if (b->mgLevels == 0) return CP_OK;
     const int T0 = b->mgTail;
     int rc;
@@ -345,7 +356,7 @@ int mgVcycle(cp_batch* b) {
         if ((rc = launchMg(b, K_MG_DOWN, l))) return rc;
     if ((rc = launchMg(b, K_MG_TAIL, 0))) return rc;
     for (int l = T0 - 1; l >= 0; --l)
-        if ((rc = launchMg(b, K_MG_UP, l))) return rc;
+        if ((rc = launchMg(b, K_MG_UP, l, (l == 0 && afterUpdate) ? int(PH_PCGU_MGZ) : -1))) return rc;
     return CP_OK;
 }
 
@@ -987,10 +998,18 @@ int enqueueFixedStep(cp_batch* b, int J, int P, int R, int grid) {
     for (int c = 0; c < R; ++c) {
         for (int k = 0; k < P; ++k) {
             if ((rc = launch(b, K_PCG_Q, grid))) return rc;
-            if ((rc = launch(b, K_PCG_U, grid))) return rc;
             // the V-cycle after the last iteration would only feed a search
-            // direction that is never used
-            if (k + 1 < P && (rc = mgVcycle(b))) return rc;
+            // direction that is never used; before it, the update's stop
+            // test is finalised together with the V-cycle's r.z
+            if (k + 1 < P && b->mgLevels > 0) {
+                const hipError_t e = launchWorkOnly(K_PCG_U, b->precision == CP_PREC_MIXED,
+                                                    pRows(b->precision == CP_PREC_MIXED, b->tileRows), grid, b->stream,
+                                                    kargsFor(b, K_PCG_U));
+                if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+                if ((rc = mgVcycle(b, true))) return rc;
+            } else if ((rc = launch(b, K_PCG_U, grid))) {
+                return rc;
+            }
         }
         if (b->precision != CP_PREC_MIXED) break;
         if ((rc = launch(b, K_REF_ACC, grid))) return rc;

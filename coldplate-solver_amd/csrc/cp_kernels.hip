@@ -1814,7 +1814,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(KArgs a) {
                 v[0] += double(b[h]) * double(z);
             }
         }
-        if (a.level == 0) blockReduceTo<1, 0u>(v, a.partials + size_t(t) * kNumPartials, nred);
+        if (a.level == 0) blockReduceTo<1, 0u>(v, a.partials + size_t(t) * kNumPartials + kMgZPartial, nred);
     }
 }
 
@@ -2139,6 +2139,21 @@ __global__ __launch_bounds__(kBlock) void k_dp_monitor(KArgs a) {
 // scalars. Launched right after the work kernel of `phase`; it applies the
 // same skip rule as that kernel, so it only reads partials written by it.
 // ---------------------------------------------------------------------------
+// r.z of the V-cycle just applied: the start of a (re)started CG (pk == 0)
+// or the next search direction's beta. Called by the whole block.
+__device__ __forceinline__ void mgZFinalize(const KArgs& a, const SlotDesc& d, SlotState& st) {
+    double tot[1];
+    slotReduce<1, 0u>(a.partials + kMgZPartial, d.lv[0].tile0, d.lv[0].ntiles, tot);
+    if (threadIdx.x == 0) {
+        if (st.pk == 0) {
+            st.beta = 0.0;
+        } else {
+            st.beta = tot[0] / st.rz;
+        }
+        st.rz = tot[0];
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int outer) {
     const int s = blockIdx.x;
     // The slot's descriptor and state are read into LDS in one round trip
@@ -2182,6 +2197,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
             [[fallthrough]];
         case PH_PCG_Q:
         case PH_PCG_U:
+        case PH_PCGU_MGZ:
             if (st.pdone || st.stepPhase) return;
             break;
         case PH_REF_ACC:
@@ -2274,7 +2290,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
             }
             break;
         }
-        case PH_PCG_U: {
+        case PH_PCG_U:
+        case PH_PCGU_MGZ: {
             double tot[2];
             slotReduce<2, 0u>(a.partials, d.ptile0, d.pntiles, tot);
             if (threadIdx.x == 0) {
@@ -2294,6 +2311,12 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
                     }
                 }
             }
+            if (phase == PH_PCG_U) break;
+            // the V-cycle ran after the update whatever the stop test said;
+            // its z only counts for a multigrid slot whose CG goes on
+            __syncthreads();
+            if (st.pdone || !d.precond) break;
+            mgZFinalize(a, d, st);
             break;
         }
         case PH_REF_ACC: {
@@ -2320,21 +2343,9 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
             }
             break;
         }
-        case PH_MG_Z: {
-            // r.z of the V-cycle just applied: the start of a (re)started CG
-            // (pk == 0) or the next search direction's beta
-            double tot[1];
-            slotReduce<1, 0u>(a.partials, d.lv[0].tile0, d.lv[0].ntiles, tot);
-            if (threadIdx.x == 0) {
-                if (st.pk == 0) {
-                    st.beta = 0.0;
-                } else {
-                    st.beta = tot[0] / st.rz;
-                }
-                st.rz = tot[0];
-            }
+        case PH_MG_Z:
+            mgZFinalize(a, d, st);
             break;
-        }
         case PH_CORR: {
             double tot[1];
             slotReduce<1, 1u>(a.partials, d.tile0, d.ntiles, tot);
@@ -2441,6 +2452,11 @@ hipError_t launchKernel(Kern k, bool f32, int rows, int grid, int nslots, hipStr
 hipError_t launchWorkOnly(Kern k, bool f32, int rows, int grid, hipStream_t s, const KArgs& a) {
     if (f32) launchT<float>(k, rows, grid, s, a);
     else launchT<double>(k, rows, grid, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launchFinalize(int phase, int nslots, hipStream_t s, const KArgs& a) {
+    hipLaunchKernelGGL(k_finalize, dim3(nslots), dim3(kBlock), 0, s, a, phase, 0);
     return hipGetLastError();
 }
 
