@@ -70,7 +70,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=8, help="slots of the request loop")
+    ap.add_argument("--batch", type=int, default=8, help="slots of the request loop (per stream)")
+    ap.add_argument("--streams", type=int, default=3,
+                    help="batches served at once (cp_serve_pool), each on its own HIP stream and host thread")
     ap.add_argument("--requests", type=int, default=24, help="solve requests per step")
     ap.add_argument("--eps", type=float, default=1e-7)
     ap.add_argument("--jacobi", type=int, default=3, help="Jacobi sweeps per momentum solve")
@@ -117,8 +119,10 @@ def n_pressure(M: int, N: int) -> int:
     return (M - 1) * (N - 2)
 
 
-def solve_stream(b: cb.Batch, a, probe: bool):
-    """Serve one stream of requests; returns per-request results."""
+def solve_stream(bs: list, a, probe: bool):
+    """Serve one stream of requests over the batches `bs` (one: cp_serve;
+    several: cp_serve_pool, one HIP stream each); returns per-request
+    results. Timing bursts run on the first batch, with the GPU to itself."""
     reqs = []
     for k in range(a.requests):
         p, ct = GEOMS[k % len(GEOMS)]
@@ -129,11 +133,12 @@ def solve_stream(b: cb.Batch, a, probe: bool):
         if a.profile_steps:
             q.maxIterations = a.profile_steps
         reqs.append((q, ct, 0.0, GAMMA_MIN))
-    b.probe(cb.K_PCG_Q if probe else -1)
+    for k, b in enumerate(bs):
+        b.probe(cb.K_PCG_Q if (probe and k == 0) else -1)
     t0 = time.perf_counter()
-    res = b.serve(reqs, a.check)
+    res = bs[0].serve(reqs, a.check) if len(bs) == 1 else cb.serve_pool(bs, reqs, a.check)
     elapsed = time.perf_counter() - t0
-    ms, unknowns = b.probe_read()
+    ms, unknowns = bs[0].probe_read()
     return dict(elapsed=elapsed, res=res, seeds=[k % len(GEOMS) for k in range(a.requests)], ms=ms,
                 unknowns=unknowns)
 
@@ -343,31 +348,37 @@ def main() -> int:
     probe_b = cb.Batch(cb.CP_PREC_MIXED, 1 << 20, 1)
     need = B * max(probe_b.footprint(p.M, p.N) for p, _ in GEOMS)
     probe_b.close()
-    b = cb.Batch(cb.CP_PREC_MIXED, need + (64 << 20), B)
+    if a.streams < 1:
+        raise SystemExit("--streams must be at least 1")
+    bs = [cb.Batch(cb.CP_PREC_MIXED, need + (64 << 20), B) for _ in range(a.streams)]
+    b = bs[0]
     if a.no_graph:
-        b.set_graph(False)
+        for x in bs:
+            x.set_graph(False)
     tuned = None
     if a.tiling:
         r_, m_ = (int(x) for x in a.tiling.split(","))
-        b.set_tiling(r_, 16)
-        b.set_mg_tiling(m_)
         tuned = {"tile_rows": r_, "mg_rows": m_, "wg_per_cu": 16, "fixed": True}
     elif not a.no_tune:
-        # autotune the tiling on the workload's own geometries, then start
-        # every measured stream from an empty batch
+        # autotune the tiling on the workload's own geometries (one batch
+        # alone), then start every measured stream from empty batches
         q0 = [served_params(p, a, a.eps) for p, _ in GEOMS]
         slots = [b.admit(q, ct) for q, (_, ct) in zip(q0[:B], GEOMS[:B])]
         tuned = b.autotune(2)
         for s_ in slots:
             b.evict(s_)
         log(f"autotuned tiling: {tuned}")
+    if tuned:
+        for x in bs:
+            x.set_tiling(tuned["tile_rows"], tuned["wg_per_cu"])
+            x.set_mg_tiling(tuned["mg_rows"])
     for w in range(a.warmup):
-        r = solve_stream(b, a, probe=False)
+        r = solve_stream(bs, a, probe=False)
         log(f"warmup {w}: {r['elapsed']:.2f}s iters {[x['iter'] for x in r['res']]}")
     runs = []
     t0 = time.perf_counter()
     for k in range(a.steps):
-        r = solve_stream(b, a, probe=not a.no_probe)
+        r = solve_stream(bs, a, probe=not a.no_probe)
         runs.append(r)
         log(f"step {k}: {r['elapsed']:.2f}s iters {[x['iter'] for x in r['res']]} max dp err "
             f"{max(abs(x['dp'] / refs[sd] - 1) for x, sd in zip(r['res'], r['seeds'])):.2e}")
@@ -424,12 +435,15 @@ def main() -> int:
         "config": {
             "workload": "network_1024x512",
             "requests_per_step": a.requests,
-            "slots": B,
+            "streams": a.streams,
+            "slots_per_stream": B,
             "grid": "1024x512 cells",
             "stop_test": f"max(residU,residV,residMass[p' cols]) <= {a.eps:g} and dp err <= {DP_TOL:g}",
             "inner_budgets": {"jacobi_sweeps": a.jacobi, "cg_per_cycle": a.pcg, "fp64_refine_cycles": a.cycles},
             "pressure_preconditioner": "aggregation-multigrid V-cycle" if a.precond == "amg" else "Jacobi",
-            "serving": "cp_serve continuous batching, all requests arrived at t=0",
+            "serving": ("cp_serve_pool: %d batches on their own streams, one first-come-first-served queue, "
+                        "continuous batching, all requests arrived at t=0" % a.streams) if a.streams > 1 else
+                       "cp_serve continuous batching, all requests arrived at t=0",
             "tiling": tuned if tuned else "default (no autotuning)",
             "precision": "mixed: fp32 stencil/jacobi/cg/v-cycle, fp64 residuals/refinement/reductions",
         },

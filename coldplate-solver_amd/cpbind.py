@@ -180,6 +180,7 @@ EXPORTS = [
     ("cp_batch_set_graph", C.c_int, [_vp, C.c_int]),
     ("cp_batch_set_mg_tiling", C.c_int, [_vp, C.c_int]),
     ("cp_serve", C.c_int, [_vp, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    ("cp_serve_pool", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     ("cp_batch_autotune", C.c_int, [_vp, C.c_int, _ip, _ip, _ip, _dp]),
     ("cp_io_read_params", C.c_int, [C.c_char_p, C.POINTER(Params), _ip, _ip]),
     ("cp_io_read_geometry", C.c_int, [C.c_char_p, C.c_int, C.c_int, _dp]),
@@ -316,6 +317,36 @@ class Solver:
         return a["avgTotal"] - b["avgTotal"]
 
 
+def _request_array(requests):
+    """cp_request array of [(params, cellType, arrival_s, gamma_min), ...],
+    with the Python objects it points into (keep them alive for the call)."""
+    n = len(requests)
+    keep = []
+    reqs = (Request * max(n, 1))()
+    for k, (prm, ct, arrival, gmin) in enumerate(requests):
+        pr = prm.copy()
+        ca = np.ascontiguousarray(ct, dtype=np.float64)
+        keep.append((pr, ca))
+        reqs[k].prm = C.pointer(pr)
+        reqs[k].cellType = ca.ctypes.data_as(C.POINTER(C.c_double))
+        reqs[k].arrival = float(arrival)
+        reqs[k].gammaMin = float(gmin)
+    return reqs, keep, n
+
+
+def serve_pool(batches, requests, steps_per_poll: int = 20) -> list[dict]:
+    """cp_serve_pool: `requests` served by several empty batches at once, one
+    host thread and HIP stream each, from one first-come-first-served queue;
+    one result dict per request, in order."""
+    reqs, keep, n = _request_array(requests)
+    hs = (C.c_void_p * max(len(batches), 1))(*[b.h for b in batches])
+    res = (Result * max(n, 1))()
+    check(load().cp_serve_pool(C.cast(hs, C.c_void_p), len(batches), C.cast(reqs, C.c_void_p), n, steps_per_poll,
+                               C.cast(res, C.c_void_p)))
+    del keep
+    return [{f: getattr(res[k], f) for f, _ in Result._fields_} for k in range(n)]
+
+
 class Batch:
     """Packed multi-geometry device solve (cp_batch)."""
 
@@ -384,19 +415,10 @@ class Batch:
     def serve(self, requests, steps_per_poll: int = 20) -> list[dict]:
         """Continuous batching over `requests` = [(params, cellType, arrival_s,
         gamma_min), ...]; returns one result dict per request, in order."""
-        n = len(requests)
-        keep = []
-        reqs = (Request * max(n, 1))()
-        for k, (prm, ct, arrival, gmin) in enumerate(requests):
-            pr = prm.copy()
-            ca = np.ascontiguousarray(ct, dtype=np.float64)
-            keep.append((pr, ca))
-            reqs[k].prm = C.pointer(pr)
-            reqs[k].cellType = ca.ctypes.data_as(C.POINTER(C.c_double))
-            reqs[k].arrival = float(arrival)
-            reqs[k].gammaMin = float(gmin)
+        reqs, keep, n = _request_array(requests)
         res = (Result * max(n, 1))()
         check(load().cp_serve(self.h, C.cast(reqs, C.c_void_p), n, steps_per_poll, C.cast(res, C.c_void_p)), self.h)
+        del keep
         return [{f: getattr(res[k], f) for f, _ in Result._fields_} for k in range(n)]
 
     def autotune(self, steps: int = 2) -> dict:

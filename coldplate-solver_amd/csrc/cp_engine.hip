@@ -17,8 +17,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -1273,90 +1275,197 @@ namespace {
 int burst(cp_batch* b, int launches, double* ms_per_launch);
 }
 
-int cp_serve(cp_batch* b, const cp_request* reqs, int n, int steps_per_poll, cp_result* results) {
-    if (!b) return setError(nullptr, CP_E_ARG, "null batch");
-    std::string& err = b->err;
-    if (n < 0 || (n > 0 && (!reqs || !results))) return setError(&err, CP_E_ARG, "null requests or results");
-    if (steps_per_poll < 1) return setError(&err, CP_E_ARG, "steps_per_poll must be >= 1");
-    for (int s = 0; s < b->maxSlots; ++s)
-        if (b->slots[s].used) return setError(&err, CP_E_STATE, "cp_serve needs an empty batch");
-    std::vector<int> order(n);
-    for (int k = 0; k < n; ++k) order[k] = k;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return reqs[x].arrival < reqs[y].arrival; });
-    for (int k = 0; k < n; ++k) {
-        results[k] = cp_result();
-        results[k].slot = -1;
-        results[k].status = CP_SLOT_FAILED;
+}  // extern "C"
+
+namespace {
+
+// The request queue of one cp_serve / cp_serve_pool call, shared by the
+// batches that serve it (one host thread each, each batch on its own
+// stream). Requests are taken first come, first served by whichever batch
+// has room. The GPU gate lets a timing burst run with no other batch's work
+// in flight: every worker holds it shared while it has device work queued
+// (admission, step graph, poll, read-out), a burst holds it exclusively.
+struct ServeQueue {
+    const cp_request* reqs = nullptr;
+    cp_result* results = nullptr;
+    int n = 0;
+    std::vector<int> order;  // request indices by arrival
+    std::mutex mu;           // admissions: next
+    int next = 0;
+    std::atomic<bool> abort{false};
+    std::chrono::steady_clock::time_point t0;
+    std::mutex gmu;
+    std::condition_variable gcv;
+    int active = 0;          // workers inside the shared gate
+    bool exclusive = false;  // a burst waits for, or holds, the GPU alone
+
+    double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+    void enterShared() {
+        std::unique_lock<std::mutex> lk(gmu);
+        gcv.wait(lk, [&] { return !exclusive; });
+        ++active;
     }
+    void leaveShared() {
+        std::lock_guard<std::mutex> lk(gmu);
+        --active;
+        gcv.notify_all();
+    }
+    void enterExclusive() {
+        std::unique_lock<std::mutex> lk(gmu);
+        gcv.wait(lk, [&] { return !exclusive; });
+        exclusive = true;
+        gcv.wait(lk, [&] { return active == 0; });
+    }
+    void leaveExclusive() {
+        std::lock_guard<std::mutex> lk(gmu);
+        exclusive = false;
+        gcv.notify_all();
+    }
+};
+
+// One batch's serve loop over the shared queue. Returns CP_OK when no
+// request is left to admit and the batch has finished its own, or the first
+// error (which stops the other workers at their next iteration).
+int serveWorker(cp_batch* b, ServeQueue& q, int steps_per_poll) {
+    if (hipSetDevice(b->device) != hipSuccess) return setError(&b->err, CP_E_HIP, "hipSetDevice failed");
     std::vector<int> reqOf(b->maxSlots, -1);
     b->probeMs.clear();
     b->probeUnknowns.clear();
     long polls = 0;
-    const auto t0 = std::chrono::steady_clock::now();
-    auto now = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
-    int next = 0, done = 0, running = 0;
-    while (done < n) {
-        // admissions, first come first served
-        while (next < n && reqs[order[next]].arrival <= now()) {
-            const int q = order[next];
-            int slot = -1;
-            int rc = batchAdmit(b, reqs[q].prm, reqs[q].cellType, &slot);
-            if (rc == CP_E_CAPACITY && running > 0) break;  // wait for an eviction
-            ++next;
-            if (rc) {  // invalid, or larger than the whole batch
-                results[q].rc = rc;
-                results[q].tAdmit = results[q].tDone = now();
-                ++done;
-                continue;
+    int running = 0;
+    while (!q.abort.load()) {
+        q.enterShared();
+        int rc = CP_OK;
+        {  // admissions into this batch, first come first served
+            std::lock_guard<std::mutex> lk(q.mu);
+            while (q.next < q.n && q.reqs[q.order[q.next]].arrival <= q.now()) {
+                const int r = q.order[q.next];
+                int slot = -1;
+                const int arc = batchAdmit(b, q.reqs[r].prm, q.reqs[r].cellType, &slot);
+                if (arc == CP_E_CAPACITY && running > 0) break;  // wait for an eviction here, or another batch
+                ++q.next;
+                if (arc) {  // invalid, or larger than the whole batch
+                    q.results[r].rc = arc;
+                    q.results[r].tAdmit = q.results[r].tDone = q.now();
+                    continue;
+                }
+                q.results[r].slot = slot;
+                q.results[r].tAdmit = q.now();
+                reqOf[slot] = r;
+                ++running;
             }
-            results[q].slot = slot;
-            results[q].tAdmit = now();
-            reqOf[slot] = q;
-            ++running;
         }
         if (running == 0) {
-            if (next < n) {
-                const double wait = reqs[order[next]].arrival - now();
-                if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.05)));
+            q.leaveShared();
+            double wait = 0.0;
+            {
+                std::lock_guard<std::mutex> lk(q.mu);
+                if (q.next >= q.n) return CP_OK;  // nothing left for this batch
+                wait = q.reqs[q.order[q.next]].arrival - q.now();
             }
+            if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.05)));
             continue;
         }
-        int rc = graphRun(b, steps_per_poll);
+        rc = graphRun(b, steps_per_poll);
         if (!rc) rc = pollState(b);
-        if (rc) return rc;
-        if (b->probeKernel == int(K_PCG_Q) && polls++ % kProbeEvery == 0) {
-            // timing burst of the PCG stencil kernel over the running slots
+        for (int s = 0; !rc && s < b->maxSlots; ++s) {
+            const int r = reqOf[s];
+            if (r < 0 || b->hState[s].status == ST_RUNNING) continue;
+            cp_result& res = q.results[r];
+            const SlotState& st = b->hState[s];
+            res.status = st.status;
+            res.iter = st.iter;
+            res.error = st.error;
+            const double xOut = double(b->slots[s].N) * b->slots[s].prm.hx;
+            cp_plane_metrics in, out;
+            if ((rc = planeSample(b, s, 0.0, q.reqs[r].gammaMin, &in))) break;
+            if ((rc = planeSample(b, s, xOut, q.reqs[r].gammaMin, &out))) break;
+            res.dp = in.avgTotal - out.avgTotal;
+            res.tDone = q.now();
+            if ((rc = batchEvict(b, s))) break;
+            reqOf[s] = -1;
+            --running;
+        }
+        q.leaveShared();
+        if (!rc && b->probeKernel == int(K_PCG_Q) && polls++ % kProbeEvery == 0) {
+            // timing burst of the PCG stencil kernel over the running slots,
+            // with the GPU to itself
             double ms = 0.0, unknowns = 0.0;
             for (int s = 0; s < b->maxSlots; ++s)
                 if (reqOf[s] >= 0 && b->hState[s].status == ST_RUNNING)
                     unknowns += double(b->slots[s].M - 1) * double(b->slots[s].N - 2);
             if (unknowns > 0) {
-                if ((rc = burst(b, kProbeLaunches, &ms))) return rc;
-                b->probeMs.push_back(ms);
-                b->probeUnknowns.push_back(unknowns);
+                q.enterExclusive();
+                rc = burst(b, kProbeLaunches, &ms);
+                q.leaveExclusive();
+                if (!rc) {
+                    b->probeMs.push_back(ms);
+                    b->probeUnknowns.push_back(unknowns);
+                }
             }
         }
-        for (int s = 0; s < b->maxSlots; ++s) {
-            const int q = reqOf[s];
-            if (q < 0 || b->hState[s].status == ST_RUNNING) continue;
-            cp_result& r = results[q];
-            const SlotState& st = b->hState[s];
-            r.status = st.status;
-            r.iter = st.iter;
-            r.error = st.error;
-            const double xOut = double(b->slots[s].N) * b->slots[s].prm.hx;
-            cp_plane_metrics in, out;
-            if ((rc = planeSample(b, s, 0.0, reqs[q].gammaMin, &in))) return rc;
-            if ((rc = planeSample(b, s, xOut, reqs[q].gammaMin, &out))) return rc;
-            r.dp = in.avgTotal - out.avgTotal;
-            r.tDone = now();
-            if ((rc = batchEvict(b, s))) return rc;
-            reqOf[s] = -1;
-            --running;
-            ++done;
+        if (rc) {
+            q.abort.store(true);
+            return rc;
         }
     }
+    return setError(&b->err, CP_E_STATE, "serve stopped: another batch of the pool failed");
+}
+
+int serveChecked(cp_batch* const* bs, int nb, const cp_request* reqs, int n, int steps_per_poll, cp_result* results) {
+    for (int i = 0; i < nb; ++i) {
+        std::string& err = bs[i]->err;
+        if (n < 0 || (n > 0 && (!reqs || !results))) return setError(&err, CP_E_ARG, "null requests or results");
+        if (steps_per_poll < 1) return setError(&err, CP_E_ARG, "steps_per_poll must be >= 1");
+        for (int s = 0; s < bs[i]->maxSlots; ++s)
+            if (bs[i]->slots[s].used) return setError(&err, CP_E_STATE, "cp_serve needs an empty batch");
+    }
+    ServeQueue q;
+    q.reqs = reqs;
+    q.results = results;
+    q.n = n;
+    q.order.resize(n);
+    for (int k = 0; k < n; ++k) q.order[k] = k;
+    std::stable_sort(q.order.begin(), q.order.end(), [&](int x, int y) { return reqs[x].arrival < reqs[y].arrival; });
+    for (int k = 0; k < n; ++k) {
+        results[k] = cp_result();
+        results[k].slot = -1;
+        results[k].status = CP_SLOT_FAILED;
+    }
+    q.t0 = std::chrono::steady_clock::now();
+    std::vector<int> rcs(nb, CP_OK);
+    std::vector<std::thread> th;
+    for (int i = 1; i < nb; ++i) th.emplace_back([&, i] { rcs[i] = serveWorker(bs[i], q, steps_per_poll); });
+    rcs[0] = serveWorker(bs[0], q, steps_per_poll);
+    for (auto& t : th) t.join();
+    // the first failure, not a worker stopped by it
+    for (int i = 0; i < nb; ++i)
+        if (rcs[i] != CP_OK && rcs[i] != CP_E_STATE) return rcs[i];
+    for (int i = 0; i < nb; ++i)
+        if (rcs[i] != CP_OK) return rcs[i];
     return CP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cp_serve(cp_batch* b, const cp_request* reqs, int n, int steps_per_poll, cp_result* results) {
+    if (!b) return setError(nullptr, CP_E_ARG, "null batch");
+    return serveChecked(&b, 1, reqs, n, steps_per_poll, results);
+}
+
+int cp_serve_pool(cp_batch* const* batches, int nbatches, const cp_request* reqs, int n, int steps_per_poll,
+                  cp_result* results) {
+    if (!batches || nbatches < 1) return setError(nullptr, CP_E_ARG, "no batches");
+    for (int i = 0; i < nbatches; ++i) {
+        if (!batches[i]) return setError(nullptr, CP_E_ARG, "null batch %d", i);
+        for (int j = 0; j < i; ++j)
+            if (batches[j] == batches[i]) return setError(nullptr, CP_E_ARG, "batch %d given twice", i);
+        if (batches[i]->precision != batches[0]->precision)
+            return setError(nullptr, CP_E_ARG, "batches of different precision");
+    }
+    return serveChecked(batches, nbatches, reqs, n, steps_per_poll, results);
 }
 
 int cp_batch_set_mg_tiling(cp_batch* b, int rows) {

@@ -372,6 +372,19 @@ typedef struct cp_result {
    Returns CP_OK once every request has finished. */
 int cp_serve(cp_batch* b, const cp_request* reqs, int n, int steps_per_poll, cp_result* results);
 
+/* cp_serve over several empty batches of one precision at once, each on its
+   own HIP stream and served by its own host thread (batches[0] by the
+   calling thread), all taking requests from one queue, first come first
+   served, into whichever batch has room. No reference counterpart (the
+   reference solves one geometry per process). Kernels of different streams
+   run concurrently, so one batch's latency-bound launches (per-slot
+   finalisations, coarse multigrid levels) overlap another's bandwidth-bound
+   ones. Timing bursts (cp_batch_probe) run with no other batch's work in
+   flight. Times in results are seconds after the start of the call.
+   Returns the first error any batch met (the others stop). */
+int cp_serve_pool(cp_batch* const* batches, int nbatches, const cp_request* reqs, int n, int steps_per_poll,
+                  cp_result* results);
+
 /* ------------------------------------------------------------------ */
 /* the reference's file formats (host only, no device needed)          */
 /* ------------------------------------------------------------------ */

@@ -101,3 +101,10 @@ def test_stream_and_batch_workload_shapes():
     for p, ct in b64:
         assert ct.shape == (p.M, p.N)
         assert 128 <= p.N <= 1024 and p.M == max(16, (p.N // 2 // 8) * 8)
+
+
+def test_serve_pool_argument_errors_need_no_gpu():
+    lib = cb.load()
+    assert lib.cp_serve_pool(None, 0, None, 0, 1, None) == cb.CP_E_ARG
+    hs = (C.c_void_p * 2)(None, None)
+    assert lib.cp_serve_pool(C.cast(hs, C.c_void_p), 2, None, 0, 1, None) == cb.CP_E_ARG

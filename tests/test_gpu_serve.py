@@ -84,3 +84,43 @@ def test_admission_waits_for_footprint_and_rejects_oversized():
     assert c["tAdmit"] >= a["tDone"] - 1e-9  # second waited for the first's space
     assert a["dp"] == c["dp"] and a["iter"] == c["iter"]
     assert d["rc"] == cb.CP_E_CAPACITY and d["slot"] == -1
+
+
+def test_pool_results_equal_alone_and_streams_overlap():
+    """cp_serve_pool: the 5 requests over two 1-slot batches (two host
+    threads, two streams). Each result equals the request served alone, bit
+    for bit, and the two batches ran at the same time."""
+    reqs = requests()
+    say("[serve] pool of 2 batches x 1 slot, 5 requests")
+    bs = [cb.Batch(cb.CP_PREC_MIXED, 128 << 20, 1) for _ in range(2)]
+    for b in bs:
+        b.probe(cb.K_PCG_Q)  # timing bursts take the GPU alone; results unchanged
+    mix = cb.serve_pool(bs, reqs, steps_per_poll=10)
+    bursts = [len(b.probe_read()[0]) for b in bs]
+    for b in bs:
+        b.close()
+    for k, r in enumerate(mix):
+        assert r["rc"] == cb.CP_OK and r["status"] == cb.SLOT_CONVERGED, (k, r)
+        assert r["tAdmit"] >= reqs[k][2] - 1e-9
+    assert sum(bursts) > 0
+    # each batch has one slot, so overlapping runs were on different streams
+    overlap = any(a["tAdmit"] < c["tDone"] and c["tAdmit"] < a["tDone"]
+                  for a in mix for c in mix if a is not c)
+    assert overlap
+    for k, req in enumerate(reqs):
+        b = cb.Batch(cb.CP_PREC_MIXED, 64 << 20, 1)
+        alone = b.serve([(req[0], req[1], 0.0, req[3])], steps_per_poll=10)[0]
+        b.close()
+        assert alone["iter"] == mix[k]["iter"], (k, alone, mix[k])
+        assert alone["dp"] == mix[k]["dp"], (k, alone["dp"], mix[k]["dp"])
+
+
+def test_pool_rejects_a_busy_batch():
+    p, ct = wl.network(48, 96, 7, cb.CP_PREC_MIXED)
+    a = cb.Batch(cb.CP_PREC_MIXED, 64 << 20, 1)
+    b = cb.Batch(cb.CP_PREC_MIXED, 64 << 20, 1)
+    b.admit(fast(p), ct)
+    with pytest.raises(cb.CpError):
+        cb.serve_pool([a, b], [(fast(p), ct, 0.0, 1.0 - 1e-6)])
+    a.close()
+    b.close()
