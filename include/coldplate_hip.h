@@ -380,8 +380,11 @@ int cp_serve(cp_batch* b, const cp_request* reqs, int n, int steps_per_poll, cp_
    run concurrently, so one batch's latency-bound launches (per-slot
    finalisations, coarse multigrid levels) overlap another's bandwidth-bound
    ones. Timing bursts (cp_batch_probe) run with no other batch's work in
-   flight. Times in results are seconds after the start of the call.
-   Returns the first error any batch met (the others stop). */
+   flight. Batches are meant to be alike: a request that does not fit an
+   empty batch is rejected there, as cp_serve rejects it. Times in results
+   are seconds after the start of the call. Returns the first error any
+   batch met; the others stop at their next poll, possibly with slots still
+   admitted. */
 int cp_serve_pool(cp_batch* const* batches, int nbatches, const cp_request* reqs, int n, int steps_per_poll,
                   cp_result* results);
 
