@@ -840,6 +840,9 @@ __global__ __launch_bounds__(kBlock) void k_pres_assemble(KArgs a) {
         const auto oAP = G(static_cast<T*>(d.pr[P_AP])), oR = G(static_cast<T*>(d.pr[P_R]));
         const auto oX = G(static_cast<T*>(d.pr[P_X]));
         const auto bm = G(d.bm), pc = G(d.pc);
+        // the FP64 imbalance is read by SOR and by refinement cycles after
+        // the first only
+        const bool keepB = (d.presType == 0) | (d.refineCycles > 1);
         // all loads first (clamped addresses)
         double uE[R], uW[R], vN[R], vS[R], dEe[R], dEw[R], dNn[R], dNs[R];
 #pragma unroll
@@ -878,7 +881,7 @@ __global__ __launch_bounds__(kBlock) void k_pres_assemble(KArgs a) {
                 const T rr = T(b);
                 oR[k] = rr;
                 oX[k] = T(0);
-                bm[k] = b;
+                if (keepB) bm[k] = b;
                 pc[k] = 0.0;
                 const T z = rr / T(aP);
                 v[1] += double(rr) * double(z);
