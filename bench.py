@@ -70,10 +70,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=8, help="slots of the request loop (per stream)")
-    ap.add_argument("--streams", type=int, default=3,
-                    help="batches served at once (cp_serve_pool), each on its own HIP stream and host thread")
-    ap.add_argument("--requests", type=int, default=24, help="solve requests per step")
+    ap.add_argument("--batch", type=int, default=None,
+                    help="slots of the request loop per stream (default: 8; stream workload: 16 over its streams)")
+    ap.add_argument("--streams", type=int, default=None,
+                    help="batches served at once (cp_serve_pool), each on its own HIP stream and host thread "
+                         "(default: 3; stream workload: 2)")
+    ap.add_argument("--requests", type=int, default=None,
+                    help="solve requests per step (default: 24; stream workload: 120)")
     ap.add_argument("--eps", type=float, default=1e-7)
     ap.add_argument("--jacobi", type=int, default=3, help="Jacobi sweeps per momentum solve")
     ap.add_argument("--pcg", type=int, default=4, help="CG iterations per refinement cycle")
@@ -268,7 +271,7 @@ def other_workload(a) -> dict:
         b.close()
         return out
     if a.workload == "stream":
-        n = a.requests
+        n = a.requests or 120
         specs = wl.stream_2000(n, 5)
         arr = wl.poisson_arrivals(n, a.rate)
         eps = 1e-6
@@ -276,10 +279,13 @@ def other_workload(a) -> dict:
         for r, t in zip(specs, arr):
             p, ct = wl.build_request(r)
             reqs.append((served_params(p, a, eps), ct, float(t), 0.5 if r["kind"] == 2 else GAMMA_MIN))
-        b = cb.Batch(cb.CP_PREC_MIXED, 24 << 30, a.batch)
+        nst = a.streams or 2
+        per = a.batch or max(1, 16 // nst)
+        bs = [cb.Batch(cb.CP_PREC_MIXED, 24 << 30, per) for _ in range(nst)]
+        b = bs[0]
         b.probe(cb.K_PCG_Q)
         t0 = time.perf_counter()
-        res = b.serve(reqs, a.check)
+        res = b.serve(reqs, a.check) if nst == 1 else cb.serve_pool(bs, reqs, a.check)
         dt = time.perf_counter() - t0
         conv = sum(r["status"] == cb.SLOT_CONVERGED for r in res)
         lat = [r["tDone"] - t for r, t in zip(res, arr)]
@@ -287,13 +293,14 @@ def other_workload(a) -> dict:
         out = dict(base, value=round(conv / dt, 4), unit="converged solves/s", ms_per_step=round(dt * 1e3, 1),
                    config={"workload": "stream", "requests": n, "arrivals": f"Poisson, {a.rate:g}/s",
                            "sizes": "log-uniform 128^2 .. 2048x1024", "mix": "straight / network / gamma, 2.5D 50%",
-                           "slots": a.batch, "stop_test": f"error <= {eps:g}, cap 20000"},
+                           "streams": nst, "slots_per_stream": per, "stop_test": f"error <= {eps:g}, cap 20000"},
                    converged=conv, capped=len(res) - conv,
                    latency_s={"p50": round(float(np.percentile(lat, 50)), 3),
                               "p99": round(float(np.percentile(lat, 99)), 3), "basis": "arrival to completion",
                               "queue_p50": round(float(np.percentile(queue, 50)), 3)},
                    roofline=pcg_roofline(b), cpu_baseline=None)
-        b.close()
+        for x in bs:
+            x.close()
         return out
     # gamma2048: mixed against the FP64 device path (pinned to the oracle on smaller grids)
     eps = 1e-5
@@ -341,7 +348,9 @@ def main() -> int:
         return 0
     if a.calibrate_copy:
         copy_calibration(a.calibrate_copy)
-    B = a.batch
+    B = a.batch or 8
+    a.streams = a.streams or 3
+    a.requests = a.requests or 24
     refs = references(8)
     GEOMS[:] = [wl.network_1024x512(s) for s in range(8)]
     # arena: B of the largest geometry's footprint plus a margin
