@@ -124,3 +124,20 @@ def test_pool_rejects_a_busy_batch():
         cb.serve_pool([a, b], [(fast(p), ct, 0.0, 1.0 - 1e-6)])
     a.close()
     b.close()
+
+
+def test_pool_rejects_an_invalid_request_and_serves_the_rest():
+    """An invalid request (uvAlpha = 0) in the shared queue gets its error code
+    and no slot; the other requests, arriving over time, all converge."""
+    reqs = requests()[:4]
+    bad = reqs[1][0].copy()
+    bad.uvAlpha = 0.0
+    reqs.insert(2, (bad, reqs[1][1], 0.01, 1.0 - 1e-6))
+    say("[serve] pool with an invalid request")
+    bs = [cb.Batch(cb.CP_PREC_MIXED, 128 << 20, 1) for _ in range(2)]
+    res = cb.serve_pool(bs, reqs, steps_per_poll=10)
+    for b in bs:
+        b.close()
+    assert res[2]["rc"] == cb.CP_E_ARG and res[2]["slot"] == -1 and res[2]["status"] == cb.SLOT_FAILED
+    for k in (0, 1, 3, 4):
+        assert res[k]["rc"] == cb.CP_OK and res[k]["status"] == cb.SLOT_CONVERGED, (k, res[k])
