@@ -983,9 +983,20 @@ int enqueueFixedStep(cp_batch* b, int J, int P, int R, int grid) {
         if ((rc = launch(b, K_MOM_EXPL, grid)) || (rc = launch(b, K_MOM_EXPL_BC, grid))) return rc;
     }
     if (b->anyImplicit) {
-        if ((rc = launch(b, K_MOM_ASM, grid))) return rc;  // with the first sweep
-        for (int k = 1; k < J; ++k)
-            if ((rc = launch(b, K_JAC, grid))) return rc;
+        // the first sweep is the assembly's; the last sweep's stop test could
+        // only end a system that the budget ends anyway, so it goes without
+        // its finalisation and k_mom_finish takes the budget as the count
+        const bool f32 = b->precision == CP_PREC_MIXED;
+        for (int k = 0; k < J; ++k) {
+            const Kern kk = k == 0 ? K_MOM_ASM : K_JAC;
+            if (k + 1 < J) {
+                if ((rc = launch(b, kk, grid))) return rc;
+            } else {
+                const hipError_t e = launchWorkOnly(kk, f32, b->tileRows, grid, b->stream, kargsFor(b, kk));
+                if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+            }
+        }
+        if (J == 0 && (rc = launch(b, K_MOM_ASM, grid))) return rc;
         if ((rc = launch(b, K_MOM_FIN, grid))) return rc;
     }
     if ((rc = launch(b, K_PRES_ASM, grid))) return rc;

@@ -757,8 +757,12 @@ __global__ __launch_bounds__(kBlock) void k_mom_finish(KArgs a) {
         const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
         const double Uin = st.inletVelocity;
         const double maxVel = 3.0 * fmax(fabs(Uin), 0.1);
-        const auto dU = G(static_cast<const T*>(d.mu[M_D0 + (st.jiters[0] & 1)]));
-        const auto dV = G(static_cast<const T*>(d.mv[M_D0 + (st.jiters[1] & 1)]));
+        // a system still marked running made every sweep of its budget (the
+        // fixed-budget step launches the last sweep without its finalisation)
+        const int itU = st.jdone[0] ? st.jiters[0] : d.maxMomIter;
+        const int itV = st.jdone[1] ? st.jiters[1] : d.maxMomIter;
+        const auto dU = G(static_cast<const T*>(d.mu[M_D0 + (itU & 1)]));
+        const auto dV = G(static_cast<const T*>(d.mv[M_D0 + (itV & 1)]));
         const auto u = G(d.u), vv = G(d.v), gam = G(d.gam);
         const auto us = G(d.us), vs = G(d.vs);
         // all loads first (clamped addresses), then the branchy selection
@@ -2240,6 +2244,11 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
                 st.residU = tot[0];
                 st.residV = tot[1];
                 if (tot[2] != 0.0) st.health = 1;
+                for (int c = 0; c < 2; ++c)
+                    if (phase == PH_MOM_FIN && !st.jdone[c]) {  // the budget's last sweep, unfinalised
+                        st.jdone[c] = 1;
+                        st.jiters[c] = d.maxMomIter;
+                    }
                 st.itU = st.jiters[0];
                 st.itV = st.jiters[1];
             }
