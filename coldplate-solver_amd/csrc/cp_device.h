@@ -155,6 +155,7 @@ enum Phase {
     PH_MOM_EXPL,  // explicit momentum: residuals
     PH_SOR,       // SOR: stop test after a red + black sweep pair
     PH_PCGU_MGZ,  // PH_PCG_U's stop test, then PH_MG_Z, after the V-cycle that follows the update
+    PH_PCGU_REFACC,  // PH_PCG_U's stop test, then PH_REF_ACC, after the accumulation that follows the last update
 };
 // Partial entry of the finest up-sweep's r.z: apart from the PCG update's
 // entries 0 and 1, which PH_PCGU_MGZ reads after the V-cycle.

@@ -1009,6 +1009,7 @@ int enqueueFixedStep(cp_batch* b, int J, int P, int R, int grid) {
     }
     if ((rc = mgSetup(b)) || (rc = mgVcycle(b))) return rc;
     for (int c = 0; c < R; ++c) {
+        bool updateUnfinalised = false;
         for (int k = 0; k < P; ++k) {
             if ((rc = launch(b, K_PCG_Q, grid))) return rc;
             // the V-cycle after the last iteration would only feed a search
@@ -1020,12 +1021,26 @@ int enqueueFixedStep(cp_batch* b, int J, int P, int R, int grid) {
                                                     kargsFor(b, K_PCG_U));
                 if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
                 if ((rc = mgVcycle(b, true))) return rc;
+            } else if (k + 1 == P && b->precision == CP_PREC_MIXED) {
+                // the last update's stop test is finalised with the
+                // accumulation that follows it (PH_PCGU_REFACC)
+                const hipError_t e = launchWorkOnly(K_PCG_U, true, pRows(true, b->tileRows), grid, b->stream,
+                                                    kargsFor(b, K_PCG_U));
+                if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+                updateUnfinalised = true;
             } else if ((rc = launch(b, K_PCG_U, grid))) {
                 return rc;
             }
         }
         if (b->precision != CP_PREC_MIXED) break;
-        if ((rc = launch(b, K_REF_ACC, grid))) return rc;
+        if (!updateUnfinalised) {
+            if ((rc = launch(b, K_REF_ACC, grid))) return rc;
+        } else {
+            KArgs ka = kargsFor(b, K_REF_ACC);
+            hipError_t e = launchWorkOnly(K_REF_ACC, true, pRows(true, b->tileRows), grid, b->stream, ka);
+            if (e == hipSuccess) e = launchFinalize(PH_PCGU_REFACC, b->maxSlots, b->stream, kargs(b));
+            if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+        }
         if (c + 1 < R) {
             if ((rc = launch(b, K_REF_RES, grid))) return rc;
             if ((rc = mgVcycle(b))) return rc;

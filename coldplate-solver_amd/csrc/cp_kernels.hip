@@ -2146,6 +2146,30 @@ __global__ __launch_bounds__(kBlock) void k_dp_monitor(KArgs a) {
 // scalars. Launched right after the work kernel of `phase`; it applies the
 // same skip rule as that kernel, so it only reads partials written by it.
 // ---------------------------------------------------------------------------
+// The PCG update's stop test and cap (pressure_solver.cpp:151, 167-178), and for
+// the Jacobi preconditioner beta. Called by the whole block.
+__device__ __forceinline__ void pcgUpdateFinalize(const KArgs& a, const SlotDesc& d, SlotState& st) {
+    double tot[2];
+    slotReduce<2, 0u>(a.partials, d.ptile0, d.pntiles, tot);
+    if (threadIdx.x == 0) {
+        st.rr = tot[0];
+        if (sqrt(tot[0]) < d.pTol) {
+            st.pdone = 1;
+            st.piters = st.pk;
+        } else {
+            if (!d.precond) {  // multigrid: beta follows the V-cycle (PH_MG_Z)
+                st.beta = tot[1] / st.rz;
+                st.rz = tot[1];
+            }
+            st.pk += 1;
+            if (st.pk >= d.maxPresIter) {
+                st.pdone = 1;
+                st.piters = d.maxPresIter;
+            }
+        }
+    }
+}
+
 // r.z of the V-cycle just applied: the start of a (re)started CG (pk == 0)
 // or the next search direction's beta. Called by the whole block.
 __device__ __forceinline__ void mgZFinalize(const KArgs& a, const SlotDesc& d, SlotState& st) {
@@ -2209,6 +2233,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
             break;
         case PH_REF_ACC:
         case PH_REF_RES:
+        case PH_PCGU_REFACC:
             if (st.stepPhase) return;
             break;
         default:
@@ -2304,25 +2329,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
         }
         case PH_PCG_U:
         case PH_PCGU_MGZ: {
-            double tot[2];
-            slotReduce<2, 0u>(a.partials, d.ptile0, d.pntiles, tot);
-            if (threadIdx.x == 0) {
-                st.rr = tot[0];
-                if (sqrt(tot[0]) < d.pTol) {
-                    st.pdone = 1;
-                    st.piters = st.pk;
-                } else {
-                    if (!d.precond) {  // multigrid: beta follows the V-cycle (PH_MG_Z)
-                        st.beta = tot[1] / st.rz;
-                        st.rz = tot[1];
-                    }
-                    st.pk += 1;
-                    if (st.pk >= d.maxPresIter) {
-                        st.pdone = 1;
-                        st.piters = d.maxPresIter;
-                    }
-                }
-            }
+            pcgUpdateFinalize(a, d, st);
             if (phase == PH_PCG_U) break;
             // the V-cycle ran after the update whatever the stop test said;
             // its z only counts for a multigrid slot whose CG goes on
@@ -2331,6 +2338,12 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
             mgZFinalize(a, d, st);
             break;
         }
+        case PH_PCGU_REFACC:
+            // the last update of the cycle (skipped by the kernel for a slot
+            // whose CG had stopped), then the accumulation's bookkeeping
+            if (!st.pdone) pcgUpdateFinalize(a, d, st);
+            __syncthreads();
+            [[fallthrough]];
         case PH_REF_ACC: {
             if (threadIdx.x == 0) {
                 st.pitTotal += st.piters;
