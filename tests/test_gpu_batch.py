@@ -195,3 +195,28 @@ def test_autotune_keeps_results():
         dps.append(b.full_dp(s))
         b.close()
     assert abs(dps[1] / dps[0] - 1) <= 1e-7, dps
+
+
+@pytest.mark.parametrize("sweeps", [1, 2, 3])
+def test_small_sweep_budgets_match_oracle(sweeps):
+    """Graph-replayed FP64 batch with a Jacobi budget of 1, 2 or 3 sweeps:
+    the first sweep is made by the momentum assembly and the budget's last
+    sweep runs without its own finalisation (the momentum finish takes the
+    budget as the count). Step by step, the sweep counts equal the oracle's
+    and the fields stay within 1e-8 of it."""
+    p, ct = wl.network(64, 128, 3)
+    p.maxMomentumIter = sweeps
+    say(f"[batch] FP64 graph, {sweeps} Jacobi sweeps")
+    b = cb.Batch(cb.CP_PREC_FP64, 64 << 20, 1)
+    s = b.admit(p, ct)
+    o = op.Oracle(p, ct)
+    for step in range(6):
+        b.run(1)
+        so = o.step()
+        info = b.poll()[s]
+        assert (info["itU"], info["itV"]) == (so["itU"], so["itV"]), (step, info, so)
+    u, v, pp = b.fields(s)
+    for a, f in ((u, cb.F_U), (v, cb.F_V), (pp, cb.F_P)):
+        ref = o.field(f)
+        assert np.abs(a - ref).max() <= 1e-8 * np.abs(ref).max()
+    b.close()
