@@ -220,3 +220,31 @@ def test_small_sweep_budgets_match_oracle(sweeps):
         ref = o.field(f)
         assert np.abs(a - ref).max() <= 1e-8 * np.abs(ref).max()
     b.close()
+
+
+def test_graph_batch_matches_polled_mixed_multigrid():
+    """Mixed precision, multigrid CG, 3 sweeps / 4 CG iterations / 1
+    refinement cycle: the graph step (its finalisations merged where a later
+    one can carry them) equals the polled context, which finalises every
+    phase on its own, bit for bit and with the same inner counts."""
+    p, ct = wl.network(64, 128, 3, cb.CP_PREC_MIXED)
+    q = wl.fast_mode(p, 3, 4, 1)
+    q.pressurePrecond = cb.CP_PRECOND_AMG
+    say("[batch] mixed multigrid: graph vs polled")
+    b = cb.Batch(cb.CP_PREC_MIXED, 64 << 20, 1)
+    b.set_tiling(1)  # the context's tiling for a 64 x 128 grid
+    s = b.admit(q, ct)
+    g = cb.Solver(q, ct)
+    for step in range(15):
+        b.run(1)
+        sg = g.step()
+        info = b.poll()[s]
+        for k in ("itU", "itV", "itP", "iter"):
+            assert info[k] == sg[k], (step, k, info[k], sg[k])
+        assert info["error"] == sg["error"], (step, info["error"], sg["error"])
+    u, v, pp = b.fields(s)
+    np.testing.assert_array_equal(u, g.field(cb.F_U))
+    np.testing.assert_array_equal(v, g.field(cb.F_V))
+    np.testing.assert_array_equal(pp, g.field(cb.F_P))
+    b.close()
+    g.close()
