@@ -222,15 +222,17 @@ def test_small_sweep_budgets_match_oracle(sweeps):
     b.close()
 
 
-def test_graph_batch_matches_polled_mixed_multigrid():
-    """Mixed precision, multigrid CG, 3 sweeps / 4 CG iterations / 1
-    refinement cycle: the graph step (its finalisations merged where a later
-    one can carry them) equals the polled context, which finalises every
-    phase on its own, bit for bit and with the same inner counts."""
+@pytest.mark.parametrize("cycles,precond", [(1, cb.CP_PRECOND_AMG), (2, cb.CP_PRECOND_AMG),
+                                            (2, cb.CP_PRECOND_JACOBI)], ids=["amg-1", "amg-2", "jacobi-2"])
+def test_graph_batch_matches_polled_mixed(cycles, precond):
+    """Mixed precision, 3 sweeps / 4 CG iterations per refinement cycle: the
+    graph step (its finalisations merged where a later one can carry them)
+    equals the polled context, which finalises every phase on its own, bit
+    for bit and with the same inner counts."""
     p, ct = wl.network(64, 128, 3, cb.CP_PREC_MIXED)
-    q = wl.fast_mode(p, 3, 4, 1)
-    q.pressurePrecond = cb.CP_PRECOND_AMG
-    say("[batch] mixed multigrid: graph vs polled")
+    q = wl.fast_mode(p, 3, 4, cycles)
+    q.pressurePrecond = precond
+    say(f"[batch] mixed, {cycles} refinement cycles, preconditioner {precond}: graph vs polled")
     b = cb.Batch(cb.CP_PREC_MIXED, 64 << 20, 1)
     b.set_tiling(1)  # the context's tiling for a 64 x 128 grid
     s = b.admit(q, ct)
