@@ -192,8 +192,11 @@ def main() -> int:
         lines += ["Commands (repository root, one rocprofv3 run each):", "", "```"]
         lines += open(a.commands).read().strip().splitlines()
         lines += ["```", ""]
-    lines += ["## Kernel time (rocprofv3 --kernel-trace --stats)", "",
-              "| kernel | calls | avg µs | min µs | max µs | % time |", "|---|---|---|---|---|---|"]
+    lines += ["## Kernel time (rocprofv3 --kernel-trace --stats)", ""]
+    if a.burst:
+        lines += ["Several streams ran at once in this trace, so kernels overlap and their durations include the "
+                  "sharing; per-kernel times without it are in the one-stream profile.", ""]
+    lines += ["| kernel | calls | avg µs | min µs | max µs | % time |", "|---|---|---|---|---|---|"]
     for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:14]:
         lines.append(f"| `{r['Name']}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
                      f"{float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
