@@ -174,6 +174,7 @@ EXPORTS = [
     ("cp_batch_run", C.c_int, [_vp, C.c_int]),
     ("cp_batch_poll", C.c_int, [_vp, C.POINTER(SlotInfo)]),
     ("cp_batch_get_fields", C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
+    ("cp_batch_set_fields", C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
     ("cp_batch_plane", C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.POINTER(PlaneMetrics)]),
     ("cp_batch_evict", C.c_int, [_vp, C.c_int]),
     ("cp_batch_set_tiling", C.c_int, [_vp, C.c_int, C.c_int]),
@@ -396,6 +397,10 @@ class Batch:
         u = np.empty((M + 1, N)); v = np.empty((M, N + 1)); p = np.empty((M + 1, N + 1))
         check(load().cp_batch_get_fields(self.h, slot, dptr(u), dptr(v), dptr(p)), self.h)
         return u, v, p
+
+    def set_fields(self, slot: int, u=None, v=None, p=None) -> None:
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, p)]
+        check(load().cp_batch_set_fields(self.h, slot, *[dptr(a) for a in arrs]), self.h)
 
     def plane(self, slot: int, x: float, gamma_min: float = 1.0 - 1e-6) -> dict:
         m = PlaneMetrics()

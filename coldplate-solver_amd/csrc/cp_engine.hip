@@ -68,6 +68,14 @@ int setError(std::string* dst, int code, const char* fmt, ...) {
 
 size_t roundUp(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// The kernels are built for gfx950 (MI355X) only: a device of any other
+// architecture cannot run them.
+bool isGfx950(int dev) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return false;
+    return strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+}
+
 // ---------------------------------------------------------------------------
 // arena: first-fit free list over one device allocation (offsets in bytes,
 // 256-byte granules), coalescing on free.
@@ -707,8 +715,10 @@ int batchCreate(int precision, size_t budget, int maxSlots, cp_batch** out, std:
     if (maxSlots < 1 || maxSlots > 4096) return setError(errOut, CP_E_ARG, "max_slots out of range");
     if (precision != CP_PREC_FP64 && precision != CP_PREC_MIXED)
         return setError(errOut, CP_E_ARG, "unknown precision %d", precision);
-    int ndev = 0;
+    int ndev = 0, dev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return setError(errOut, CP_E_NODEV, "no HIP device");
+    if (hipGetDevice(&dev) != hipSuccess || !isGfx950(dev))
+        return setError(errOut, CP_E_NODEV, "the current HIP device %d is not a gfx950 (MI355X) device", dev);
     cp_batch* b = new cp_batch();
     std::string& err = b->err;
     b->precision = precision;
@@ -1196,7 +1206,9 @@ const char* cp_last_error(const void* ctx) {
 int cp_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
-    return n;
+    int usable = 0;
+    for (int d = 0; d < n; ++d) usable += isGfx950(d) ? 1 : 0;
+    return usable;
 }
 
 // ---- batch -----------------------------------------------------------------
@@ -1402,6 +1414,7 @@ int serveWorker(cp_batch* b, ServeQueue& q, int steps_per_poll) {
             res.status = st.status;
             res.iter = st.iter;
             res.error = st.error;
+            if (st.health) res.rc = CP_E_DIVERGED;  // failed on a non-finite field; evicted like any other
             const double xOut = double(b->slots[s].N) * b->slots[s].prm.hx;
             cp_plane_metrics in, out;
             if ((rc = planeSample(b, s, 0.0, q.reqs[r].gammaMin, &in))) break;
@@ -1670,11 +1683,11 @@ int cp_destroy(cp_ctx* ctx) {
     return CP_OK;
 }
 
-int cp_set_fields(cp_ctx* ctx, const double* u, const double* v, const double* p) {
-    if (!ctx) return setError(nullptr, CP_E_ARG, "null ctx");
-    cp_batch* b = ctx->b;
-    const SlotHost& sh = b->slots[ctx->slot];
-    const SlotDesc& d = b->hDesc[ctx->slot];
+static int setFieldsSlot(cp_batch* b, int slot, const double* u, const double* v, const double* p) {
+    if (slot < 0 || slot >= b->maxSlots || !b->slots[slot].used)
+        return setError(&b->err, CP_E_ARG, "slot %d not in use", slot);
+    const SlotHost& sh = b->slots[slot];
+    const SlotDesc& d = b->hDesc[slot];
     const int M = sh.M, N = sh.N, P = sh.pitch;
     int rc;
     if (u) {
@@ -1687,6 +1700,16 @@ int cp_set_fields(cp_ctx* ctx, const double* u, const double* v, const double* p
     }
     if (p && (rc = uploadPlane(b, d.p, P, M + 1, N + 1, p))) return rc;
     return CP_OK;
+}
+
+int cp_set_fields(cp_ctx* ctx, const double* u, const double* v, const double* p) {
+    if (!ctx) return setError(nullptr, CP_E_ARG, "null ctx");
+    return setFieldsSlot(ctx->b, ctx->slot, u, v, p);
+}
+
+int cp_batch_set_fields(cp_batch* b, int slot, const double* u, const double* v, const double* p) {
+    if (!b) return setError(nullptr, CP_E_ARG, "null batch");
+    return setFieldsSlot(b, slot, u, v, p);
 }
 
 int cp_get_fields(cp_ctx* ctx, double* u, double* v, double* p) {

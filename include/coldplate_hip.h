@@ -187,7 +187,9 @@ int cp_params_default(cp_params* p);
 /* Message of the last error of ctx (NULL: the library-wide last error). */
 const char* cp_last_error(const void* ctx);
 
-/* Number of usable gfx950 devices (0 without a GPU). */
+/* Number of usable gfx950 devices (0 without a GPU); devices of other
+   architectures are not counted. cp_create / cp_batch_create return
+   CP_E_NODEV when the current device is not a gfx950. */
 int cp_device_count(void);
 
 /* ------------------------------------------------------------------ */
@@ -286,6 +288,10 @@ int cp_batch_run(cp_batch* b, int steps);
 int cp_batch_poll(cp_batch* b, cp_slot_info* info);
 /* Read fields / plane metrics of a slot (same shapes as cp_get_fields). */
 int cp_batch_get_fields(cp_batch* b, int slot, double* u, double* v, double* p);
+/* Restart a slot from given fields, as cp_set_fields does for a context
+   (SIMPLE.cpp:203-232): any pointer may be NULL; u* = u and v* = v. Only
+   that slot's arrays change. Between cp_batch_run calls only. */
+int cp_batch_set_fields(cp_batch* b, int slot, const double* u, const double* v, const double* p);
 int cp_batch_plane(cp_batch* b, int slot, double x, double gamma_min, cp_plane_metrics* out);
 /* Release a slot's arena space. */
 int cp_batch_evict(cp_batch* b, int slot);
@@ -293,9 +299,9 @@ int cp_batch_evict(cp_batch* b, int slot);
 /* Launch shape of the batch (no counterpart in the reference): tile_rows R
    in {1, 2, 4, 8} gives 64 x 4R-node tiles for every slot (0 = keep; the
    default is 4), wg_per_cu the workgroups per CU of the replayed launches
-   (0 = keep; default 8). Changing R retiles the admitted slots; results of a
-   slot depend on R only through the order of its partial sums. Chosen by
-   the autotuner (coldplate-solver_amd/autotune.py). */
+   (0 = keep; default 16). Changing R retiles the admitted slots; results of
+   a slot depend on R only through the order of its partial sums.
+   cp_batch_autotune chooses both by timing. */
 int cp_batch_set_tiling(cp_batch* b, int tile_rows, int wg_per_cu);
 
 /* 1 (default): cp_batch_run replays the outer iteration as a captured
@@ -355,7 +361,9 @@ typedef struct cp_result {
     int32_t status;           /* CP_SLOT_CONVERGED / _MAXITER / _FAILED */
     int32_t iter;             /* outer iterations */
     int32_t slot;             /* slot it ran in (-1: never admitted) */
-    int32_t rc;               /* CP_OK, or the error that rejected it */
+    int32_t rc;               /* CP_OK; the error that rejected it; or
+                                 CP_E_DIVERGED for a solve that failed on a
+                                 non-finite field (status CP_SLOT_FAILED) */
     double  error;            /* final max(residU, residV, residMass) */
     double  dp;               /* full-system total-pressure drop, x = 0 .. N hx */
     double  tAdmit, tDone;    /* seconds after the start of cp_serve */

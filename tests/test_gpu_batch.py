@@ -250,3 +250,93 @@ def test_graph_batch_matches_polled_mixed(cycles, precond):
     np.testing.assert_array_equal(pp, g.field(cb.F_P))
     b.close()
     g.close()
+
+
+@pytest.mark.parametrize("precision", [cb.CP_PREC_FP64, cb.CP_PREC_MIXED], ids=["fp64", "mixed"])
+def test_failed_slot_is_isolated(precision):
+    """A slot whose state turns non-finite ends CP_SLOT_FAILED with
+    CP_E_DIVERGED at the end of that outer iteration. The slots packed with
+    it stay bit-identical to the same geometries run alone (every reduction
+    and every skip rule is per slot), and its arena space is reused."""
+    gs = []
+    for p, ct in geoms():
+        q = wl.fast_mode(p) if precision == cb.CP_PREC_MIXED else p.copy()
+        q.precision = precision
+        gs.append((q, ct))
+    say(f"[batch] failure isolation, precision {precision}")
+    alone = {}
+    for k in (0, 2):
+        b = cb.Batch(precision, 256 << 20, 1)
+        s = b.admit(*gs[k])
+        b.run(20)
+        alone[k] = b.fields(s)
+        b.close()
+    b = cb.Batch(precision, 256 << 20, 4)
+    slots = [b.admit(*g) for g in gs]
+    b.run(8)
+    u, _, _ = b.fields(slots[1])
+    u[u.shape[0] // 2, u.shape[1] // 2] = np.nan
+    b.set_fields(slots[1], u=u)
+    b.run(12)
+    info = b.poll()
+    bad = info[slots[1]]
+    assert bad["status"] == cb.SLOT_FAILED and bad["health"] == cb.CP_E_DIVERGED, bad
+    assert bad["iter"] == 9, bad  # the iteration that read the NaN ends it
+    for k in (0, 2):
+        assert info[slots[k]]["iter"] == 20 and info[slots[k]]["status"] == cb.SLOT_RUNNING
+        for x, y in zip(b.fields(slots[k]), alone[k]):
+            np.testing.assert_array_equal(x, y)
+    b.evict(slots[1])
+    s_new = b.admit(*gs[1])
+    assert s_new == slots[1]
+    b.run(3)
+    info = b.poll()
+    assert info[s_new]["status"] == cb.SLOT_RUNNING and info[s_new]["iter"] == 3 and info[s_new]["health"] == 0
+    assert info[slots[0]]["iter"] == 23
+    b.close()
+
+
+@pytest.mark.parametrize("precision", [cb.CP_PREC_FP64, cb.CP_PREC_MIXED], ids=["fp64", "mixed"])
+def test_graph_batch_mixed_budgets_equal_alone(precision):
+    """One graph batch holding slots whose Jacobi, CG and refinement budgets
+    differ: the captured step takes the largest budget of each loop, and
+    every slot stops its own loops at its own budget. Step by step, each
+    slot's inner counts and error, and its final fields bit for bit, equal
+    the same slot run alone in a batch of its own."""
+    base = [wl.network(64, 128, 3, precision), wl.network(48, 128, 11, precision),
+            wl.gamma_field(40, 96, 5, precision, corr=5.0)]
+    # (Jacobi sweeps, CG iterations, refinement cycles)
+    budgets = [(2, 3, 1), (4, 5, 2), (3, 2, 1)]
+    gs = []
+    for (p, ct), (js, ps, rc) in zip(base, budgets):
+        q = wl.fast_mode(p, js, ps, rc) if precision == cb.CP_PREC_MIXED else p.copy()
+        q.precision = precision
+        q.pressurePrecond = cb.CP_PRECOND_AMG
+        if precision == cb.CP_PREC_FP64:
+            q.maxMomentumIter, q.maxPressureIter = js, ps
+        gs.append((q, ct))
+    say(f"[batch] mixed budgets in one graph, precision {precision}")
+    steps = 8
+    ref = []
+    for q, ct in gs:
+        b = cb.Batch(precision, 128 << 20, 1)
+        s = b.admit(q, ct)
+        hist = []
+        for _ in range(steps):
+            b.run(1)
+            i = b.poll()[s]
+            hist.append((i["itU"], i["itV"], i["itP"], i["error"]))
+        ref.append((hist, b.fields(s)))
+        b.close()
+    b = cb.Batch(precision, 256 << 20, 3)
+    slots = [b.admit(q, ct) for q, ct in gs]
+    for step in range(steps):
+        b.run(1)
+        info = b.poll()
+        for k, s in enumerate(slots):
+            got = (info[s]["itU"], info[s]["itV"], info[s]["itP"], info[s]["error"])
+            assert got == ref[k][0][step], (k, step, got, ref[k][0][step])
+    for k, s in enumerate(slots):
+        for x, y in zip(b.fields(s), ref[k][1]):
+            np.testing.assert_array_equal(x, y)
+    b.close()
