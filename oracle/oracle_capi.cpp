@@ -225,6 +225,41 @@ int or_get_system(void* hp, int which, int* n, int* nnz, int* row_ptr, int* col_
     return rc;
 }
 
+// The pseudo-CFL controller alone: for k = 0 .. n-1, the updates of
+// SIMPLE.cpp:321-327 in that order (updateCflSER with iteration k,
+// applyLineSearch, updateCflRamp; time_control.cpp:43-165) on monitored
+// residual monitor[k], from the initial CFL of SIMPLE.cpp:79-82. Writes the
+// CFL after each iteration and, when lsAlpha is non-null, the line search's
+// step size.
+int or_cfl_sequence(const cp_params* prm, int n, const double* monitor, double* cfl, double* lsAlpha) {
+    if (!prm || n < 0 || (n > 0 && (!monitor || !cfl))) return -1;
+    Simple<double> s;
+    s.prm = *prm;
+    s.pseudoCFL = prm->enableCflRamp ? prm->pseudoCFLInitial : 0.1;
+    for (int k = 0; k < n; ++k) {
+        s.updateCflSER(monitor[k], k);
+        s.applyLineSearch(monitor[k]);
+        s.updateCflRamp(monitor[k]);
+        cfl[k] = s.pseudoCFL;
+        if (lsAlpha) lsAlpha[k] = s.lsCurrentAlpha;
+    }
+    return 0;
+}
+
+// The pressure-drop window exit alone (SIMPLE.cpp:356-409) on given core
+// drops dp[0 .. n-1] of iterations 0 .. n-1: returns the first iteration
+// after which the run stops, or -1.
+int or_dp_sequence(const cp_params* prm, int n, const double* dp) {
+    if (!prm || n < 0 || (n > 0 && !dp)) return -2;
+    Simple<double> s;
+    s.prm = *prm;
+    for (int k = 0; k < n; ++k) {
+        s.monitorDrop(k, dp[k]);
+        if (s.dpConverged) return k;
+    }
+    return -1;
+}
+
 // Standalone restated solvers on caller CSR data (for the solver fixtures).
 int or_jacobi_f32(int n, const int* row_ptr, const int* col_idx, const float* values, const float* b,
                   float* x, const int* diagOff, float tol, int maxIter) {

@@ -60,6 +60,8 @@ def load() -> C.CDLL:
             "or_mg_apply": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp]),
             "or_mg_apply_f32": (C.c_int, [C.c_int, C.c_int, _fp, _fp, _fp, C.c_float, C.c_float, _fp, _fp]),
             "or_mg_level": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, _ip, _ip, _dp, _dp, _dp]),
+            "or_cfl_sequence": (C.c_int, [C.POINTER(Params), C.c_int, _dp, _dp, _dp]),
+            "or_dp_sequence": (C.c_int, [C.POINTER(Params), C.c_int, _dp]),
         }
         for k, (r, a) in sig.items():
             f = getattr(lib, k)
@@ -251,3 +253,21 @@ def mg_level(aP, cE, cN, level):
     load().or_mg_level(H, W, *(_ptr(x, _dp) for x in a), level, C.byref(hl), C.byref(wl),
                        *(_ptr(x, _dp) for x in out))
     return out[0], out[1], out[2], n
+
+
+def cfl_sequence(prm: Params, monitor) -> tuple[np.ndarray, np.ndarray]:
+    """The oracle's pseudo-CFL controller over a sequence of monitored
+    residuals: (CFL, line-search step size) after each iteration."""
+    m = np.ascontiguousarray(monitor, np.float64)
+    cfl = np.zeros(len(m))
+    ls = np.zeros(len(m))
+    rc = load().or_cfl_sequence(C.byref(prm), len(m), _ptr(m, _dp), _ptr(cfl, _dp), _ptr(ls, _dp))
+    assert rc == 0
+    return cfl, ls
+
+
+def dp_exit_iteration(prm: Params, drops) -> int:
+    """The oracle's pressure-drop window exit over given core drops: the
+    iteration after which it stops the run, or -1."""
+    d = np.ascontiguousarray(drops, np.float64)
+    return int(load().or_dp_sequence(C.byref(prm), len(d), _ptr(d, _dp)))

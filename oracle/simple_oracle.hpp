@@ -1017,7 +1017,14 @@ struct Simple {
         const bool gate = prm.usePressureDropSlopeGate && prm.dpSlopeWindowIters > 0;
         if (!prm.usePressureDropConvergence && !gate) return;
         const T gmin = T(1) - T(1e-6);
-        const T dp = plane(T(prm.xPlaneCoreInlet), gmin).avgTotal - plane(T(prm.xPlaneCoreOutlet), gmin).avgTotal;
+        monitorDrop(it, plane(T(prm.xPlaneCoreInlet), gmin).avgTotal - plane(T(prm.xPlaneCoreOutlet), gmin).avgTotal);
+    }
+
+    // The window and slope rule applied to the core drop `dp` of iteration
+    // `it` (SIMPLE.cpp:356-409).
+    void monitorDrop(int it, T dp) {
+        const bool gate = prm.usePressureDropSlopeGate && prm.dpSlopeWindowIters > 0;
+        if (!prm.usePressureDropConvergence && !gate) return;
         if (gate) {
             dpSlope.emplace_back(it, dp);
             if (int(dpSlope.size()) > prm.dpSlopeWindowIters) dpSlope.erase(dpSlope.begin());
