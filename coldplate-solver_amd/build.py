@@ -55,6 +55,20 @@ def build_lib() -> Path:
     return LIB
 
 
+def build_binding_check() -> Path | None:
+    """The reference-side binding (binding/hip_backend.cpp) linked into its
+    check program against the SIMPLE stand-in under tests/binding."""
+    stub = REPO / "tests" / "binding"
+    if not (stub / "check_binding.cpp").exists() or shutil.which("g++") is None:
+        return None
+    out = stub / "_build" / "check_binding"
+    out.parent.mkdir(parents=True, exist_ok=True)
+    run(["g++", "-std=c++17", "-O1", "-Wall", "-I", str(stub), "-I", str(REPO / "include"), "-I",
+         str(PKG / "binding"), str(stub / "check_binding.cpp"), str(PKG / "binding" / "hip_backend.cpp"),
+         "-L", str(LIB_DIR), "-lcoldplate_hip", f"-Wl,-rpath,{LIB_DIR}", "-o", str(out)])
+    return out
+
+
 def build_oracle(with_ref: bool = True) -> None:
     targets = ["all"]
     if with_ref and Path("/root/reference/Utilities/solvers/cpu/pcg_solver.cpp").exists():
@@ -69,6 +83,7 @@ def main(argv: list[str]) -> int:
     build_lib()
     if "--lib" not in argv:
         build_oracle()
+        build_binding_check()
     return 0
 
 
