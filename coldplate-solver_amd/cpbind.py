@@ -183,6 +183,7 @@ EXPORTS = [
     ("cp_serve", C.c_int, [_vp, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     ("cp_serve_pool", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     ("cp_batch_autotune", C.c_int, [_vp, C.c_int, _ip, _ip, _ip, _dp]),
+    ("cp_run_noalloc", C.c_int, [C.POINTER(Params), _dp, _dp, _dp, _dp, C.POINTER(StepStats)]),
     ("cp_io_read_params", C.c_int, [C.c_char_p, C.POINTER(Params), _ip, _ip]),
     ("cp_io_read_geometry", C.c_int, [C.c_char_p, C.c_int, C.c_int, _dp]),
     ("cp_io_read_matrix", C.c_int, [C.c_char_p, C.c_int, C.c_int, _dp]),

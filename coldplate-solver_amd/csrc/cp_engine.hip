@@ -250,6 +250,11 @@ struct cp_batch {
     }
     // replay the captured step graph (1) or launch its kernels directly (0)
     int useGraph = 1;
+    // host mirrors of a batch laid out in the static device scratch (no
+    // pinned allocations): hState / hPlaneReq / hPlaneOut point here
+    SlotState staticState;
+    PlaneReq staticPlaneReq[2];
+    PlaneOut staticPlaneOut[2];
 
     size_t tElem() const { return precision == CP_PREC_MIXED ? sizeof(float) : sizeof(double); }
     size_t bytesPerNode() const {
@@ -1796,6 +1801,157 @@ int cp_sync(cp_ctx* ctx) {
     if (!ctx) return setError(nullptr, CP_E_ARG, "null ctx");
     std::string& err = ctx->b->err;
     HIP_TRY(hipStreamSynchronize(ctx->b->stream));
+    return CP_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// the solve that allocates nothing: cp_run_noalloc
+// ---------------------------------------------------------------------------
+// Device scratch of cp_run_noalloc: one single-slot batch (descriptor, slot
+// state, tile lists, reduction partials, plane buffers and the slot's arena)
+// is laid out in it per call. Every device holds its own copy for the life
+// of the process. In host code `scratch` names no device memory: the host
+// uses the address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) char scratch[CP_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratchMu;
+char* g_scratchPtr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on device `dev` (current), looked up once.
+int scratchOf(int dev, char** out) {
+    if (dev < 0 || dev >= kMaxScratchDevices)
+        return setError(nullptr, CP_E_NODEV, "device ordinal %d beyond the %d the scratch table covers", dev,
+                        kMaxScratchDevices);
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    if (!g_scratchPtr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return setError(nullptr, CP_E_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        g_scratchPtr[dev] = static_cast<char*>(ptr);
+    }
+    *out = g_scratchPtr[dev];
+    return CP_OK;
+}
+
+// Byte offsets of a one-slot batch in the scratch (256-byte aligned parts).
+struct ScratchLayout {
+    size_t desc, state, nTiles, pnTiles, mgnTiles, tileSlot, ptileSlot, mgTileSlot, partials, planeReq, planeOut;
+    size_t arena, arenaBytes, total, maxTiles;
+};
+
+ScratchLayout scratchLayout(const SlotDesc& d, size_t footprint) {
+    ScratchLayout L;
+    size_t mt = size_t(std::max(d.ntiles, d.pntiles));
+    for (int l = 0; l < d.nLevels; ++l) mt = std::max(mt, size_t(d.lv[l].ntiles));
+    L.maxTiles = mt;
+    size_t off = 0;
+    auto take = [&](size_t n) {
+        const size_t o = off;
+        off = roundUp(off + n, 256);
+        return o;
+    };
+    L.desc = take(sizeof(SlotDesc));
+    L.state = take(sizeof(SlotState));
+    L.nTiles = take(sizeof(int));
+    L.pnTiles = take(sizeof(int));
+    L.mgnTiles = take(sizeof(int) * kMaxLevels);
+    L.tileSlot = take(sizeof(int) * mt);
+    L.ptileSlot = take(sizeof(int) * mt);
+    L.mgTileSlot = take(sizeof(int) * mt * kMaxLevels);
+    L.partials = take(sizeof(double) * kNumPartials * mt);
+    L.planeReq = take(sizeof(PlaneReq) * 2);
+    L.planeOut = take(sizeof(PlaneOut) * 2);
+    L.arenaBytes = roundUp(footprint, 256);
+    L.arena = take(L.arenaBytes);
+    L.total = off;
+    return L;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cp_run_noalloc(const cp_params* prm, const double* cellType, double* u, double* v, double* p,
+                   cp_step_stats* last) {
+    std::string perr;
+    int rc = validate(prm, &perr);
+    if (rc) return rc;
+    if (!cellType) return setError(nullptr, CP_E_ARG, "null cellType");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || !isGfx950(dev))
+        return setError(nullptr, CP_E_NODEV, "the current HIP device is not a gfx950 (MI355X) device");
+    // a single-slot batch exactly as cp_create makes it, with its device
+    // parts in the scratch, its host mirrors in the object, and the null
+    // stream
+    cp_batch b;
+    std::string& err = b.err;
+    b.precision = prm->precision;
+    b.maxSlots = 1;
+    b.device = dev;
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    b.numCU = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    b.stream = nullptr;
+    b.tileRows = defaultTileRows(prm->M, prm->N);
+    const SlotDesc probe = makeDesc(*prm, b.precision, b.tileRows, b.mgRowsEff());
+    const ScratchLayout L = scratchLayout(probe, b.footprint(prm->M, prm->N));
+    if (L.total > size_t(CP_NOALLOC_SCRATCH_BYTES))
+        return setError(nullptr, CP_E_CAPACITY,
+                        "cp_run_noalloc: a %d x %d grid needs %zu bytes of device scratch, more than the %zu "
+                        "of CP_NOALLOC_SCRATCH_BYTES",
+                        prm->M, prm->N, L.total, size_t(CP_NOALLOC_SCRATCH_BYTES));
+    char* base = nullptr;
+    if ((rc = scratchOf(dev, &base))) return rc;
+    b.dDesc = reinterpret_cast<SlotDesc*>(base + L.desc);
+    b.dState = reinterpret_cast<SlotState*>(base + L.state);
+    b.dNTiles = reinterpret_cast<int*>(base + L.nTiles);
+    b.dPNTiles = reinterpret_cast<int*>(base + L.pnTiles);
+    b.dMgNTiles = reinterpret_cast<int*>(base + L.mgnTiles);
+    b.dTileSlot = reinterpret_cast<int*>(base + L.tileSlot);
+    b.dPTileSlot = reinterpret_cast<int*>(base + L.ptileSlot);
+    b.dMgTileSlot = reinterpret_cast<int*>(base + L.mgTileSlot);
+    b.dPartials = reinterpret_cast<double*>(base + L.partials);
+    b.dPlaneReq = reinterpret_cast<PlaneReq*>(base + L.planeReq);
+    b.dPlaneOut = reinterpret_cast<PlaneOut*>(base + L.planeOut);
+    b.maxTiles = L.maxTiles;
+    b.arenaBase = base + L.arena;
+    b.arena.init(b.arenaBase, L.arenaBytes);
+    b.hState = &b.staticState;
+    b.hPlaneReq = b.staticPlaneReq;
+    b.hPlaneOut = b.staticPlaneOut;
+    memset(b.hState, 0, sizeof(SlotState));
+    b.hDesc.assign(1, SlotDesc());
+    memset(b.hDesc.data(), 0, sizeof(SlotDesc));
+    b.slots.assign(1, SlotHost());
+    // the previous call's contents stay in the scratch: the slot state and
+    // the tile counts start from zero here; admission zeroes the arena
+    // region and writes the descriptor and the tile lists
+    const int zeros[kMaxLevels] = {};
+    HIP_TRY(hipMemcpyAsync(b.dState, b.hState, sizeof(SlotState), hipMemcpyHostToDevice, b.stream));
+    HIP_TRY(hipMemcpyAsync(b.dNTiles, zeros, sizeof(int), hipMemcpyHostToDevice, b.stream));
+    HIP_TRY(hipMemcpyAsync(b.dPNTiles, zeros, sizeof(int), hipMemcpyHostToDevice, b.stream));
+    HIP_TRY(hipMemcpyAsync(b.dMgNTiles, zeros, sizeof(zeros), hipMemcpyHostToDevice, b.stream));
+    HIP_TRY(hipStreamSynchronize(b.stream));
+    int slot = -1;
+    if ((rc = batchAdmit(&b, prm, cellType, &slot))) return rc;
+    if ((u || v || p) && (rc = setFieldsSlot(&b, slot, u, v, p))) return rc;
+    // cp_run's loop
+    if ((rc = pollState(&b))) return rc;
+    while (b.hState[slot].status == ST_RUNNING) {
+        if ((rc = polledStep(&b, true))) return rc;
+        if ((rc = pollState(&b))) return rc;
+    }
+    if (u && (rc = getFieldSlot(&b, slot, CP_F_U, u))) return rc;
+    if (v && (rc = getFieldSlot(&b, slot, CP_F_V, v))) return rc;
+    if (p && (rc = getFieldSlot(&b, slot, CP_F_P, p))) return rc;
+    fillStats(b.hState[slot], b.hDesc[slot], last);
+    HIP_TRY(hipStreamSynchronize(b.stream));
     return CP_OK;
 }
 

@@ -397,6 +397,31 @@ int cp_serve_pool(cp_batch* const* batches, int nbatches, const cp_request* reqs
                   cp_result* results);
 
 /* ------------------------------------------------------------------ */
+/* a solve that allocates no device memory                             */
+/* ------------------------------------------------------------------ */
+/* Device scratch of cp_run_noalloc: a static array of this many bytes that
+   every device the library runs on holds until the process ends. */
+#define CP_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/* No counterpart in the reference. For callers that cannot allocate device
+   memory (tight serving loops, harnesses that forbid hipMalloc): cp_create,
+   cp_set_fields, cp_run and cp_get_fields in one call that calls no
+   hipMalloc, hipHostMalloc, hipFree, stream or graph creation. The solve's
+   descriptor, slot state, tile lists, reduction partials and arrays live in
+   the static device scratch of the current device, with the tiling cp_create
+   chooses, so the results are those of cp_run on a context, bit for bit.
+   u, v, p (reference shapes, row-major, host): the initial fields when
+   non-NULL (else zero), overwritten with the final ones. A grid whose solve
+   does not fit CP_NOALLOC_SCRATCH_BYTES gets CP_E_CAPACITY (an FP64 solve
+   needs about 300 bytes per node of the (M+1) x (N+1) grid). Runs on the
+   null stream and returns when the solve has ended (error <= epsilon,
+   maxIterations, or a non-finite field: health in *last). Calls that
+   overlap on one device share the scratch and race; the caller serialises
+   them. Each call clears what it reads before writing it. */
+int cp_run_noalloc(const cp_params* prm, const double* cellType, double* u, double* v, double* p,
+                   cp_step_stats* last);
+
+/* ------------------------------------------------------------------ */
 /* the reference's file formats (host only, no device needed)          */
 /* ------------------------------------------------------------------ */
 /* Parameter file "M N hy hx U_in N_in_buffer N_out_buffer [Ht [rho nu]]"
