@@ -26,6 +26,7 @@ F_U, F_V, F_P, F_USTAR, F_VSTAR, F_PSTAR, F_DE, F_DN, F_GAMMA, F_ALPHA = range(1
 
 SLOT_FREE, SLOT_RUNNING, SLOT_CONVERGED, SLOT_MAXITER, SLOT_FAILED = range(5)
 
+K_MOM_ASM = 0
 K_PCG_Q = 4
 
 
@@ -191,7 +192,7 @@ EXPORTS = [
     ("cp_io_save_all", C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                  _dp, _dp, _dp, _dp, _dp, _dp, C.c_int]),
     ("cp_batch_probe", C.c_int, [_vp, C.c_int]),
-    ("cp_batch_probe_read", C.c_int, [_vp, _dp, _dp, C.c_int]),
+    ("cp_batch_probe_read", C.c_int, [_vp, _ip, _ip, _dp, _dp, C.c_int]),
     ("cp_batch_probe_burst", C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     ("cp_geom_straight", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
     ("cp_geom_network", C.c_int, [C.c_int, C.c_int, C.c_uint64, _dp]),
@@ -437,17 +438,25 @@ class Batch:
     def set_mg_tiling(self, rows: int) -> None:
         check(load().cp_batch_set_mg_tiling(self.h, rows), self.h)
 
-    def probe(self, kernel: int) -> None:
-        check(load().cp_batch_probe(self.h, kernel), self.h)
+    def probe(self, kernels=()) -> None:
+        """Time the kernels (K_PCG_Q, K_MOM_ASM) in bursts during serve."""
+        mask = 0
+        for k in kernels:
+            mask |= 1 << k
+        check(load().cp_batch_probe(self.h, mask), self.h)
 
-    def probe_read(self, cap: int = 65536) -> tuple[np.ndarray, np.ndarray]:
-        """Timing bursts of the last serve: (ms per launch, p' unknowns covered)."""
+    def probe_read(self, cap: int = 65536) -> list[dict]:
+        """Timing bursts of the last serve: kernel, launches, elapsed ms and
+        work units over the burst's launches."""
+        kern = np.zeros(cap, np.int32)
+        nl = np.zeros(cap, np.int32)
         ms = np.zeros(cap)
         un = np.zeros(cap)
-        n = load().cp_batch_probe_read(self.h, dptr(ms), dptr(un), cap)
+        n = load().cp_batch_probe_read(self.h, kern.ctypes.data_as(_ip), nl.ctypes.data_as(_ip), dptr(ms), dptr(un),
+                                       cap)
         if n < 0:
             check(n, self.h)
-        return ms[:n].copy(), un[:n].copy()
+        return [dict(kernel=int(kern[k]), launches=int(nl[k]), ms=float(ms[k]), units=float(un[k])) for k in range(n)]
 
     def probe_burst(self, launches: int, kernel: int = K_PCG_Q) -> float:
         """Mean ms per launch of `launches` back-to-back k_pcg_q launches."""

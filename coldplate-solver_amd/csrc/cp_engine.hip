@@ -234,11 +234,15 @@ struct cp_batch {
     // graph mode
     hipGraphExec_t graphExec = nullptr;
     int graphJ = -1, graphP = -1, graphR = -1, graphL = -1;
-    // measurement: k_pcg_q timing bursts inside cp_serve (every kProbeEvery
-    // polls), as (ms per launch, p' unknowns of the running slots)
-    int probeKernel = -1;
-    std::vector<hipEvent_t> burstEv;  // event pairs of the timing bursts
-    std::vector<double> probeMs, probeUnknowns;
+    // measurement: timing bursts inside cp_serve (every kProbeEvery polls)
+    // of the kernels in probeMask (bit k: kernel k), one record per burst
+    int probeMask = 0;
+    std::vector<hipEvent_t> burstEv;  // event pair of the timing bursts
+    struct ProbeRec {
+        int kernel, launches;
+        double ms, units;  // whole burst: elapsed time, work units over all launches
+    };
+    std::vector<ProbeRec> probes;
     // tile height R (rows per wave) shared by every slot: 1, 2, 4 or 8
     int tileRows = 4;
     // workgroups per CU of the replayed graph's launches
@@ -1083,6 +1087,10 @@ int graphRun(cp_batch* b, int steps) {
     const int shapeL = ((b->mgLevels * (kMaxLevels + 1) + b->mgTail) * 2 + b->dpAny) * 16 + families;
     if (!b->graphExec || J != b->graphJ || P != b->graphP || R != b->graphR || shapeL != b->graphL) {
         if (b->graphExec) {
+            // a replay of the old graph may still be in flight (a tiling
+            // change that only sets the workgroups per CU does not
+            // synchronise): let it finish before its executable goes
+            HIP_TRY(hipStreamSynchronize(b->stream));
             HIP_TRY(hipGraphExecDestroy(b->graphExec));
             b->graphExec = nullptr;
         }
@@ -1312,10 +1320,10 @@ int cp_batch_set_tiling(cp_batch* b, int tile_rows, int wg_per_cu) {
 }
 
 constexpr int kProbeEvery = 10;     // polls between timing bursts
-constexpr int kProbeLaunches = 16;  // k_pcg_q launches per burst
+constexpr int kProbeLaunches = 16;  // rounds of launches per burst
 
 namespace {
-int burst(cp_batch* b, int launches, double* ms_per_launch);
+int burst(cp_batch* const* bs, int nb, Kern k, int rounds, double* ms, double* units, int* launches);
 }
 
 }  // extern "C"
@@ -1329,6 +1337,8 @@ namespace {
 // in flight: every worker holds it shared while it has device work queued
 // (admission, step graph, poll, read-out), a burst holds it exclusively.
 struct ServeQueue {
+    cp_batch* const* batches = nullptr;  // every batch serving the queue
+    int nbatches = 0;
     const cp_request* reqs = nullptr;
     cp_result* results = nullptr;
     int n = 0;
@@ -1372,8 +1382,7 @@ struct ServeQueue {
 int serveWorker(cp_batch* b, ServeQueue& q, int steps_per_poll) {
     if (hipSetDevice(b->device) != hipSuccess) return setError(&b->err, CP_E_HIP, "hipSetDevice failed");
     std::vector<int> reqOf(b->maxSlots, -1);
-    b->probeMs.clear();
-    b->probeUnknowns.clear();
+    b->probes.clear();
     long polls = 0;
     int running = 0;
     while (!q.abort.load()) {
@@ -1431,22 +1440,19 @@ int serveWorker(cp_batch* b, ServeQueue& q, int steps_per_poll) {
             --running;
         }
         q.leaveShared();
-        if (!rc && b->probeKernel == int(K_PCG_Q) && polls++ % kProbeEvery == 0) {
-            // timing burst of the PCG stencil kernel over the running slots,
-            // with the GPU to itself
-            double ms = 0.0, unknowns = 0.0;
-            for (int s = 0; s < b->maxSlots; ++s)
-                if (reqOf[s] >= 0 && b->hState[s].status == ST_RUNNING)
-                    unknowns += double(b->slots[s].M - 1) * double(b->slots[s].N - 2);
-            if (unknowns > 0) {
-                q.enterExclusive();
-                rc = burst(b, kProbeLaunches, &ms);
-                q.leaveExclusive();
-                if (!rc) {
-                    b->probeMs.push_back(ms);
-                    b->probeUnknowns.push_back(unknowns);
-                }
+        if (!rc && b->probeMask && polls++ % kProbeEvery == 0) {
+            // timing bursts of the probed kernels, each launch round covering
+            // the running slots of every batch of the queue in turn, with the
+            // GPU to itself (the other workers' streams are idle)
+            q.enterExclusive();
+            for (int k = 0; !rc && k < 32; ++k) {
+                if (!(b->probeMask & (1 << k))) continue;
+                double ms = 0.0, units = 0.0;
+                int launches = 0;
+                rc = burst(q.batches, q.nbatches, Kern(k), kProbeLaunches, &ms, &units, &launches);
+                if (!rc && launches > 0) b->probes.push_back({k, launches, ms, units});
             }
+            q.leaveExclusive();
         }
         if (rc) {
             q.abort.store(true);
@@ -1465,6 +1471,8 @@ int serveChecked(cp_batch* const* bs, int nb, const cp_request* reqs, int n, int
             if (bs[i]->slots[s].used) return setError(&err, CP_E_STATE, "cp_serve needs an empty batch");
     }
     ServeQueue q;
+    q.batches = bs;
+    q.nbatches = nb;
     q.reqs = reqs;
     q.results = results;
     q.n = n;
@@ -1598,20 +1606,31 @@ int cp_batch_set_graph(cp_batch* b, int use_graph) {
     return CP_OK;
 }
 
-int cp_batch_probe(cp_batch* b, int kernel) {
+// Kernels the timing bursts can time: each recomputes its outputs from
+// inputs no launch of it changes, between two outer iterations (k_pcg_q: the
+// search direction and q = A p; k_mom_assemble: the momentum coefficients,
+// r0 and the first sweep, which the next outer iteration recomputes before
+// any reader), so a burst leaves every solve as it was.
+static bool probeable(int k) { return k == int(K_PCG_Q) || k == int(K_MOM_ASM); }
+
+int cp_batch_probe(cp_batch* b, int kernel_mask) {
     if (!b) return setError(nullptr, CP_E_ARG, "null batch");
-    if (kernel != -1 && kernel != int(K_PCG_Q))
-        return setError(&b->err, CP_E_ARG, "timing bursts are defined for CP_K_PCG_Q only (an idempotent kernel)");
-    b->probeKernel = kernel;
+    if (kernel_mask & ~((1 << CP_K_PCG_Q) | (1 << CP_K_MOM_ASM)))
+        return setError(&b->err, CP_E_ARG,
+                        "timing bursts are defined for CP_K_PCG_Q and CP_K_MOM_ASM only (mask 0x%x)", kernel_mask);
+    b->probeMask = kernel_mask;
     return CP_OK;
 }
 
-int cp_batch_probe_read(cp_batch* b, double* ms, double* unknowns, int cap) {
+int cp_batch_probe_read(cp_batch* b, int* kernel, int* launches, double* ms, double* units, int cap) {
     if (!b || !ms) return setError(nullptr, CP_E_ARG, "null argument");
     int n = 0;
-    for (; n < int(b->probeMs.size()) && n < cap; ++n) {
-        ms[n] = b->probeMs[n];
-        if (unknowns) unknowns[n] = b->probeUnknowns[n];
+    for (; n < int(b->probes.size()) && n < cap; ++n) {
+        const auto& r = b->probes[n];
+        if (kernel) kernel[n] = r.kernel;
+        if (launches) launches[n] = r.launches;
+        ms[n] = r.ms;
+        if (units) units[n] = r.units;
     }
     return n;
 }
@@ -1619,37 +1638,78 @@ int cp_batch_probe_read(cp_batch* b, double* ms, double* unknowns, int cap) {
 int cp_batch_probe_burst(cp_batch* b, int kernel, int launches, double* ms_per_launch) {
     if (!b || !ms_per_launch) return setError(nullptr, CP_E_ARG, "null argument");
     std::string& err = b->err;
-    if (kernel != int(K_PCG_Q))
-        return setError(&err, CP_E_ARG, "timing bursts are defined for CP_K_PCG_Q only (an idempotent kernel)");
+    if (!probeable(kernel))
+        return setError(&err, CP_E_ARG, "timing bursts are defined for CP_K_PCG_Q and CP_K_MOM_ASM only");
     if (launches < 1 || launches > 4096) return setError(&err, CP_E_ARG, "launches must be in [1, 4096]");
-    return burst(b, launches, ms_per_launch);
+    double ms = 0.0, units = 0.0;
+    int n = 0;
+    const int rc = burst(&b, 1, Kern(kernel), launches, &ms, &units, &n);
+    if (rc) return rc;
+    *ms_per_launch = n > 0 ? ms / n : 0.0;
+    return CP_OK;
 }
 
 namespace {
-// One pair of events around `launches` back-to-back launches: the average
-// is the kernel's duration plus the (small) dispatch gap between
-// consecutive launches on one queue, as a timing loop measures it.
-int burst(cp_batch* b, int launches, double* ms_per_launch) {
-    std::string& err = b->err;
-    while (b->burstEv.size() < 2) {
+// Work units of kernel k over a batch's running slots: p' unknowns of the CG
+// slots for k_pcg_q, nodes of the (M+1) x (N+1) grid of the implicit-momentum
+// slots for k_mom_assemble.
+double burstUnits(const cp_batch* b, Kern k) {
+    double u = 0.0;
+    for (int s = 0; s < b->maxSlots; ++s) {
+        if (!b->slots[s].used || b->hState[s].status != ST_RUNNING) continue;
+        const SlotDesc& d = b->hDesc[s];
+        if (k == K_PCG_Q && d.presType == 1) u += double(d.M - 1) * double(d.N - 2);
+        if (k == K_MOM_ASM && d.momType == 1) u += double(d.M + 1) * double(d.N + 1);
+    }
+    return u;
+}
+
+// `rounds` rounds of back-to-back launches of kernel k, each round one
+// launch per batch with running slots, all on the first batch's stream,
+// between one pair of events: the time per launch is the kernel's duration
+// plus the dispatch gap between consecutive launches on one queue, as a
+// timing loop measures it. With several batches the rounds rotate over all
+// their arrays, so the working set is the pool's, not one batch's. Every
+// batch's stream must be idle.
+int burst(cp_batch* const* bs, int nb, Kern k, int rounds, double* ms, double* units, int* launches) {
+    cp_batch* b0 = bs[0];
+    std::string& err = b0->err;
+    while (b0->burstEv.size() < 2) {
         hipEvent_t ev = nullptr;
         HIP_TRY(hipEventCreate(&ev));
-        b->burstEv.push_back(ev);
+        b0->burstEv.push_back(ev);
     }
-    KArgs a = kargsFor(b, K_PCG_Q);
-    a.force = 1;
-    const int grid = gridFor(b, 1 << 30);
-    const bool f32 = b->precision == CP_PREC_MIXED;
-    HIP_TRY(hipEventRecord(b->burstEv[0], b->stream));
-    for (int k = 0; k < launches; ++k) {
-        const hipError_t e = launchWorkOnly(K_PCG_Q, f32, pRows(f32, b->tileRows), grid, b->stream, a);
-        if (e != hipSuccess) return setError(&err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+    double perRound = 0.0;
+    int nl = 0;
+    for (int i = 0; i < nb; ++i) {
+        const double u = burstUnits(bs[i], k);
+        perRound += u;
+        nl += u > 0.0 ? 1 : 0;
     }
-    HIP_TRY(hipEventRecord(b->burstEv[1], b->stream));
-    HIP_TRY(hipEventSynchronize(b->burstEv[1]));
+    *ms = 0.0;
+    *units = 0.0;
+    *launches = 0;
+    if (nl == 0) return CP_OK;
+    HIP_TRY(hipEventRecord(b0->burstEv[0], b0->stream));
+    for (int r = 0; r < rounds; ++r) {
+        for (int i = 0; i < nb; ++i) {
+            cp_batch* b = bs[i];
+            if (burstUnits(b, k) <= 0.0) continue;
+            KArgs a = kargsFor(b, k);
+            a.force = 1;
+            const bool f32 = b->precision == CP_PREC_MIXED;
+            const int rows = pOnly(k) ? pRows(f32, b->tileRows) : b->tileRows;
+            const hipError_t e = launchWorkOnly(k, f32, rows, gridFor(b, 1 << 30), b0->stream, a);
+            if (e != hipSuccess) return setError(&err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+        }
+    }
+    HIP_TRY(hipEventRecord(b0->burstEv[1], b0->stream));
+    HIP_TRY(hipEventSynchronize(b0->burstEv[1]));
     float t = 0.f;
-    HIP_TRY(hipEventElapsedTime(&t, b->burstEv[0], b->burstEv[1]));
-    *ms_per_launch = double(t) / launches;
+    HIP_TRY(hipEventElapsedTime(&t, b0->burstEv[0], b0->burstEv[1]));
+    *ms = double(t);
+    *units = perRound * rounds;
+    *launches = nl * rounds;
     return CP_OK;
 }
 }  // namespace

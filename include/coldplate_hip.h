@@ -327,21 +327,28 @@ int cp_batch_set_mg_tiling(cp_batch* b, int rows);
    CU and ms per outer iteration. */
 int cp_batch_autotune(cp_batch* b, int steps, int* rows, int* mg_rows, int* wg_per_cu, double* ms_per_step);
 
-/* Kernel of the PCG iteration timed by the measurement hooks below. */
-#define CP_K_PCG_Q        4
+/* Kernels the measurement hooks below can time (cp_batch_probe takes a mask
+   of 1 << CP_K_*). */
+#define CP_K_MOM_ASM      0   /* momentum assembly with the first Jacobi sweep */
+#define CP_K_PCG_Q        4   /* PCG: search direction and q = A p */
 
 /* Measurement hooks (no counterpart in the reference, which has no device
-   timing). cp_batch_probe(b, CP_K_PCG_Q) makes cp_serve time the PCG
-   stencil kernel every 10 polls: a burst of 16 back-to-back launches over
-   the running slots between two HIP events on the batch stream (the
-   kernel recomputes the search direction and q = A p from unchanged
-   inputs, so a burst leaves every solve as it was); -1 disables.
-   cp_batch_probe_read returns the bursts of the last cp_serve, oldest
-   first: ms per launch, and the p' unknowns the launches covered
-   (unknowns may be NULL); the return value is their count.
-   cp_batch_probe_burst runs one such burst now and synchronises. */
-int cp_batch_probe(cp_batch* b, int kernel);
-int cp_batch_probe_read(cp_batch* b, double* ms, double* unknowns, int cap);
+   timing). cp_batch_probe(b, mask) makes cp_serve / cp_serve_pool (with b
+   as the first batch) time each kernel of the mask every 10 polls: a burst
+   of 16 rounds of back-to-back launches between two HIP events on b's
+   stream, with the GPU to itself; each round launches the kernel once per
+   batch serving the queue, over that batch's running slots, so with several
+   batches the rounds rotate over all of their arrays. Both kernels
+   recompute their outputs from inputs no launch of theirs changes, between
+   two outer iterations, so a burst leaves every solve as it was. mask 0
+   disables. cp_batch_probe_read returns the bursts of the last serve,
+   oldest first: kernel, launches, elapsed ms, and work units over all its
+   launches (p' unknowns for CP_K_PCG_Q, (M+1) x (N+1) grid nodes for
+   CP_K_MOM_ASM); any output but ms may be NULL; the return value is their
+   count. cp_batch_probe_burst runs one burst of `launches` launches on b
+   alone now, synchronises, and gives the mean ms per launch. */
+int cp_batch_probe(cp_batch* b, int kernel_mask);
+int cp_batch_probe_read(cp_batch* b, int* kernel, int* launches, double* ms, double* units, int cap);
 int cp_batch_probe_burst(cp_batch* b, int kernel, int launches, double* ms_per_launch);
 
 /* ------------------------------------------------------------------ */

@@ -1,17 +1,21 @@
 """Writes tests/golden/network_1024x512_dp.json: converged full-system
 pressure drops of the 1024 x 512 channel-network workload (seeds 0..7).
 
-Two sources, recorded side by side:
+Sources, recorded side by side:
   * "dp_oracle": the CPU FP64 oracle (oracle/simple_oracle.hpp) run to a
     fixed point (mass residual over the p' columns, epsilon 1e-9, PCG cap
-    300 at tolerance 1e-10);
+    300 at tolerance 1e-10); the bench's reference for every seed;
   * "dp_fp64_gpu": the device library in FP64 with the same settings (its
-    per-step parity with the oracle is tested in tests/test_gpu_parity.py).
-The bench uses dp_oracle where present, else dp_fp64_gpu; the test suite
-checks that the two agree wherever both exist.
+    per-step parity with the oracle is tested in tests/test_gpu_parity.py);
+    the test suite checks that the two agree;
+  * "iters_float_oracle": outer iterations the oracle in float arithmetic
+    (the reference's arithmetic, algorithm and inner budgets) needs to reach
+    the bench's stop test (epsilon 1e-7 over the p' columns); the CPU
+    baseline scales its timed sample by it.
 
     python tests/golden/make_network_refs.py --gpu          # needs an MI355X
     python tests/golden/make_network_refs.py --oracle 0 1   # CPU, ~10 min/seed
+    python tests/golden/make_network_refs.py --float 0 1    # CPU, a few min/seed
 """
 from __future__ import annotations
 
@@ -83,16 +87,38 @@ def run_oracle(d: dict, seeds: list[int]) -> None:
         OUT.write_text(json.dumps(d, indent=1))
 
 
+def run_float(d: dict, seeds: list[int], eps: float = 1e-7, cap: int = 40000) -> None:
+    import oracle_py as op
+    import workloads as wl
+
+    for s in seeds:
+        p, ct = wl.network_1024x512(s)
+        p.massResidualScope = 1
+        p.epsilon = eps
+        p.maxIterations = cap
+        o = op.Oracle(p, ct, fp32=True)
+        t = time.time()
+        n, st, _ = o.run(cap)
+        e = d["seeds"].setdefault(str(s), {})
+        e["iters_float_oracle"] = st["iter"] if st["error"] <= eps else None
+        e["dp_float_oracle"] = o.full_dp(GAMMA_MIN)
+        print(s, e, f"{time.time() - t:.0f}s", flush=True)
+        OUT.write_text(json.dumps(d, indent=1))
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpu", action="store_true")
     ap.add_argument("--oracle", type=int, nargs="*")
+    ap.add_argument("--float", type=int, nargs="*", dest="float_seeds")
     a = ap.parse_args()
     d = load()
     if a.gpu:
         run_gpu(d)
     if a.oracle:
         run_oracle(d, a.oracle)
+    if a.float_seeds:
+        run_float(d, a.float_seeds)
     OUT.write_text(json.dumps(d, indent=1))
     return 0
 

@@ -148,12 +148,13 @@ def test_tile_heights_agree_with_oracle(rows):
         assert np.abs(a - ref).max() <= 1e-8 * np.abs(ref).max(), f
 
 
-def test_timing_burst_leaves_results_unchanged():
-    """k_pcg_q timing bursts between replays (as bench.py runs them) do not
-    change any slot's results."""
+@pytest.mark.parametrize("kernel", [cb.K_PCG_Q, cb.K_MOM_ASM], ids=["pcg_q", "mom_assemble"])
+def test_timing_burst_leaves_results_unchanged(kernel):
+    """Timing bursts of k_pcg_q or k_mom_assemble between replays (as
+    bench.py runs them) do not change any slot's results."""
     p, ct = wl.network(64, 128, 17, cb.CP_PREC_MIXED)
     q = wl.fast_mode(p)
-    say("[batch] timing burst")
+    say(f"[batch] timing burst of kernel {kernel}")
     runs = []
     for burst in (False, True):
         b = cb.Batch(cb.CP_PREC_MIXED, 64 << 20, 2)
@@ -161,7 +162,7 @@ def test_timing_burst_leaves_results_unchanged():
         for _ in range(4):
             b.run(5)
             if burst:
-                assert b.probe_burst(4) > 0.0
+                assert b.probe_burst(4, kernel) > 0.0
         runs.append(b.fields(s))
         b.close()
     for x, y in zip(*runs):

@@ -94,9 +94,16 @@ def test_pool_results_equal_alone_and_streams_overlap():
     say("[serve] pool of 2 batches x 1 slot, 5 requests")
     bs = [cb.Batch(cb.CP_PREC_MIXED, 128 << 20, 1) for _ in range(2)]
     for b in bs:
-        b.probe(cb.K_PCG_Q)  # timing bursts take the GPU alone; results unchanged
+        # timing bursts of both probed kernels take the GPU alone and rotate
+        # over both batches; results unchanged
+        b.probe((cb.K_PCG_Q, cb.K_MOM_ASM))
     mix = cb.serve_pool(bs, reqs, steps_per_poll=10)
-    bursts = [len(b.probe_read()[0]) for b in bs]
+    recs = bs[0].probe_read()
+    bursts = [len(b.probe_read()) for b in bs]
+    assert {r["kernel"] for r in recs} == {cb.K_PCG_Q, cb.K_MOM_ASM}
+    assert all(r["ms"] > 0 and r["units"] > 0 and r["launches"] >= 16 for r in recs)
+    # a round launches once per batch with running slots: both, while both run
+    assert max(r["launches"] for r in recs) == 32
     for b in bs:
         b.close()
     for k, r in enumerate(mix):
