@@ -266,33 +266,35 @@ __device__ __forceinline__ T fromEast(T x) { return __shfl_down(x, 1, 64); }
 // ---------------------------------------------------------------------------
 // Second-order upwind deferred correction (convection.cpp:53-186). The index
 // guards are the reference's, including its U/V asymmetry.
-template <bool IsU, typename Ptr>
-__device__ __forceinline__ double souCorrection(Ptr f, size_t k, int i, int j, int P, int M, int N,
-                                                double Fe, double Fw, double Fn, double Fs) {
-    double S = 0.0;
+template <bool IsU, typename Ptr, typename F>
+__device__ __forceinline__ F souCorrection(Ptr f, size_t k, int i, int j, int P, int M, int N, F Fe, F Fw, F Fn,
+                                           F Fs) {
+    F S = F(0);
     const bool eOK = IsU ? (j + 2 < N) : (j + 2 <= N);
     const bool wOK = IsU ? (j + 1 < N) : (j + 1 <= N);
     const bool nOK = IsU ? (i + 2 <= M) : (i + 2 < M);
     const bool sOK = IsU ? (i + 1 <= M) : (i + 1 < M);
-    if (Fe >= 0.0) {
-        if (j >= 2) S -= Fe * ((1.5 * f[k] - 0.5 * f[k - 1]) - f[k]);
+    // the face differences in the precision of the field, the products in
+    // that of the fluxes
+    if (Fe >= F(0)) {
+        if (j >= 2) S -= Fe * F((1.5 * f[k] - 0.5 * f[k - 1]) - f[k]);
     } else {
-        if (eOK) S -= Fe * ((1.5 * f[k + 1] - 0.5 * f[k + 2]) - f[k + 1]);
+        if (eOK) S -= Fe * F((1.5 * f[k + 1] - 0.5 * f[k + 2]) - f[k + 1]);
     }
-    if (Fw >= 0.0) {
-        if (j >= 2) S += Fw * ((1.5 * f[k - 1] - 0.5 * f[k - 2]) - f[k - 1]);
+    if (Fw >= F(0)) {
+        if (j >= 2) S += Fw * F((1.5 * f[k - 1] - 0.5 * f[k - 2]) - f[k - 1]);
     } else {
-        if (wOK) S += Fw * ((1.5 * f[k] - 0.5 * f[k + 1]) - f[k]);
+        if (wOK) S += Fw * F((1.5 * f[k] - 0.5 * f[k + 1]) - f[k]);
     }
-    if (Fn >= 0.0) {
-        if (i >= 1) S -= Fn * ((1.5 * f[k] - 0.5 * f[k - P]) - f[k]);
+    if (Fn >= F(0)) {
+        if (i >= 1) S -= Fn * F((1.5 * f[k] - 0.5 * f[k - P]) - f[k]);
     } else {
-        if (nOK) S -= Fn * ((1.5 * f[k + P] - 0.5 * f[k + 2 * size_t(P)]) - f[k + P]);
+        if (nOK) S -= Fn * F((1.5 * f[k + P] - 0.5 * f[k + 2 * size_t(P)]) - f[k + P]);
     }
-    if (Fs >= 0.0) {
-        if (i >= 2) S += Fs * ((1.5 * f[k - P] - 0.5 * f[k - 2 * size_t(P)]) - f[k - P]);
+    if (Fs >= F(0)) {
+        if (i >= 2) S += Fs * F((1.5 * f[k - P] - 0.5 * f[k - 2 * size_t(P)]) - f[k - P]);
     } else {
-        if (sOK) S += Fs * ((1.5 * f[k] - 0.5 * f[k + P]) - f[k]);
+        if (sOK) S += Fs * F((1.5 * f[k] - 0.5 * f[k + P]) - f[k]);
     }
     return S;
 }
@@ -388,6 +390,77 @@ __device__ __forceinline__ T momRow(const MomConst<T>& d, const double* u, const
     return r0t;
 }
 
+// The mixed-precision form of momRow (T = float). Everything that enters
+// the residual r0 = b - A u is computed in double, as in the FP64 row, so
+// that delta = 0 (the outer fixed point) means the momentum equations hold
+// in double precision: coefficients rounded to single precision there would
+// move with every last-bit change of u and leave a noise floor of ~1e-8 in
+// the converged velocities. What changes is how: the per-node divisions of
+// the FP64 row become per-slot constants, 1/dt = max(1/timeStep, speed /
+// (cfl hChar)) for the reference's dt = min(timeStep, cfl hChar / speed) and
+// aP = aP0 / uvAlpha by the reciprocal of uvAlpha, and dE = hy / aP0, which
+// only steers the correction and is 0 in effect at the fixed point, takes a
+// single-precision reciprocal. The stored coefficients are single precision.
+struct MomScalars {
+    double invTimeStep, invCflH, invUvAlpha, relaxK, rhoVol;
+    float hy, hx;
+};
+
+template <bool IsU>
+__device__ __forceinline__ float momRowMixed(const MomConst<float>& d, const MomScalars& m, const double* u,
+                                             const double* v, const double* pp, size_t ks, int P, size_t k, int i,
+                                             int j, double maxVel, double alphaLocal, bool sweep) {
+    double ue, uw, vn, vs;
+    if (IsU) {
+        ue = clampv(0.5 * (u[ks] + u[ks + 1]), maxVel);
+        uw = clampv(0.5 * (u[ks - 1] + u[ks]), maxVel);
+        vn = clampv(0.5 * (v[ks] + v[ks + 1]), maxVel);
+        vs = clampv(0.5 * (v[ks - P] + v[ks - P + 1]), maxVel);
+    } else {
+        ue = clampv(0.5 * (u[ks] + u[ks + P]), maxVel);
+        uw = clampv(0.5 * (u[ks - 1] + u[ks + P - 1]), maxVel);
+        vn = clampv(0.5 * (v[ks] + v[ks + P]), maxVel);
+        vs = clampv(0.5 * (v[ks - P] + v[ks]), maxVel);
+    }
+    const double cs = d.convScale;
+    const double Fe = cs * d.rho * d.hy * ue, Fw = cs * d.rho * d.hy * uw;
+    const double Fn = cs * d.rho * d.hx * vn, Fs = cs * d.rho * d.hx * vs;
+    const double aE = d.De + fmax(0.0, -Fe);
+    const double aW = d.De + fmax(0.0, Fw);
+    const double aN = d.Dn + fmax(0.0, -Fn);
+    const double aS = d.Dn + fmax(0.0, Fs);
+    const auto f = IsU ? u : v;
+    const double cur = f[ks];
+    double trans = 0.0;
+    if (d.pseudo) {
+        double invDt = m.invTimeStep;
+        if (d.localPseudo) invDt = fmax(invDt, fmax(fabs(cur), d.minSpeed) * m.invCflH);
+        trans = m.rhoVol * fmin(invDt, 1e20);
+    }
+    double aP0 = (aE + aW + aN + aS) + (Fe - Fw + Fn - Fs) + trans + d.sinkDiag + alphaLocal * d.vol;
+    aP0 = fmax(aP0, 1e-10);
+    const double aP = aP0 * m.invUvAlpha;
+    double Sp = IsU ? (pp[ks] - pp[ks + 1]) * d.hy : (pp[ks] - pp[ks + P]) * d.hx;
+    Sp += (trans + m.relaxK * aP0) * cur;  // uOld == u at momentum time (iterations.cpp:118-119)
+    if (d.convScheme == 1) Sp += souCorrection<IsU>(f, ks, i, j, P, d.M, d.N, Fe, Fw, Fn, Fs);
+    const float rcpAP0 = __builtin_amdgcn_rcpf(float(aP0));
+    if (IsU) d.dE[k] = double(m.hy * rcpAP0);
+    else d.dN[k] = double(m.hx * rcpAP0);
+    const double r0 = Sp + aE * f[ks + 1] + aW * f[ks - 1] + aN * f[ks + P] + aS * f[ks - P] - aP * cur;
+    const gp<float>* A = IsU ? d.mu : d.mv;
+    const float aPt = float(aP);
+    A[M_AP][k] = aPt;
+    A[M_AE][k] = float(aE);
+    A[M_AW][k] = float(aW);
+    A[M_AN][k] = float(aN);
+    A[M_AS][k] = float(aS);
+    const float r0t = float(r0);
+    A[M_R0][k] = r0t;
+    if (sweep) A[M_D1][k] = 0.0f + r0t / aPt;
+    else A[M_D0][k] = 0.0f;
+    return r0t;
+}
+
 // Occupancy hint of the momentum row kernels: without it the compiler keeps
 // 197 VGPRs (two waves per SIMD); at three waves it fits 168 without spills,
 // and the assembly of 8 x 1024x512 runs in 118 instead of 134 us. Eight-row
@@ -420,6 +493,16 @@ __global__ __launch_bounds__(kBlock, momWaves<R>()) void k_mom_assemble(KArgs a)
         const double cfl = st.pseudoCFL;
         const auto gu = G(d.u), gv = G(d.v), gpp = G(d.p);
         const MomConst<T> c(d);
+        MomScalars ms;
+        if constexpr (sizeof(T) == 4) {
+            ms.invTimeStep = 1.0 / fmax(d.timeStep, 1e-20);
+            ms.invCflH = 1.0 / (cfl * d.hChar);
+            ms.invUvAlpha = 1.0 / d.uvAlpha;
+            ms.relaxK = (1.0 - d.uvAlpha) / d.uvAlpha;
+            ms.rhoVol = d.rho * d.vol;
+            ms.hy = float(d.hy);
+            ms.hx = float(d.hx);
+        }
         // every staging load of the tile first (clamped addresses), then the
         // LDS stores with the out-of-grid nodes zeroed
         constexpr int NE = HH * HW, NIT = (NE + kBlock - 1) / kBlock;
@@ -462,11 +545,15 @@ __global__ __launch_bounds__(kBlock, momWaves<R>()) void k_mom_assemble(KArgs a)
             const size_t k = size_t(i) * P + j;
             const size_t ks = size_t(wv * R + r + 2) * HW + (lane + 2);
             if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 2) {
-                const T res = momRow<T, true>(c, su, sv, sp, ks, HW, k, i, j, maxVel, cfl, aUr[r], sweep);
+                T res;
+                if constexpr (sizeof(T) == 4) res = momRowMixed<true>(c, ms, su, sv, sp, ks, HW, k, i, j, maxVel, aUr[r], sweep);
+                else res = momRow<T, true>(c, su, sv, sp, ks, HW, k, i, j, maxVel, cfl, aUr[r], sweep);
                 v[0] += double(res) * double(res);
             }
             if (i >= 1 && i <= M - 2 && j >= 1 && j <= N - 1) {
-                const T res = momRow<T, false>(c, su, sv, sp, ks, HW, k, i, j, maxVel, cfl, aVr[r], sweep);
+                T res;
+                if constexpr (sizeof(T) == 4) res = momRowMixed<false>(c, ms, su, sv, sp, ks, HW, k, i, j, maxVel, aVr[r], sweep);
+                else res = momRow<T, false>(c, su, sv, sp, ks, HW, k, i, j, maxVel, cfl, aVr[r], sweep);
                 v[1] += double(res) * double(res);
             }
         }
