@@ -8,10 +8,11 @@ Sources, recorded side by side:
   * "dp_fp64_gpu": the device library in FP64 with the same settings (its
     per-step parity with the oracle is tested in tests/test_gpu_parity.py);
     the test suite checks that the two agree;
-  * "iters_float_oracle": outer iterations the oracle in float arithmetic
-    (the reference's arithmetic, algorithm and inner budgets) needs to reach
-    the bench's stop test (epsilon 1e-7 over the p' columns); the CPU
-    baseline scales its timed sample by it.
+  * "iters_float_oracle_dp": outer iterations the oracle in float
+    arithmetic (the reference's arithmetic, algorithm and inner budgets)
+    needs until its pressure drop is within 1e-4 of the FP64 oracle's (its
+    residuals floor above the bench's 1e-7 stop test in single precision);
+    the CPU baseline scales its timed sample by it.
 
     python tests/golden/make_network_refs.py --gpu          # needs an MI355X
     python tests/golden/make_network_refs.py --oracle 0 1   # CPU, ~10 min/seed
@@ -87,21 +88,33 @@ def run_oracle(d: dict, seeds: list[int]) -> None:
         OUT.write_text(json.dumps(d, indent=1))
 
 
-def run_float(d: dict, seeds: list[int], eps: float = 1e-7, cap: int = 40000) -> None:
+def run_float(d: dict, seeds: list[int], cap: int = 40000, every: int = 100) -> None:
+    """The float oracle (the reference's arithmetic, algorithm and inner
+    budgets) in chunks of `every` outer iterations until its full-system
+    pressure drop is within 1e-4 of the FP64 oracle's: its residuals floor
+    above the 1e-7 stop test in single precision, so the metric's accuracy
+    bar is what it can reach."""
     import oracle_py as op
     import workloads as wl
 
     for s in seeds:
+        ref = d["seeds"][str(s)]["dp_oracle"]
         p, ct = wl.network_1024x512(s)
         p.massResidualScope = 1
-        p.epsilon = eps
+        p.epsilon = 0.0
         p.maxIterations = cap
         o = op.Oracle(p, ct, fp32=True)
         t = time.time()
-        n, st, _ = o.run(cap)
+        it, err, st = 0, 1.0, {}
+        while it < cap:
+            n, st, _ = o.run(every)
+            it = st["iter"]
+            err = abs(o.full_dp(GAMMA_MIN) / ref - 1)
+            if err <= 1e-4 or n < every:
+                break
         e = d["seeds"].setdefault(str(s), {})
-        e["iters_float_oracle"] = st["iter"] if st["error"] <= eps else None
-        e["dp_float_oracle"] = o.full_dp(GAMMA_MIN)
+        e["iters_float_oracle_dp"] = it if err <= 1e-4 else None
+        e["float_oracle_error_there"] = st.get("error")
         print(s, e, f"{time.time() - t:.0f}s", flush=True)
         OUT.write_text(json.dumps(d, indent=1))
 
