@@ -41,7 +41,7 @@ def run(cmd: list[str]) -> None:
 def build_lib() -> Path:
     OBJ_DIR.mkdir(parents=True, exist_ok=True)
     objs = []
-    for src in ("cp_kernels.hip", "cp_engine.hip"):
+    for src in ("cp_kernels.hip", "cp_engine.hip", "cp_thermal.hip"):
         obj = OBJ_DIR / (src + ".o")
         run([HIPCC, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(obj)

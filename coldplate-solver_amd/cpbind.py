@@ -107,6 +107,39 @@ class PlaneMetrics(C.Structure):
     ]
 
 
+class ThermalParams(C.Structure):
+    """cp_thermal_params (include/coldplate_hip.h)."""
+
+    _fields_ = [
+        ("Nx", C.c_int32), ("Ny", C.c_int32), ("nz_solid", C.c_int32), ("nz_fluid", C.c_int32),
+        ("dx", C.c_double), ("dy", C.c_double), ("H_b", C.c_double), ("H_t", C.c_double),
+        ("k_s", C.c_double), ("k_f", C.c_double), ("rho", C.c_double), ("Cp", C.c_double),
+        ("T_inlet", C.c_double), ("q0", C.c_double), ("k_interp_mode", C.c_int32), ("qk", C.c_double),
+        ("rtol", C.c_double), ("maxiter", C.c_int32),
+    ]
+
+    def copy(self) -> "ThermalParams":
+        q = ThermalParams()
+        C.memmove(C.byref(q), C.byref(self), C.sizeof(ThermalParams))
+        return q
+
+
+class ThermalStats(C.Structure):
+    """cp_thermal_stats."""
+
+    _fields_ = [
+        ("iters", C.c_int32), ("converged", C.c_int32), ("relres", C.c_double),
+        ("T_base_min", C.c_double), ("T_base_max", C.c_double), ("T_fin_min", C.c_double),
+        ("T_fin_max", C.c_double), ("T_fluid_min", C.c_double), ("T_fluid_max", C.c_double),
+        ("T_avg_base", C.c_double), ("T_avg_fluid", C.c_double), ("T_avg_solid", C.c_double),
+        ("T_avg_global", C.c_double), ("Q_in", C.c_double), ("Q_out", C.c_double), ("T_out", C.c_double),
+        ("imbalance_pct", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Request(C.Structure):
     _fields_ = [("prm", C.POINTER(Params)), ("cellType", C.POINTER(C.c_double)), ("arrival", C.c_double),
                 ("gammaMin", C.c_double)]
@@ -185,6 +218,8 @@ EXPORTS = [
     ("cp_serve_pool", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     ("cp_batch_autotune", C.c_int, [_vp, C.c_int, _ip, _ip, _ip, _dp]),
     ("cp_run_noalloc", C.c_int, [C.POINTER(Params), _dp, _dp, _dp, _dp, C.POINTER(StepStats)]),
+    ("cp_thermal_params_default", C.c_int, [C.POINTER(ThermalParams)]),
+    ("cp_thermal_solve", C.c_int, [C.POINTER(ThermalParams), _dp, _dp, _dp, _dp, C.POINTER(ThermalStats)]),
     ("cp_io_read_params", C.c_int, [C.c_char_p, C.POINTER(Params), _ip, _ip]),
     ("cp_io_read_geometry", C.c_int, [C.c_char_p, C.c_int, C.c_int, _dp]),
     ("cp_io_read_matrix", C.c_int, [C.c_char_p, C.c_int, C.c_int, _dp]),
@@ -506,3 +541,40 @@ def save_all(directory, prm: Params, u, v, p, cell_type, gamma, alpha, n_in: int
     arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (u, v, p, cell_type, gamma, alpha)]
     check(load().cp_io_save_all(str(directory).encode(), prm.M, prm.N, prm.hx, prm.hy, n_in, n_out,
                                  *(dptr(x) for x in arrs), precision))
+
+
+# ---------------------------------------------------------------------------
+# downstream conjugate heat transfer
+# ---------------------------------------------------------------------------
+def thermal_default_params() -> ThermalParams:
+    p = ThermalParams()
+    check(load().cp_thermal_params_default(C.byref(p)))
+    return p
+
+
+def thermal_inputs(u: np.ndarray, v: np.ndarray, cell_type: np.ndarray, n_in: int = 0, n_out: int = 0):
+    """The thermal solve's inputs from a SIMPLE result, as the reference's
+    files hold them: u_thermal / v_thermal are the cell-centred velocities
+    (Utilities/output.cpp:87-91) of columns n_in .. N - n_out - 1
+    (output.cpp:155-181), and gamma = 1 - cellType over the same columns
+    (geometry_thermal.txt, inverted by thermal_solver.cpp:738-743)."""
+    M, N = cell_type.shape
+    assert u.shape == (M + 1, N) and v.shape == (M, N + 1)
+    assert 0 <= n_in and 0 <= n_out and n_in + n_out < N
+    uc = 0.5 * (u[:-1, :] + u[1:, :])
+    vc = 0.5 * (v[:, :-1] + v[:, 1:])
+    cols = slice(n_in, N - n_out)
+    return (np.ascontiguousarray(1.0 - cell_type[:, cols]), np.ascontiguousarray(uc[:, cols]),
+            np.ascontiguousarray(vc[:, cols]))
+
+
+def thermal_solve(prm: ThermalParams, gamma: np.ndarray, u: np.ndarray, v: np.ndarray):
+    """cp_thermal_solve: (T of shape (Nz, Ny, Nx), statistics dict)."""
+    g, uu, vv = (np.ascontiguousarray(a, dtype=np.float64) for a in (gamma, u, v))
+    assert g.shape == uu.shape == vv.shape == (prm.Ny, prm.Nx)
+    T = np.empty((prm.nz_solid + prm.nz_fluid, prm.Ny, prm.Nx))
+    st = ThermalStats()
+    pr = prm.copy()
+    rc = load().cp_thermal_solve(C.byref(pr), dptr(g), dptr(uu), dptr(vv), dptr(T), C.byref(st))
+    check(rc)
+    return T, st.as_dict()

@@ -210,4 +210,10 @@ inline int pRows(bool f32, int rows) {
 }
 hipError_t launchPlane(int nreq, hipStream_t s, const SlotDesc* desc, const PlaneReq* req, PlaneOut* out);
 
+// Shared by the library's translation units (defined in cp_engine.hip):
+// record the message cp_last_error() returns and pass `code` through; the
+// check that the current device is a gfx950.
+int reportError(int code, const char* fmt, ...);
+bool currentDeviceIsGfx950();
+
 }  // namespace cpdev

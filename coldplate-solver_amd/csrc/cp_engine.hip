@@ -76,6 +76,24 @@ bool isGfx950(int dev) {
     return strncmp(prop.gcnArchName, "gfx950", 6) == 0;
 }
 
+}  // namespace
+
+int cpdev::reportError(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    return setError(nullptr, code, "%s", buf);
+}
+
+bool cpdev::currentDeviceIsGfx950() {
+    int dev = 0;
+    return hipGetDevice(&dev) == hipSuccess && isGfx950(dev);
+}
+
+namespace {
+
 // ---------------------------------------------------------------------------
 // arena: first-fit free list over one device allocation (offsets in bytes,
 // 256-byte granules), coalescing on free.

@@ -49,4 +49,19 @@ static inline void cp_fill_reference_defaults(cp_params* p) {
     p->sorOmega = 0.0;                                                   /* SIMPLE.h:156 */
 }
 
+/* Defaults of the thermal solve: struct Params of
+   ThermalSolver/thermal_solver.cpp:142-189. Grid sizes, spacings and the
+   heat flux come from the caller (the reference reads them from
+   thermal_params.txt and the geometry file). */
+static inline void cp_fill_thermal_defaults(cp_thermal_params* p) {
+    memset(p, 0, sizeof(*p));
+    p->nz_solid = 2; p->nz_fluid = 7;              /* :147-148 */
+    p->H_b = 0.0005; p->H_t = 0.004;               /* :152-153 */
+    p->k_s = 400.0; p->k_f = 0.6;                  /* :156-157 */
+    p->rho = 998.0; p->Cp = 4180.0;                /* :158-159 */
+    p->T_inlet = 30.0;                             /* :160 */
+    p->rtol = 1e-6; p->maxiter = 1000;             /* :163-164 */
+    p->k_interp_mode = 1; p->qk = 1.0;             /* :179-180 */
+}
+
 #endif /* COLDPLATE_DEFAULTS_H */

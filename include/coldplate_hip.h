@@ -404,6 +404,60 @@ int cp_serve_pool(cp_batch* const* batches, int nbatches, const cp_request* reqs
                   cp_result* results);
 
 /* ------------------------------------------------------------------ */
+/* downstream conjugate heat transfer (the reference's ThermalSolver)   */
+/* ------------------------------------------------------------------ */
+/* Parameters of the 3D conjugate-heat-transfer solve; names and defaults
+   follow struct Params of ThermalSolver/thermal_solver.cpp:142-189. The
+   grid is Nx x Ny cells in the plane and nz_solid base-plate layers (H_b
+   thick) under nz_fluid channel layers (H_t thick). */
+typedef struct cp_thermal_params {
+    int32_t Nx, Ny;               /* cells in x (columns) and y (rows) */
+    int32_t nz_solid, nz_fluid;   /* 2, 7 */
+    double  dx, dy;               /* cell width, height [m] */
+    double  H_b, H_t;             /* base plate and channel heights [m]: 5e-4, 4e-3 */
+    double  k_s, k_f;             /* solid / fluid conductivity [W/m K]: 400, 0.6 */
+    double  rho, Cp;              /* fluid density, heat capacity: 998, 4180 */
+    double  T_inlet;              /* 30 */
+    double  q0;                   /* heat flux into the base [W/m^2] (thermal_params.txt) */
+    int32_t k_interp_mode;        /* 0 linear, 1 RAMP (default), 2 harmonic */
+    double  qk;                   /* RAMP convexity: 1 */
+    double  rtol;                 /* relative residual of the linear solve: 1e-6 */
+    int32_t maxiter;              /* 1000 */
+} cp_thermal_params;
+
+/* Results of cp_thermal_solve: the solver's iterations and final relative
+   residual, and the statistics the reference prints and writes
+   (thermal_solver.cpp:556-604, thermal_output.cpp:135-205). */
+typedef struct cp_thermal_stats {
+    int32_t iters, converged;
+    double  relres;
+    double  T_base_min, T_base_max, T_fin_min, T_fin_max, T_fluid_min, T_fluid_max;
+    double  T_avg_base, T_avg_fluid, T_avg_solid, T_avg_global;
+    double  Q_in, Q_out, T_out, imbalance_pct;
+} cp_thermal_stats;
+
+/* Fill p with the defaults of the reference's thermal Params. */
+int cp_thermal_params_default(cp_thermal_params* p);
+
+/* Replaces ThermalSolver3D::solve (ThermalSolver/thermal_solver.cpp:421-
+   604): assembles the reference's finite-volume system on the device (one
+   7-point row per cell: harmonic-mean lateral conductances, series
+   conductances in z, first-order upwind advection of the depth-profiled
+   velocity 4z(1-z) u in fluid cells, the inlet temperature and convective
+   outlet, the base heat flux), solves it in FP64 to relative residual rtol
+   with BiCGSTAB preconditioned by one aggregation-multigrid V-cycle
+   (2 x 2 aggregates in the plane, every layer kept, z-line Jacobi
+   smoothing), and returns T and the statistics. gamma (1 = fluid), u, v:
+   Ny x Nx row-major cell values, as the reference's geometry_thermal.txt
+   (inverted) and u_thermal.txt / v_thermal.txt; T: (nz_solid + nz_fluid)
+   x Ny x Nx, cell (i, j, k) at k Nx Ny + j Nx + i, the reference's order.
+   Allocates its device arrays per call. Returns CP_E_DIVERGED when the
+   residual becomes non-finite; a solve that ends at maxiter returns CP_OK
+   with converged = 0. */
+int cp_thermal_solve(const cp_thermal_params* p, const double* gamma, const double* u, const double* v, double* T,
+                     cp_thermal_stats* st);
+
+/* ------------------------------------------------------------------ */
 /* a solve that allocates no device memory                             */
 /* ------------------------------------------------------------------ */
 /* Device scratch of cp_run_noalloc: a static array of this many bytes that

@@ -7,6 +7,7 @@
 // Every function takes a `fp32` switch: 0 = FP64 oracle, 1 = the
 // reference's all-float arithmetic (CPU baseline).
 #include "simple_oracle.hpp"
+#include "thermal_oracle.hpp"
 #include "../include/coldplate_defaults.h"
 
 #include <chrono>
@@ -259,6 +260,35 @@ int or_dp_sequence(const cp_params* prm, int n, const double* dp) {
     }
     return -1;
 }
+
+// Thermal oracle: the reference's CSR system of the conjugate heat transfer
+// solve (pass NULL arrays to query n and nnz first) and the statistics of a
+// temperature field.
+int or_thermal_assemble(const cp_thermal_params* p, const double* gamma, const double* u, const double* v, int* n,
+                        int* nnz, int* row_ptr, int* col_idx, double* values, double* b) {
+    if (!p || !gamma || !u || !v || p->Nx < 1 || p->Ny < 1) return -1;
+    oracle::Thermal t;
+    t.init(*p, gamma, u, v);
+    const oracle::ThermalCSR A = t.assemble();
+    if (n) *n = A.n;
+    if (nnz) *nnz = int(A.values.size());
+    if (row_ptr) std::copy(A.row_ptr.begin(), A.row_ptr.end(), row_ptr);
+    if (col_idx) std::copy(A.col_idx.begin(), A.col_idx.end(), col_idx);
+    if (values) std::copy(A.values.begin(), A.values.end(), values);
+    if (b) std::copy(A.b.begin(), A.b.end(), b);
+    return 0;
+}
+
+int or_thermal_stats(const cp_thermal_params* p, const double* gamma, const double* u, const double* v,
+                     const double* T, cp_thermal_stats* st) {
+    if (!p || !gamma || !u || !v || !T || !st) return -1;
+    oracle::Thermal t;
+    t.init(*p, gamma, u, v);
+    t.stats(T, st);
+    return 0;
+}
+
+void or_thermal_params_default(cp_thermal_params* p) { cp_fill_thermal_defaults(p); }
 
 // Standalone restated solvers on caller CSR data (for the solver fixtures).
 int or_jacobi_f32(int n, const int* row_ptr, const int* col_idx, const float* values, const float* b,

@@ -19,7 +19,7 @@ import numpy as np
 ORACLE_DIR = Path(__file__).resolve().parent
 REPO = ORACLE_DIR.parent
 sys.path.insert(0, str(REPO / "coldplate-solver_amd"))
-from cpbind import Params, PlaneMetrics, StepStats, field_shape  # noqa: E402
+from cpbind import Params, PlaneMetrics, StepStats, ThermalParams, ThermalStats, field_shape  # noqa: E402
 
 LIB = ORACLE_DIR / "lib" / "liboracle.so"
 REF_LIB = ORACLE_DIR / "_ref" / "libref.so"
@@ -62,6 +62,9 @@ def load() -> C.CDLL:
             "or_mg_level": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, _ip, _ip, _dp, _dp, _dp]),
             "or_cfl_sequence": (C.c_int, [C.POINTER(Params), C.c_int, _dp, _dp, _dp]),
             "or_dp_sequence": (C.c_int, [C.POINTER(Params), C.c_int, _dp]),
+            "or_thermal_assemble": (C.c_int, [C.POINTER(ThermalParams), _dp, _dp, _dp, _ip, _ip, _ip, _ip, _dp, _dp]),
+            "or_thermal_stats": (C.c_int, [C.POINTER(ThermalParams), _dp, _dp, _dp, _dp, C.POINTER(ThermalStats)]),
+            "or_thermal_params_default": (None, [C.POINTER(ThermalParams)]),
         }
         for k, (r, a) in sig.items():
             f = getattr(lib, k)
@@ -271,3 +274,36 @@ def dp_exit_iteration(prm: Params, drops) -> int:
     iteration after which it stops the run, or -1."""
     d = np.ascontiguousarray(drops, np.float64)
     return int(load().or_dp_sequence(C.byref(prm), len(d), _ptr(d, _dp)))
+
+
+def thermal_default_params() -> ThermalParams:
+    p = ThermalParams()
+    load().or_thermal_params_default(C.byref(p))
+    return p
+
+
+def thermal_assemble(prm: ThermalParams, gamma, u, v):
+    """The reference's CSR system of the heat-transfer solve (scipy CSR
+    matrix, right-hand side)."""
+    import scipy.sparse as sp
+
+    g, uu, vv = (np.ascontiguousarray(a, dtype=np.float64) for a in (gamma, u, v))
+    n, nnz = C.c_int(0), C.c_int(0)
+    rc = load().or_thermal_assemble(C.byref(prm), _ptr(g, _dp), _ptr(uu, _dp), _ptr(vv, _dp), C.byref(n),
+                                    C.byref(nnz), None, None, None, None)
+    assert rc == 0
+    rp = np.empty(n.value + 1, np.int32)
+    ci = np.empty(nnz.value, np.int32)
+    va = np.empty(nnz.value)
+    b = np.empty(n.value)
+    load().or_thermal_assemble(C.byref(prm), _ptr(g, _dp), _ptr(uu, _dp), _ptr(vv, _dp), None, None,
+                               _ptr(rp, _ip), _ptr(ci, _ip), _ptr(va, _dp), _ptr(b, _dp))
+    return sp.csr_matrix((va, ci, rp), shape=(n.value, n.value)), b
+
+
+def thermal_stats(prm: ThermalParams, gamma, u, v, T) -> dict:
+    g, uu, vv, tt = (np.ascontiguousarray(a, dtype=np.float64) for a in (gamma, u, v, T))
+    st = ThermalStats()
+    assert load().or_thermal_stats(C.byref(prm), _ptr(g, _dp), _ptr(uu, _dp), _ptr(vv, _dp), _ptr(tt, _dp),
+                                   C.byref(st)) == 0
+    return st.as_dict()

@@ -74,6 +74,38 @@ def test_create_rejects_bad_arguments_without_gpu_work():
     assert b"at least 4" in lib.cp_last_error(None)
 
 
+def test_thermal_defaults_and_argument_errors_need_no_gpu():
+    p = cb.thermal_default_params()
+    # struct Params, ThermalSolver/thermal_solver.cpp:142-189
+    assert (p.nz_solid, p.nz_fluid, p.H_b, p.H_t) == (2, 7, 0.0005, 0.004)
+    assert (p.k_s, p.k_f, p.rho, p.Cp, p.T_inlet) == (400.0, 0.6, 998.0, 4180.0, 30.0)
+    assert (p.rtol, p.maxiter, p.k_interp_mode, p.qk) == (1e-6, 1000, 1, 1.0)
+    lib = cb.load()
+    g = np.ones((4, 4))
+    T = np.empty((9, 4, 4))
+    st = cb.ThermalStats()
+    for field, val, msg in (("Nx", 0, b"grid"), ("nz_fluid", 20, b"layers"), ("dx", 0.0, b"dx"),
+                            ("k_interp_mode", 3, b"k_interp_mode"), ("rtol", 0.0, b"rtol")):
+        q = cb.thermal_default_params()
+        q.Nx, q.Ny, q.dx, q.dy, q.q0 = 4, 4, 1e-4, 1e-4, 1e5
+        setattr(q, field, val)
+        rc = lib.cp_thermal_solve(C.byref(q), cb.dptr(g), cb.dptr(g), cb.dptr(g), cb.dptr(T), C.byref(st))
+        assert rc == -1, field
+        assert msg in lib.cp_last_error(None), field
+
+
+def test_thermal_structs_match_header():
+    text = re.sub(r"/\*.*?\*/", "", (REPO / "include" / "coldplate_hip.h").read_text(), flags=re.S)
+    for name, cls in (("cp_thermal_params", cb.ThermalParams), ("cp_thermal_stats", cb.ThermalStats)):
+        body = text[text.index(f"typedef struct {name} {{") + len(f"typedef struct {name} {{"):text.index(f"}} {name};")]
+        fields = []
+        for decl in body.split(";"):
+            decl = decl.strip()
+            if decl:
+                fields += [f.strip() for f in decl.split(None, 1)[1].split(",")]
+        assert [f for f, _ in cls._fields_] == fields, name
+
+
 def test_geometry_generators_deterministic_and_valid():
     a = wl.geom_network(128, 256, 5)
     b = wl.geom_network(128, 256, 5)
