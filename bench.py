@@ -41,6 +41,9 @@ Reported beside the value (one JSON line):
   * plain_fp64_single: the plain FP64 single-geometry path (the drop-in:
     reference algorithm and inner budgets, Jacobi-preconditioned CG,
     polled) on every seed with the same stop test;
+  * thermal_cht: the downstream conjugate-heat-transfer solve on the
+    seed-0 solution (1024 x 512 x 9 cells), its time per solve and the
+    roofline of its finest-level z-line sweep;
   * cpu_baseline: the oracle in float arithmetic (the reference's
     arithmetic and algorithm) on the box's physical host cores for a
     bounded number of outer iterations of seed 0, scaled by the
@@ -79,6 +82,11 @@ GOLDEN = REPO / "tests" / "golden" / "network_1024x512_dp.json"
 TRAFFIC = REPO / "profiles" / "counters_network_1024x512.json"
 GEOMS: list = []
 N_SEEDS = 8
+FIELDS0: dict = {}  # the seed-0 FP64 solution (u, v), the thermal leg's input
+# thermal leg: finest-level z-line sweep from a nonzero iterate, per cell:
+# read b, x and the nine coefficient / Thomas-factor planes, write x (8 B each)
+TH_ZLINE_BYTES = 96
+TH_Q0 = 1e6  # W/m^2: the geometry generators' 100 W/cm^2 (GeometryGeneratorV1.py:46)
 
 
 def log(msg: str) -> None:
@@ -102,7 +110,8 @@ def parse():
     ap.add_argument("--precond", choices=["amg", "jacobi"], default="amg",
                     help="pressure-CG preconditioner of the served form")
     ap.add_argument("--check", type=int, default=10, help="outer iterations per graph launch between polls")
-    ap.add_argument("--workload", choices=["network", "batch64", "stream", "gamma2048"], default="network",
+    ap.add_argument("--thermal-solves", type=int, default=3, help="timed thermal CHT solves (0: no thermal leg)")
+    ap.add_argument("--workload", choices=["network", "batch64", "stream", "gamma2048", "thermal"], default="network",
                     help="network: the headline line (1024x512 networks); batch64, stream, gamma2048: the "
                          "batched, served and 2048^2 gamma-field configurations, reported on their own line")
     ap.add_argument("--rate", type=float, default=2.0, help="served-stream subsample: Poisson arrival rate (1/s)")
@@ -217,6 +226,8 @@ def dropin_baseline(a, refs: list[float]) -> dict:
             if st["converged"] or st["health"]:
                 break
         dt = time.perf_counter() - t0
+        if s == 0:
+            FIELDS0["uv"] = (g.field(cb.F_U), g.field(cb.F_V))
         g.close()
         per.append(dict(seed=s, seconds=round(dt, 3), iters=it, iters_at_eps=reached, dp_err=err))
         log(f"drop-in FP64 seed {s}: {per[-1]}")
@@ -277,6 +288,82 @@ def cpu_baseline(a) -> dict:
                        f"threads, scaled by the {need} outer iterations the float oracle needs on seed 0 until its "
                        f"pressure drop is within 1e-4 of the FP64 oracle's (in single precision its residuals "
                        f"floor above the 1e-7 stop test)")
+
+
+def thermal_leg(a) -> dict:
+    """The downstream conjugate-heat-transfer solve (cp_thermal_solve) on the
+    seed-0 solution: 1024 x 512 x 9 cells (the reference's 2 base + 7
+    channel layers), 100 W/cm^2 into the base, default rtol 1e-6. Reports
+    the wall time per solve (host arrays in and out, device allocation
+    included), the device time, and the roofline of its dominant kernel,
+    the finest-level z-line sweep, from HIP events around every such sweep.
+    CPU comparison: the reference's solver (AMGCL) is not available, so the
+    oracle's system is solved directly (SciPy SuperLU) on a 128 x 64 window
+    of the same inputs, and the device solves the same window."""
+    p, ct = GEOMS[0]
+    if "uv" not in FIELDS0:
+        q = p.copy()
+        q.massResidualScope = 1
+        q.epsilon = a.eps
+        g = cb.Solver(q, ct)
+        g.run()
+        FIELDS0["uv"] = (g.field(cb.F_U), g.field(cb.F_V))
+        g.close()
+    u, v = FIELDS0["uv"]
+    gam, ut, vt = cb.thermal_inputs(u, v, ct)
+
+    def prm(Nx, Ny):
+        tp = cb.thermal_default_params()
+        tp.Nx, tp.Ny, tp.dx, tp.dy, tp.H_t, tp.q0, tp.timing = Nx, Ny, p.hx, p.hy, p.Ht, TH_Q0, 1
+        return tp
+
+    tp = prm(p.N, p.M)
+    cb.thermal_solve(tp, gam, ut, vt)  # first call: code objects, allocator
+    runs = []
+    for _ in range(max(a.thermal_solves, 1)):
+        t0 = time.perf_counter()
+        _, st = cb.thermal_solve(tp, gam, ut, vt)
+        runs.append((time.perf_counter() - t0, st))
+    n = tp.Nx * tp.Ny * (tp.nz_solid + tp.nz_fluid)
+    sweeps = sum(st["sweeps"] for _, st in runs)
+    sweep_ms = sum(st["sweep_ms"] * st["sweeps"] for _, st in runs) / max(sweeps, 1)
+    achieved = TH_ZLINE_BYTES * n / (sweep_ms * 1e-3) / 1e9
+    wall = float(np.median([w for w, _ in runs]))
+    st = runs[-1][1]
+    out = {"workload": f"thermal CHT on the seed-0 solution, {tp.Nx}x{tp.Ny}x{tp.nz_solid + tp.nz_fluid} cells, "
+                       f"q0 {TH_Q0:g} W/m^2, rtol {tp.rtol:g}",
+           "solves": len(runs), "ms_per_solve": round(wall * 1e3, 2),
+           "device_ms": round(float(np.median([s_["device_ms"] for _, s_ in runs])), 2),
+           "iters": st["iters"], "relres": st["relres"], "converged": st["converged"],
+           "T_out": round(st["T_out"], 4), "T_base_max": round(st["T_base_max"], 4),
+           "imbalance_pct": round(st["imbalance_pct"], 4),
+           "roofline": {"bound": "hbm", "kernel": "k_th_zline<9> (finest level, nonzero iterate)",
+                        "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                        "avg_launch_us": round(sweep_ms * 1e3, 2), "bytes_per_launch": TH_ZLINE_BYTES * n,
+                        "bytes_per_unit": TH_ZLINE_BYTES, "launches_timed": sweeps}}
+    if not a.no_cpu:
+        sys.path.insert(0, str(REPO / "oracle"))
+        import oracle_py as op  # CPU baseline leg only
+        import scipy.sparse.linalg as spla
+
+        wn = prm(128, 64)
+        win = tuple(np.ascontiguousarray(x[:64, :128]) for x in (gam, ut, vt))
+        t0 = time.perf_counter()
+        A, b = op.thermal_assemble(wn, *win)
+        spla.spsolve(A.tocsc(), b)
+        cpu_s = time.perf_counter() - t0
+        gpu = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            cb.thermal_solve(wn, *win)
+            gpu.append(time.perf_counter() - t0)
+        out["cpu_baseline"] = {"value": round(cpu_s * 1e3, 1), "unit": "ms per solve", "cores": 1, "kind": "port",
+                               "sample": "the oracle's system (thermal_oracle.hpp) of a 128x64x9 window of the "
+                                         "same inputs, assembled and solved directly by SciPy SuperLU on one "
+                                         "host core; AMGCL, the reference's solver, is not in the mount",
+                               "device_ms_same_window": round(min(gpu) * 1e3, 2)}
+    return out
 
 
 def copy_calibration(mib: int) -> None:
@@ -416,6 +503,16 @@ def main() -> int:
     lib = cb.load()
     if lib.cp_device_count() < 1:
         raise SystemExit("no gfx950 device")
+    if a.workload == "thermal":
+        GEOMS[:] = [wl.network_1024x512(0)]
+        t = thermal_leg(a)
+        print(json.dumps({"metric": "thermal CHT solves/s", "value": round(1e3 / t["ms_per_solve"], 3),
+                          "unit": "solves/s", "n_gpus": 1, "steps": t["solves"], "warmup": 1,
+                          "ms_per_step": t["ms_per_solve"], "higher_is_better": True, "scaling": "weak",
+                          "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+                          "config": {"workload": t["workload"]}, "thermal_cht": t,
+                          "roofline": t["roofline"], "cpu_baseline": t.get("cpu_baseline")}), flush=True)
+        return 0
     if a.workload != "network":
         print(json.dumps(other_workload(a)), flush=True)
         return 0
@@ -509,6 +606,9 @@ def main() -> int:
     roof_dom = roofline_of(runs, cb.K_MOM_ASM, MOM_ASM_BYTES, "k_mom_assemble<float>", ws_mom,
                            traffic.get("k_mom_assemble"))
     drop = None if a.no_dropin else dropin_baseline(a, refs)
+    thermal = thermal_leg(a) if a.thermal_solves > 0 else None
+    if thermal:
+        log(f"thermal leg: {thermal}")
     cpu = None if a.no_cpu else cpu_baseline(a)
     if cpu:
         log(f"cpu baseline: {cpu}")
@@ -556,6 +656,7 @@ def main() -> int:
         },
         "roofline": roof,
         "roofline_dominant": roof_dom,
+        "thermal_cht": thermal,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

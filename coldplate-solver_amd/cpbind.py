@@ -115,7 +115,7 @@ class ThermalParams(C.Structure):
         ("dx", C.c_double), ("dy", C.c_double), ("H_b", C.c_double), ("H_t", C.c_double),
         ("k_s", C.c_double), ("k_f", C.c_double), ("rho", C.c_double), ("Cp", C.c_double),
         ("T_inlet", C.c_double), ("q0", C.c_double), ("k_interp_mode", C.c_int32), ("qk", C.c_double),
-        ("rtol", C.c_double), ("maxiter", C.c_int32),
+        ("rtol", C.c_double), ("maxiter", C.c_int32), ("timing", C.c_int32),
     ]
 
     def copy(self) -> "ThermalParams":
@@ -133,7 +133,8 @@ class ThermalStats(C.Structure):
         ("T_fin_max", C.c_double), ("T_fluid_min", C.c_double), ("T_fluid_max", C.c_double),
         ("T_avg_base", C.c_double), ("T_avg_fluid", C.c_double), ("T_avg_solid", C.c_double),
         ("T_avg_global", C.c_double), ("Q_in", C.c_double), ("Q_out", C.c_double), ("T_out", C.c_double),
-        ("imbalance_pct", C.c_double),
+        ("imbalance_pct", C.c_double), ("device_ms", C.c_double), ("sweep_ms", C.c_double),
+        ("sweeps", C.c_int32),
     ]
 
     def as_dict(self) -> dict:

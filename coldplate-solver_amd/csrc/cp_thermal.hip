@@ -243,10 +243,10 @@ __device__ inline double thPlaneAx(const ThLevel& L, const double* __restrict__ 
 }
 
 // One z-line Jacobi sweep: xout = xin + omega Z^-1 (b - A xin), Z the
-// tridiagonal z-part of A. xin == nullptr stands for a zero iterate. NZ is
-// the layer count: the column lives in registers and every load of the
+// tridiagonal z-part of A; FROM_ZERO: the iterate is zero (xin unused). NZ
+// is the layer count: the column lives in registers and every load of the
 // column is independent of the elimination, so all are in flight at once.
-template <int NZ>
+template <int NZ, bool FROM_ZERO>
 __global__ __launch_bounds__(kThBlock) void k_th_zline(ThLevel L, const double* __restrict__ xin,
                                                        const double* __restrict__ b, double* __restrict__ xout,
                                                        double omega) {
@@ -260,12 +260,12 @@ __global__ __launch_bounds__(kThBlock) void k_th_zline(ThLevel L, const double* 
         const int c = k * plane + c2;
         r[k] = b[c];
         xc[k] = 0.0;
-        if (xin) {
+        if (!FROM_ZERO) {
             xc[k] = xin[c];
             r[k] += thPlaneAx(L, xin, c, i, j);
         }
     }
-    if (xin) {
+    if (!FROM_ZERO) {
 #pragma unroll
         for (int k = 0; k < NZ; ++k) {
             const int c = k * plane + c2;
@@ -397,11 +397,15 @@ __global__ __launch_bounds__(kThCoarseCols) void k_th_coarsest(ThLevel L, const 
 #define TH_FOR_NZ(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
 static_assert(kThMaxNz == 16, "TH_FOR_NZ lists every layer count up to kThMaxNz");
 
+// xin == nullptr: a sweep from the zero iterate.
 void launchZline(int Nz, int grid, hipStream_t s, const ThLevel& L, const double* xin, const double* b, double* xout,
                  double omega) {
     switch (Nz) {
-#define TH_CASE(n) \
-    case n: k_th_zline<n><<<grid, kThBlock, 0, s>>>(L, xin, b, xout, omega); break;
+#define TH_CASE(n)                                                                          \
+    case n:                                                                                 \
+        if (xin) k_th_zline<n, false><<<grid, kThBlock, 0, s>>>(L, xin, b, xout, omega);   \
+        else k_th_zline<n, true><<<grid, kThBlock, 0, s>>>(L, xin, b, xout, omega);        \
+        break;
         TH_FOR_NZ(TH_CASE)
 #undef TH_CASE
     }
@@ -581,10 +585,12 @@ int dotGrid(size_t n) { return std::min(gridOf(n), kThDotBlocks); }
 
 struct ThDev {
     std::vector<void*> allocs;
+    std::vector<hipEvent_t> events;
     hipStream_t stream = nullptr;
     double* hostSc = nullptr;
     ~ThDev() {  // teardown: errors here have no caller left to report to
         if (stream) (void)hipStreamSynchronize(stream);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
         for (void* p : allocs) (void)hipFree(p);
         if (hostSc) (void)hipHostFree(hostSc);
         if (stream) (void)hipStreamDestroy(stream);
@@ -607,6 +613,29 @@ struct ThDev {
 struct ThHierarchy {
     std::vector<ThLevel> lv;
     std::vector<double*> x, b, y;  // per level (level 0: x, b set per call; y scratch)
+    // timing: event pairs around the finest level's sweeps from a nonzero
+    // iterate (three per V-cycle); evUsed pairs recorded since the last read
+    hipEvent_t* ev = nullptr;
+    int evCap = 0, evUsed = 0;
+
+    void fineSweep(hipStream_t s, const ThLevel& F, const double* xin, const double* bb, double* xout) {
+        const bool timed = ev && evUsed < evCap;
+        if (timed) (void)hipEventRecord(ev[2 * evUsed], s);
+        launchZline(F.Nz, gridOf(size_t(F.W) * F.H), s, F, xin, bb, xout, kThOmega);
+        if (timed) (void)hipEventRecord(ev[2 * evUsed++ + 1], s);
+    }
+
+    // Sum of the recorded sweeps' durations (ms), after a stream sync.
+    double readSweeps(int* count) {
+        double ms = 0;
+        for (int q = 0; q < evUsed; ++q) {
+            float t = 0;
+            if (hipEventElapsedTime(&t, ev[2 * q], ev[2 * q + 1]) == hipSuccess) ms += t;
+        }
+        *count += evUsed;
+        evUsed = 0;
+        return ms;
+    }
 
     hipError_t vcycle(hipStream_t s, const double* b0, double* x0) {
         b[0] = const_cast<double*>(b0);
@@ -616,7 +645,8 @@ struct ThHierarchy {
             const ThLevel& F = lv[l];
             const int cols = F.W * F.H;
             launchZline(F.Nz, gridOf(cols), s, F, nullptr, b[l], y[l], kThOmega);
-            launchZline(F.Nz, gridOf(cols), s, F, y[l], b[l], x[l], kThOmega);
+            if (l == 0) fineSweep(s, F, y[l], b[l], x[l]);
+            else launchZline(F.Nz, gridOf(cols), s, F, y[l], b[l], x[l], kThOmega);
             const ThLevel& C = lv[l + 1];
             k_th_restrict<<<gridOf(size_t(C.W) * C.H * C.Nz), kThBlock, 0, s>>>(F, x[l], b[l], C.W, C.H, b[l + 1]);
         }
@@ -626,8 +656,13 @@ struct ThHierarchy {
             const ThLevel& C = lv[l + 1];
             const int cols = F.W * F.H;
             k_th_prolong<<<gridOf(size_t(cols) * F.Nz), kThBlock, 0, s>>>(F.W, F.H, F.Nz, x[l + 1], C.W, C.H, x[l]);
-            launchZline(F.Nz, gridOf(cols), s, F, x[l], b[l], y[l], kThOmega);
-            launchZline(F.Nz, gridOf(cols), s, F, y[l], b[l], x[l], kThOmega);
+            if (l == 0) {
+                fineSweep(s, F, x[l], b[l], y[l]);
+                fineSweep(s, F, y[l], b[l], x[l]);
+            } else {
+                launchZline(F.Nz, gridOf(cols), s, F, x[l], b[l], y[l], kThOmega);
+                launchZline(F.Nz, gridOf(cols), s, F, y[l], b[l], x[l], kThOmega);
+            }
         }
         return hipGetLastError();
     }
@@ -716,6 +751,21 @@ int cp_thermal_solve(const cp_thermal_params* prm, const double* gamma, const do
     TH_TRY(hipMemcpyAsync(dU, u, plane * sizeof(double), hipMemcpyHostToDevice, s));
     TH_TRY(hipMemcpyAsync(dV, v, plane * sizeof(double), hipMemcpyHostToDevice, s));
 
+    // timing events: the solve's start and end, and six sweep pairs (the
+    // finest level's timed sweeps of one BiCGSTAB iteration)
+    hipEvent_t evSolve[2] = {nullptr, nullptr};
+    double sweepMs = 0;
+    int sweepCount = 0;
+    if (p.timing) {
+        dev.events.resize(2 + 12, nullptr);
+        for (auto& e : dev.events) TH_TRY(hipEventCreate(&e));
+        evSolve[0] = dev.events[0];
+        evSolve[1] = dev.events[1];
+        h.ev = dev.events.data() + 2;
+        h.evCap = 6;
+        TH_TRY(hipEventRecord(evSolve[0], s));
+    }
+
     // assemble, coarsen, factor
     ThAssembleArgs a{};
     a.Nx = p.Nx;
@@ -783,6 +833,7 @@ int cp_thermal_solve(const cp_thermal_params* prm, const double* gamma, const do
         k_th_fin<<<1, kThBlock, 0, s>>>(partials, nblk, F_RHO, sc);
         TH_TRY(hipGetLastError());
         TH_TRY(readScalars());
+        if (p.timing) sweepMs += h.readSweeps(&sweepCount);
         ++iters;
         const double rr = dev.hostSc[S_RR];
         if (!std::isfinite(rr)) return cpdev::reportError(CP_E_DIVERGED, "thermal residual is not finite at iteration %d", iters);
@@ -803,12 +854,20 @@ int cp_thermal_solve(const cp_thermal_params* prm, const double* gamma, const do
         relres = bb > 0 ? std::sqrt(dev.hostSc[S_RR] / bb) : 0.0;
         if (!std::isfinite(relres)) return cpdev::reportError(CP_E_DIVERGED, "thermal residual is not finite");
     }
+    if (p.timing) TH_TRY(hipEventRecord(evSolve[1], s));
     TH_TRY(hipMemcpyAsync(T, vec[V_X], n * sizeof(double), hipMemcpyDeviceToHost, s));
     TH_TRY(hipStreamSynchronize(s));
     std::memset(st, 0, sizeof(*st));
     st->iters = iters;
     st->converged = converged;
     st->relres = relres;
+    if (p.timing) {
+        float ms = 0;
+        TH_TRY(hipEventElapsedTime(&ms, evSolve[0], evSolve[1]));
+        st->device_ms = ms;
+        st->sweeps = sweepCount;
+        st->sweep_ms = sweepCount ? sweepMs / sweepCount : 0.0;
+    }
     hostStats(p, gamma, u, T, st);
     return CP_OK;
 }

@@ -423,6 +423,8 @@ typedef struct cp_thermal_params {
     double  qk;                   /* RAMP convexity: 1 */
     double  rtol;                 /* relative residual of the linear solve: 1e-6 */
     int32_t maxiter;              /* 1000 */
+    int32_t timing;               /* nonzero: fill the timing fields of cp_thermal_stats (no reference
+                                     counterpart; HIP events on the solve's stream) */
 } cp_thermal_params;
 
 /* Results of cp_thermal_solve: the solver's iterations and final relative
@@ -434,6 +436,11 @@ typedef struct cp_thermal_stats {
     double  T_base_min, T_base_max, T_fin_min, T_fin_max, T_fluid_min, T_fluid_max;
     double  T_avg_base, T_avg_fluid, T_avg_solid, T_avg_global;
     double  Q_in, Q_out, T_out, imbalance_pct;
+    /* with timing: device time of the solve (assembly to the last
+       iteration, ms) and the average duration (ms) of one finest-level
+       z-line sweep from a nonzero iterate, over `sweeps` such sweeps */
+    double  device_ms, sweep_ms;
+    int32_t sweeps;
 } cp_thermal_stats;
 
 /* Fill p with the defaults of the reference's thermal Params. */
