@@ -80,6 +80,8 @@ struct alignas(64) SlotDesc {
     int maxIterations, inletRamp, rampSteps;
     int enableSER, serMinIter, serUseMax, enableLS, lsMaxTries, enableRamp;
     int mixed;
+    int dE32;                    // dE, dN hold floats (mixed-precision implicit momentum, which forms
+                                 // them with a single-precision reciprocal): their arrays' first halves
     int dpConv, dpWindow;        // pressure-drop window exit and its length
     int slopeGate, slopeWindow;  // slope gate of the window exit and its length
     double dpPercent, slopeMaxDelta, xCoreIn, xCoreOut;

@@ -593,6 +593,7 @@ SlotDesc makeDesc(const cp_params& p, int precision, int tileRows, int mgRows0) 
     d.xCoreOut = p.xPlaneCoreOutlet;
     d.presType = p.pressureSolverType == 0 ? 0 : 1;
     d.momType = p.momentumSolverType == 0 ? 0 : 1;
+    d.dE32 = (precision == CP_PREC_MIXED && d.momType == 1) ? 1 : 0;
     d.sorOmega = sorOmegaFor(p.sorOmega, p.M, p.N);
     d.precond = d.presType == 0 ? CP_PRECOND_JACOBI : p.pressurePrecond;  // SOR slots build no hierarchy
     d.mgOmega = p.mgOmega;
@@ -726,6 +727,17 @@ int downloadPlane(cp_batch* b, const double* dsrc, int pitch, int rows, int cols
     HIP_TRY(hipMemcpy2DAsync(out, sizeof(double) * cols, dsrc, sizeof(double) * pitch, sizeof(double) * cols, rows,
                              hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
+    return CP_OK;
+}
+
+// the same for an array that holds floats (dE, dN of SlotDesc::dE32)
+int downloadPlaneF(cp_batch* b, const double* dsrc, int pitch, int rows, int cols, double* out) {
+    std::string& err = b->err;
+    std::vector<float> tmp(size_t(rows) * cols);
+    HIP_TRY(hipMemcpy2DAsync(tmp.data(), sizeof(float) * cols, dsrc, sizeof(float) * pitch, sizeof(float) * cols, rows,
+                             hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (size_t q = 0; q < tmp.size(); ++q) out[q] = tmp[q];
     return CP_OK;
 }
 
@@ -1182,8 +1194,8 @@ int getFieldSlot(cp_batch* b, int s, int which, double* out) {
         case CP_F_USTAR: return downloadPlane(b, d.us, P, M + 1, N, out);
         case CP_F_VSTAR: return downloadPlane(b, d.vs, P, M, N + 1, out);
         case CP_F_PSTAR: return downloadPlane(b, d.pc, P, M + 1, N + 1, out);
-        case CP_F_DE: return downloadPlane(b, d.dE, P, M + 1, N, out);
-        case CP_F_DN: return downloadPlane(b, d.dN, P, M, N + 1, out);
+        case CP_F_DE: return (d.dE32 ? downloadPlaneF : downloadPlane)(b, d.dE, P, M + 1, N, out);
+        case CP_F_DN: return (d.dE32 ? downloadPlaneF : downloadPlane)(b, d.dN, P, M, N + 1, out);
         case CP_F_GAMMA:
         case CP_F_ALPHA: {
             // derived on the host from the parameters, as at admission

@@ -307,12 +307,13 @@ struct MomConst {
     int M, N, pseudo, localPseudo, convScheme;
     double convScale, rho, hx, hy, De, Dn, timeStep, minSpeed, hChar, sinkDiag, vol, uvAlpha;
     gp<double> dE, dN;
+    gp<float> dE32, dN32;  // the same arrays holding floats (mixed-precision implicit rows)
     gp<T> mu[M_COUNT], mv[M_COUNT];
     __device__ __forceinline__ explicit MomConst(const SlotDesc& d)
         : M(d.M), N(d.N), pseudo(d.pseudo), localPseudo(d.localPseudo), convScheme(d.convScheme),
           convScale(d.convScale), rho(d.rho), hx(d.hx), hy(d.hy), De(d.De), Dn(d.Dn), timeStep(d.timeStep),
           minSpeed(d.minSpeed), hChar(d.hChar), sinkDiag(d.sinkDiag), vol(d.vol), uvAlpha(d.uvAlpha), dE(G(d.dE)),
-          dN(G(d.dN)) {
+          dN(G(d.dN)), dE32(G(reinterpret_cast<float*>(d.dE))), dN32(G(reinterpret_cast<float*>(d.dN))) {
 #pragma unroll
         for (int q = 0; q < M_COUNT; ++q) {
             mu[q] = G(static_cast<T*>(d.mu[q]));
@@ -400,7 +401,8 @@ __device__ __forceinline__ T momRow(const MomConst<T>& d, const double* u, const
 // (cfl hChar)) for the reference's dt = min(timeStep, cfl hChar / speed) and
 // aP = aP0 / uvAlpha by the reciprocal of uvAlpha, and dE = hy / aP0, which
 // only steers the correction and is 0 in effect at the fixed point, takes a
-// single-precision reciprocal. The stored coefficients are single precision.
+// single-precision reciprocal and is stored as a float. The stored
+// coefficients are single precision.
 struct MomScalars {
     double invTimeStep, invCflH, invUvAlpha, relaxK, rhoVol;
     float hy, hx;
@@ -444,8 +446,8 @@ __device__ __forceinline__ float momRowMixed(const MomConst<float>& d, const Mom
     Sp += (trans + m.relaxK * aP0) * cur;  // uOld == u at momentum time (iterations.cpp:118-119)
     if (d.convScheme == 1) Sp += souCorrection<IsU>(f, ks, i, j, P, d.M, d.N, Fe, Fw, Fn, Fs);
     const float rcpAP0 = __builtin_amdgcn_rcpf(float(aP0));
-    if (IsU) d.dE[k] = double(m.hy * rcpAP0);
-    else d.dN[k] = double(m.hx * rcpAP0);
+    if (IsU) d.dE32[k] = m.hy * rcpAP0;  // a float value: stored as one (SlotDesc::dE32)
+    else d.dN32[k] = m.hx * rcpAP0;
     const double r0 = Sp + aE * f[ks + 1] + aW * f[ks - 1] + aN * f[ks + P] + aS * f[ks - P] - aP * cur;
     const gp<float>* A = IsU ? d.mu : d.mv;
     const float aPt = float(aP);
@@ -934,6 +936,10 @@ __global__ __launch_bounds__(kBlock) void k_pres_assemble(KArgs a) {
         // the FP64 imbalance is read by SOR and by refinement cycles after
         // the first only
         const bool keepB = (d.presType == 0) | (d.refineCycles > 1);
+        // p' starts at 0 for SOR, which sweeps on it, and in FP64, where the
+        // CG iterate is p' itself; a mixed-precision CG slot's p' is set whole
+        // by the first refinement accumulation (p' = 0 + x)
+        const bool zeroPc = (d.presType == 0) | (d.mixed == 0);
         // all loads first (clamped addresses)
         double uE[R], uW[R], vN[R], vS[R], dEe[R], dEw[R], dNn[R], dNs[R];
 #pragma unroll
@@ -944,10 +950,28 @@ __global__ __launch_bounds__(kBlock) void k_pres_assemble(KArgs a) {
             uW[r] = us[kw];
             vN[r] = vs[kc];
             vS[r] = vs[ks];
-            dEe[r] = dE[kc];
-            dEw[r] = dE[kw];
-            dNn[r] = dN[kc];
-            dNs[r] = dN[ks];
+        }
+        if (d.dE32) {
+            const auto fE = G(reinterpret_cast<const float*>(d.dE)), fN = G(reinterpret_cast<const float*>(d.dN));
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = tl.i + r;
+                const size_t kc = clampIdx(i, j, M, P), kw = clampIdx(i, j - 1, M, P), ks = clampIdx(i - 1, j, M, P);
+                dEe[r] = fE[kc];
+                dEw[r] = fE[kw];
+                dNn[r] = fN[kc];
+                dNs[r] = fN[ks];
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = tl.i + r;
+                const size_t kc = clampIdx(i, j, M, P), kw = clampIdx(i, j - 1, M, P), ks = clampIdx(i - 1, j, M, P);
+                dEe[r] = dE[kc];
+                dEw[r] = dE[kw];
+                dNn[r] = dN[kc];
+                dNs[r] = dN[ks];
+            }
         }
         double v[2] = {0.0, 0.0};
 #pragma unroll
@@ -973,7 +997,7 @@ __global__ __launch_bounds__(kBlock) void k_pres_assemble(KArgs a) {
                 oR[k] = rr;
                 oX[k] = T(0);
                 if (keepB) bm[k] = b;
-                pc[k] = 0.0;
+                if (zeroPc) pc[k] = 0.0;
                 const T z = rr / T(aP);
                 v[1] += double(rr) * double(z);
             }
@@ -1131,6 +1155,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
         const auto Q = G(static_cast<const T*>(d.pr[P_Q]));
         const auto aP = G(static_cast<const T*>(d.pr[P_AP]));
         double v[2] = {0.0, 0.0};
+        // the Jacobi r.z (r / aP) feeds beta only without the multigrid
+        // preconditioner, whose beta follows the V-cycle: a multigrid slot
+        // neither loads aP nor divides
+        const bool jacobiZ = d.precond == 0;
         // all loads first (column pairs, clamped addresses); stores only on
         // unknowns
         Pair<T> xs[R], ps[R], rs[R], qs[R], as[R];
@@ -1141,7 +1169,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
             ps[rr] = ld2(pk, k);
             rs[rr] = ld2<T>(r, k);
             qs[rr] = ld2(Q, k);
-            as[rr] = ld2(aP, k);
+        }
+        if (jacobiZ) {
+#pragma unroll
+            for (int rr = 0; rr < R; ++rr) as[rr] = ld2(aP, clampPair32(tl.i + rr, j, M, P));
         }
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
@@ -1155,13 +1186,11 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
             rn.y = rs[rr].y - alpha * qs[rr].y;
             stMasked(x, k, xn, d0, d1);
             stMasked(r, k, rn, d0, d1);
-            if (d0) {
-                v[0] += double(rn.x) * double(rn.x);
-                v[1] += double(rn.x) * double(rn.x / as[rr].x);
-            }
-            if (d1) {
-                v[0] += double(rn.y) * double(rn.y);
-                v[1] += double(rn.y) * double(rn.y / as[rr].y);
+            if (d0) v[0] += double(rn.x) * double(rn.x);
+            if (d1) v[0] += double(rn.y) * double(rn.y);
+            if (jacobiZ) {
+                if (d0) v[1] += double(rn.x) * double(rn.x / as[rr].x);
+                if (d1) v[1] += double(rn.y) * double(rn.y / as[rr].y);
             }
         }
         blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials, nred);
@@ -1183,14 +1212,20 @@ __global__ __launch_bounds__(kBlock) void k_refine_accum(KArgs a) {
         const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
         const auto x = G(static_cast<const T*>(d.pr[P_X]));
         const auto pc = G(d.pc);
-        // all loads first (column pairs, clamped addresses)
+        // all loads first (column pairs, clamped addresses); the first cycle
+        // accumulates onto p' = 0 (the pressure assembly does not clear it)
+        const bool first = st.pcycle == 0;
         Pair<T> xs[R];
         Pair<double> ps[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const uint32_t k = clampPair32(tl.i + r, j, M, P);
             xs[r] = ld2(x, k);
-            ps[r] = ld2<double>(pc, k);
+            ps[r].x = ps[r].y = 0.0;
+        }
+        if (!first) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) ps[r] = ld2<double>(pc, clampPair32(tl.i + r, j, M, P));
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -1260,16 +1295,20 @@ __global__ __launch_bounds__(kBlock) void k_sor_sweep(KArgs a) {
         const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
         const double rho = d.rho, hx = d.hx, hy = d.hy, om = d.sorOmega;
         const auto dE = G(d.dE), dN = G(d.dN), bm = G(d.bm), pc = G(d.pc);
+        const auto fE = G(reinterpret_cast<const float*>(d.dE)), fN = G(reinterpret_cast<const float*>(d.dN));
+        const bool e32 = d.dE32 != 0;
+        auto ldE = [&](size_t k) { return e32 ? double(fE[k]) : dE[k]; };
+        auto ldN = [&](size_t k) { return e32 ? double(fN[k]) : dN[k]; };
         double mc = 0.0;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int i = tl.i + r;
             if (!(i >= 1 && i <= M - 1 && j >= 1 && j <= N - 1) || ((i + j) & 1) != a.color) continue;
             const size_t k = size_t(i) * P + j;
-            const double aE = (j < N - 1) ? rho * dE[k] * hy : 0.0;
-            const double aW = (j > 1) ? rho * dE[k - 1] * hy : 0.0;
-            const double aN = (i < M - 1) ? rho * dN[k] * hx : 0.0;
-            const double aS = (i > 1) ? rho * dN[k - P] * hx : 0.0;
+            const double aE = (j < N - 1) ? rho * ldE(k) * hy : 0.0;
+            const double aW = (j > 1) ? rho * ldE(k - 1) * hy : 0.0;
+            const double aN = (i < M - 1) ? rho * ldN(k) * hx : 0.0;
+            const double aS = (i > 1) ? rho * ldN(k - P) * hx : 0.0;
             const double aP = aE + aW + aN + aS;
             if (j == N - 1 || aP < 1e-20) {
                 pc[k] = 0.0;
@@ -2001,6 +2040,7 @@ __global__ __launch_bounds__(kBlock) void k_correct(KArgs a) {
         const double maxVel = 3.0 * fmax(fabs(Uin), 0.1);
         const double pAlpha = d.pAlpha;
         const auto gp_ = G(d.p), pc = G(d.pc), dE = G(d.dE), dN = G(d.dN);
+        const auto fE = G(reinterpret_cast<const float*>(d.dE)), fN = G(reinterpret_cast<const float*>(d.dN));
         const auto us = G(d.us), vs = G(d.vs), gam = G(d.gam), u = G(d.u), vv = G(d.v);
         // the u row at the outlet column copies its west neighbour, the v row
         // at j = N too: load the node each row reads, from clamped addresses
@@ -2012,15 +2052,29 @@ __global__ __launch_bounds__(kBlock) void k_correct(KArgs a) {
             const size_t kc = clampIdx(i, j, M, P), ku = clampIdx(i, ju, M, P), kv = clampIdx(i, jv, M, P);
             pr[rr] = gp_[kc];
             pcc[rr] = pc[kc];
-            dEu[rr] = dE[ku];
+            dEu[rr] = 0.0;
             pcu0[rr] = pc[ku];
             pcu1[rr] = pc[clampIdx(i, ju + 1, M, P)];
             usu[rr] = us[ku];
             ge[rr] = gam[clampIdx(i, 1, M, P)];  // read at j = 0 only: one address per row
-            dNv[rr] = dN[kv];
+            dNv[rr] = 0.0;
             pcv0[rr] = pc[kv];
             pcv1[rr] = pc[clampIdx(i + 1, jv, M, P)];
             vsv[rr] = vs[kv];
+        }
+        // dE, dN as floats or doubles (SlotDesc::dE32), a uniform choice
+        if (d.dE32) {
+#pragma unroll
+            for (int rr = 0; rr < R; ++rr) {
+                dEu[rr] = fE[clampIdx(tl.i + rr, ju, M, P)];
+                dNv[rr] = fN[clampIdx(tl.i + rr, jv, M, P)];
+            }
+        } else {
+#pragma unroll
+            for (int rr = 0; rr < R; ++rr) {
+                dEu[rr] = dE[clampIdx(tl.i + rr, ju, M, P)];
+                dNv[rr] = dN[clampIdx(tl.i + rr, jv, M, P)];
+            }
         }
         double bad = 0.0;
 #pragma unroll
