@@ -126,8 +126,9 @@ def parse():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run 3 device-to-device copies of MIB MiB (byte-counter calibration)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tiling (no autotuning pass)")
-    ap.add_argument("--tiling", default="", metavar="R,MG",
-                    help="fixed tiling instead of autotuning: rows per wave R and finest multigrid rows MG")
+    ap.add_argument("--tiling", default="", metavar="R,MG[,ORDER]",
+                    help="fixed tiling instead of autotuning: rows per wave R, finest multigrid rows MG and the "
+                         "tile-order mask (cp_batch_set_tile_order, default 0)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the step's kernels directly instead of replaying the captured graph")
     ap.add_argument("--profile-steps", type=int, default=0,
@@ -551,8 +552,9 @@ def main() -> int:
             x.set_graph(False)
     tuned = None
     if a.tiling:
-        r_, m_ = (int(x) for x in a.tiling.split(","))
-        tuned = {"tile_rows": r_, "mg_rows": m_, "wg_per_cu": 16, "fixed": True}
+        f_ = [int(x) for x in a.tiling.split(",")]
+        tuned = {"tile_rows": f_[0], "mg_rows": f_[1], "wg_per_cu": 16, "tile_order": f_[2] if len(f_) > 2 else 0,
+                 "fixed": True}
     elif not a.no_tune:
         # autotune the tiling on the workload's own geometries (one batch
         # alone), then start every measured step from empty batches
@@ -567,6 +569,7 @@ def main() -> int:
         for x in bs:
             x.set_tiling(tuned["tile_rows"], tuned["wg_per_cu"])
             x.set_mg_tiling(tuned["mg_rows"])
+            x.set_tile_order(tuned["tile_order"])
     for w in range(a.warmup):
         r = serve(bs, network_requests(a, a.requests), a, probe=False)
         log(f"warmup {w}: {r['elapsed']:.2f}s")

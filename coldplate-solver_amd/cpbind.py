@@ -217,7 +217,8 @@ EXPORTS = [
     ("cp_batch_set_mg_tiling", C.c_int, [_vp, C.c_int]),
     ("cp_serve", C.c_int, [_vp, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     ("cp_serve_pool", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
-    ("cp_batch_autotune", C.c_int, [_vp, C.c_int, _ip, _ip, _ip, _dp]),
+    ("cp_batch_autotune", C.c_int, [_vp, C.c_int, _ip, _ip, _ip, _ip, _dp]),
+    ("cp_batch_set_tile_order", C.c_int, [_vp, C.c_int]),
     ("cp_run_noalloc", C.c_int, [C.POINTER(Params), _dp, _dp, _dp, _dp, C.POINTER(StepStats)]),
     ("cp_thermal_params_default", C.c_int, [C.POINTER(ThermalParams)]),
     ("cp_thermal_solve", C.c_int, [C.POINTER(ThermalParams), _dp, _dp, _dp, _dp, C.POINTER(ThermalStats)]),
@@ -465,11 +466,17 @@ class Batch:
         return [{f: getattr(res[k], f) for f, _ in Result._fields_} for k in range(n)]
 
     def autotune(self, steps: int = 2) -> dict:
-        """Time every candidate tiling on the admitted slots; keep the fastest."""
-        r, m, w = C.c_int(0), C.c_int(0), C.c_int(0)
+        """Time every candidate tiling, then every tile order, on the admitted
+        slots; keep the fastest."""
+        r, m, w, o = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
         ms = C.c_double(0.0)
-        check(load().cp_batch_autotune(self.h, steps, C.byref(r), C.byref(m), C.byref(w), C.byref(ms)), self.h)
-        return {"tile_rows": r.value, "mg_rows": m.value, "wg_per_cu": w.value, "ms_per_step": ms.value}
+        check(load().cp_batch_autotune(self.h, steps, C.byref(r), C.byref(m), C.byref(w), C.byref(o), C.byref(ms)),
+              self.h)
+        return {"tile_rows": r.value, "mg_rows": m.value, "wg_per_cu": w.value, "tile_order": o.value,
+                "ms_per_step": ms.value}
+
+    def set_tile_order(self, order: int) -> None:
+        check(load().cp_batch_set_tile_order(self.h, order), self.h)
 
     def set_mg_tiling(self, rows: int) -> None:
         check(load().cp_batch_set_mg_tiling(self.h, rows), self.h)

@@ -149,6 +149,7 @@ struct KArgs {
     int force;  // k_pcg_q only: also run slots whose PCG has stopped (timing bursts)
     int level;  // multigrid kernels: hierarchy level of the tile list
     int color;  // SOR sweep: 0 = cells with i + j even, 1 = odd
+    int order;  // tile order of the launch: 0 = one contiguous range per XCD, 1 = launch order
 };
 
 // Finalisation phases (k_finalize), one per reducing work kernel.

@@ -267,6 +267,10 @@ struct cp_batch {
     int wgPerCU = 16;
     // rows per wave of the finest multigrid level's tiles (0: from tileRows)
     int mgTileRows = 0;
+    // tile order of the launches (KArgs::order), a mask: bit 0 for the node
+    // kernels, bit 1 for the p' kernels, bit 2 for the multigrid levels;
+    // 1 = launch order
+    int tileOrder = 0;
     int mgRowsEff() const {
         return mgTileRows > 0 ? mgTileRows : mgRows(precision == CP_PREC_MIXED, tileRows);
     }
@@ -314,6 +318,7 @@ KArgs kargs(cp_batch* b) {
     a.force = 0;
     a.level = 0;
     a.color = 0;
+    a.order = b->tileOrder & 1;
     return a;
 }
 
@@ -325,6 +330,7 @@ KArgs kargsFor(cp_batch* b, Kern k) {
     if (pOnly(k)) {
         a.tileSlot = b->dPTileSlot;
         a.nTiles = b->dPNTiles;
+        a.order = (b->tileOrder >> 1) & 1;
     }
     return a;
 }
@@ -355,6 +361,7 @@ int launchMg(cp_batch* b, Kern k, int level, int finalPhase = -1) {
     a.tileSlot = b->dMgTileSlot + size_t(level) * b->maxTiles;
     a.nTiles = b->dMgNTiles + level;
     a.level = level;
+    a.order = (b->tileOrder >> 2) & 1;
     const int rows = level == 0 ? b->mgRowsEff() : 2;
     const int grid = perSlot ? b->maxSlots : mgGrid(b, level);
     hipError_t e;
@@ -1565,7 +1572,8 @@ int cp_batch_set_mg_tiling(cp_batch* b, int rows) {
     return CP_OK;
 }
 
-int cp_batch_autotune(cp_batch* b, int steps, int* rows_out, int* mg_rows_out, int* wg_out, double* ms_out) {
+int cp_batch_autotune(cp_batch* b, int steps, int* rows_out, int* mg_rows_out, int* wg_out, int* order_out,
+                      double* ms_out) {
     if (!b) return setError(nullptr, CP_E_ARG, "null batch");
     std::string& err = b->err;
     if (steps < 1 || steps > 1000) return setError(&err, CP_E_ARG, "steps must be in [1, 1000]");
@@ -1576,31 +1584,42 @@ int cp_batch_autotune(cp_batch* b, int steps, int* rows_out, int* mg_rows_out, i
     const int rowsC[] = {1, 2, 4, 8};
     const int mgC[] = {2, 4};
     const int wgC[] = {4, 8, 16};
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    // one candidate: retile, capture, time `steps` replays
+    auto timed = [&](int r, int m, int w, int order, double& per) -> int {
+        b->tileRows = r;
+        b->mgTileRows = m;
+        b->wgPerCU = w;
+        b->tileOrder = order;
+        for (int s = 0; s < b->maxSlots; ++s)
+            if (b->slots[s].used) setTiling(b->hDesc[s], r, b->mgRowsEff());
+        int rc = rebuildTiles(b);
+        if (rc) return rc;
+        b->graphJ = -1;
+        if ((rc = graphRun(b, 1))) return rc;  // captures the graph for this shape
+        HIP_TRY(hipEventRecord(ev[0], b->stream));
+        if ((rc = graphRun(b, steps))) return rc;
+        HIP_TRY(hipEventRecord(ev[1], b->stream));
+        HIP_TRY(hipEventSynchronize(ev[1]));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        per = double(ms) / steps;
+        return CP_OK;
+    };
     double best = 1e300;
-    int bestR = b->tileRows, bestMg = b->mgTileRows, bestWg = b->wgPerCU;
+    int bestR = b->tileRows, bestMg = b->mgTileRows, bestWg = b->wgPerCU, bestOrder = b->tileOrder;
     int rc = CP_OK;
+    if (hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess)
+        rc = setError(&err, CP_E_HIP, "hipEventCreate failed");
+    // tile shapes in the batch's current order, then the orders with the
+    // fastest shape
+    const int order0 = b->tileOrder;
     for (int r : rowsC) {
         for (int m : mgC) {
-            if (m > (f32 ? 4 : 2)) continue;
             for (int w : wgC) {
-                b->tileRows = r;
-                b->mgTileRows = m;
-                b->wgPerCU = w;
-                for (int s = 0; s < b->maxSlots; ++s)
-                    if (b->slots[s].used) setTiling(b->hDesc[s], r, b->mgRowsEff());
-                if ((rc = rebuildTiles(b))) break;
-                b->graphJ = -1;
-                if ((rc = graphRun(b, 1))) break;  // captures the graph for this shape
-                HIP_TRY(hipEventRecord(e0, b->stream));
-                if ((rc = graphRun(b, steps))) break;
-                HIP_TRY(hipEventRecord(e1, b->stream));
-                HIP_TRY(hipEventSynchronize(e1));
-                float ms = 0.f;
-                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-                const double per = double(ms) / steps;
+                if (rc || m > (f32 ? 4 : 2)) continue;
+                double per = 0.0;
+                if ((rc = timed(r, m, w, order0, per))) break;
                 if (per < best) {
                     best = per;
                     bestR = r;
@@ -1608,16 +1627,24 @@ int cp_batch_autotune(cp_batch* b, int steps, int* rows_out, int* mg_rows_out, i
                     bestWg = w;
                 }
             }
-            if (rc) break;
         }
-        if (rc) break;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    for (int order = 0; order < 8 && !rc; ++order) {
+        if (order == order0) continue;
+        double per = 0.0;
+        if ((rc = timed(bestR, bestMg, bestWg, order, per))) break;
+        if (per < best) {
+            best = per;
+            bestOrder = order;
+        }
+    }
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
     if (rc) return rc;
     b->tileRows = bestR;
     b->mgTileRows = bestMg;
     b->wgPerCU = bestWg;
+    b->tileOrder = bestOrder;
     for (int s = 0; s < b->maxSlots; ++s)
         if (b->slots[s].used) setTiling(b->hDesc[s], bestR, b->mgRowsEff());
     if ((rc = rebuildTiles(b))) return rc;
@@ -1626,7 +1653,16 @@ int cp_batch_autotune(cp_batch* b, int steps, int* rows_out, int* mg_rows_out, i
     if (rows_out) *rows_out = bestR;
     if (mg_rows_out) *mg_rows_out = bestMg;
     if (wg_out) *wg_out = bestWg;
+    if (order_out) *order_out = bestOrder;
     if (ms_out) *ms_out = best;
+    return CP_OK;
+}
+
+int cp_batch_set_tile_order(cp_batch* b, int order) {
+    if (!b) return setError(nullptr, CP_E_ARG, "null batch");
+    if (order < 0 || order > 7) return setError(&b->err, CP_E_ARG, "tile order %d out of range [0, 7]", order);
+    b->tileOrder = order;
+    b->graphJ = -1;  // the captured launches carry the order
     return CP_OK;
 }
 

@@ -211,16 +211,26 @@ __device__ __forceinline__ Tile decodeTile(const KArgs& a, int t) {
     return tl;
 }
 
-// XCD-aware tile order. The hardware dispatches workgroup b to XCD b mod 8
-// (each XCD has its own L2), and launches use a multiple of 8 workgroups. Each
-// XCD walks one contiguous range of the tile list, so tiles that share halo
+// Tile order of a launch (KArgs::order, uniform over the launch).
+// 0, XCD-split: the hardware dispatches workgroup b to XCD b mod 8 (each XCD
+// has its own L2), and launches use a multiple of 8 workgroups. Each XCD
+// walks one contiguous range of the tile list, so tiles that share halo
 // cache lines (east/west and north/south neighbours) meet in the same L2
-// instead of fetching those lines once per XCD.
-#define XCD_TILE_LOOP(t, nT)                                                     \
-    const int xcd_ = int(blockIdx.x) % kNumXCD, lb_ = int(blockIdx.x) / kNumXCD; \
-    const int per_ = ((nT) + kNumXCD - 1) / kNumXCD;                              \
-    const int tEnd_ = min((nT), (xcd_ + 1) * per_);                               \
-    const int gx_ = int(gridDim.x) / kNumXCD;                                     \
+// instead of fetching those lines once per XCD. This pays while the arrays
+// stay in the Infinity Cache.
+// 1, launch order: workgroup b takes tiles b, b + grid, ...; the tiles in
+// flight at any moment cover one contiguous stretch of the arrays, which
+// DRAM serves faster when the working set exceeds the Infinity Cache
+// (halo lines fetched by a second XCD then come from that cache).
+// Every tile does the same work and writes its own partials in either order,
+// so results do not depend on it.
+#define XCD_TILE_LOOP(t, nT)                                                                       \
+    const bool xo_ = a.order == 0;                                                                 \
+    const int xcd_ = xo_ ? int(blockIdx.x) % kNumXCD : 0;                                          \
+    const int lb_ = xo_ ? int(blockIdx.x) / kNumXCD : int(blockIdx.x);                             \
+    const int per_ = xo_ ? ((nT) + kNumXCD - 1) / kNumXCD : (nT);                                  \
+    const int tEnd_ = min((nT), (xcd_ + 1) * per_);                                                \
+    const int gx_ = xo_ ? int(gridDim.x) / kNumXCD : int(gridDim.x);                               \
     for (int t = xcd_ * per_ + lb_; t < tEnd_; t += gx_)
 
 __device__ __forceinline__ double clampv(double x, double m) { return fmax(-m, fmin(m, x)); }

@@ -316,16 +316,29 @@ int cp_batch_set_graph(cp_batch* b, int use_graph);
    slot is retiled. */
 int cp_batch_set_mg_tiling(cp_batch* b, int rows);
 
+/* Tile order of the batch's launches (no counterpart in the reference), a
+   mask: bit 0 for the node-grid kernels (momentum, pressure assembly,
+   correction), bit 1 for the p' kernels (CG, refinement), bit 2 for the
+   multigrid levels. A clear bit walks the tile list in one contiguous range per XCD
+   (halo lines shared in that XCD's L2); a set bit walks it in launch order
+   (the tiles in flight cover one stretch of memory, faster for DRAM once
+   the working set exceeds the Infinity Cache). Results do not depend on the
+   order. Default 0. */
+int cp_batch_set_tile_order(cp_batch* b, int order);
+
 /* Autotuner (no counterpart in the reference): for every candidate tiling
    (rows per wave R in {1, 2, 4, 8}, finest-multigrid-level rows in {2, 4}
    (2 in FP64), workgroups per CU in {4, 8, 16}) replays `steps`
    outer iterations of the admitted slots, timed with HIP events on the batch
-   stream, and keeps the fastest tiling for the batch. The replays are real
-   work: every candidate advances the running slots by steps + 1 outer
-   iterations. Tilings differ only in the summation order of reductions.
-   Outputs (any may be NULL): the chosen R, multigrid rows, workgroups per
-   CU and ms per outer iteration. */
-int cp_batch_autotune(cp_batch* b, int steps, int* rows, int* mg_rows, int* wg_per_cu, double* ms_per_step);
+   stream, and keeps the fastest tiling for the batch; then it times the eight
+   tile orders (cp_batch_set_tile_order) with that tiling and keeps the
+   fastest. The replays are real work: every candidate advances the running
+   slots by steps + 1 outer iterations. Tilings differ only in the summation
+   order of reductions; tile orders not at all. Outputs (any may be NULL):
+   the chosen R, multigrid rows, workgroups per CU, tile order and ms per
+   outer iteration. */
+int cp_batch_autotune(cp_batch* b, int steps, int* rows, int* mg_rows, int* wg_per_cu, int* tile_order,
+                      double* ms_per_step);
 
 /* Kernels the measurement hooks below can time (cp_batch_probe takes a mask
    of 1 << CP_K_*). */

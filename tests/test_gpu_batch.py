@@ -148,6 +148,35 @@ def test_tile_heights_agree_with_oracle(rows):
         assert np.abs(a - ref).max() <= 1e-8 * np.abs(ref).max(), f
 
 
+@pytest.mark.parametrize("precision", [cb.CP_PREC_FP64, cb.CP_PREC_MIXED], ids=["fp64", "mixed"])
+def test_tile_orders_are_bitwise_identical(precision):
+    """Every tile-order mask (XCD-split or launch order, for the node kernels
+    and for the p' and multigrid kernels) gives bit-identical fields and inner
+    counts: a tile's work and its partials do not depend on which workgroup
+    takes it."""
+    geoms = [wl.network(64, 128, 5, precision), wl.network(48, 200, 9, precision)]
+    say(f"[batch] tile orders, precision {precision}")
+    runs = []
+    for order in range(8):
+        b = cb.Batch(precision, 128 << 20, 2)
+        b.set_tile_order(order)
+        sl = []
+        for p, ct in geoms:
+            q = wl.fast_mode(p, 3, 4, 1) if precision == cb.CP_PREC_MIXED else p
+            if precision == cb.CP_PREC_MIXED:
+                q.pressurePrecond = cb.CP_PRECOND_AMG
+            sl.append(b.admit(q, ct))
+        b.run(15)
+        info = b.poll()
+        runs.append([(b.fields(s), info[s]["itU"], info[s]["itV"], info[s]["itP"], info[s]["iter"]) for s in sl])
+        b.close()
+    for other in runs[1:]:
+        for (f0, *c0), (f1, *c1) in zip(runs[0], other):
+            assert c0 == c1
+            for x, y in zip(f0, f1):
+                np.testing.assert_array_equal(x, y)
+
+
 @pytest.mark.parametrize("kernel", [cb.K_PCG_Q, cb.K_MOM_ASM], ids=["pcg_q", "mom_assemble"])
 def test_timing_burst_leaves_results_unchanged(kernel):
     """Timing bursts of k_pcg_q or k_mom_assemble between replays (as
@@ -187,6 +216,7 @@ def test_autotune_keeps_results():
         if tune:
             t = b.autotune(1)
             assert t["tile_rows"] in (1, 2, 4, 8) and t["mg_rows"] in (2, 4) and t["wg_per_cu"] in (4, 8, 16)
+            assert 0 <= t["tile_order"] <= 7
             assert t["ms_per_step"] > 0
         for _ in range(400):
             b.run(50)
