@@ -75,8 +75,8 @@ PEAK_HBM_GBS = 8000.0
 DP_TOL = 1e-4
 # algorithmic bytes per work unit of the timed kernels (mixed precision)
 PCG_Q_BYTES = 28    # per p' unknown: read z, p_old, aP, cE, cN; write p_new, q (4 B each)
-MOM_ASM_BYTES = 104  # per grid node: read u, v, p and the Brinkman faces aU, aV (8 B each); write dE, dN
-#                      (4 B each) and per system the 5 coefficients, r0 and the first sweep's delta (4 B each)
+MOM_ASM_BYTES = 72  # per grid node: read u, v, p (8 B each) and the Brinkman faces aU, aV (4 B each); write
+#                     dE, dN (4 B each) and per system aP, the face fluxes Fe, Fn and r0 (4 B each)
 GAMMA_MIN = 1.0 - 1e-6  # reference plane row rule (binary geometry)
 GOLDEN = REPO / "tests" / "golden" / "network_1024x512_dp.json"
 TRAFFIC = REPO / "profiles" / "counters_network_1024x512.json"
@@ -98,7 +98,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=6)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=None, help="slots per batch (default 8)")
+    ap.add_argument("--batch", type=int, default=None, help="slots per batch (default 12; 8 for --workload stream)")
     ap.add_argument("--streams", type=int, default=None,
                     help="batches served at once (cp_serve_pool), each on its own HIP stream (default 3)")
     ap.add_argument("--requests", type=int, default=None,
@@ -535,7 +535,7 @@ def main() -> int:
         return 0
     if a.calibrate_copy:
         copy_calibration(a.calibrate_copy)
-    a.batch = a.batch or 8
+    a.batch = a.batch or 12
     a.streams = a.streams or 3
     if a.streams < 1:
         raise SystemExit("--streams must be at least 1")

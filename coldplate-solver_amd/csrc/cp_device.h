@@ -44,7 +44,12 @@ constexpr int ST_MAXITER = 3;
 constexpr int ST_FAILED = 4;
 
 // Solver-typed arrays (T = double in FP64 mode, float in mixed mode).
-enum MomArr { M_AP = 0, M_AE, M_AW, M_AN, M_AS, M_R0, M_D0, M_D1, M_COUNT };
+// Momentum system (U or V): the relaxed diagonal aP, the east and north face
+// fluxes Fe, Fn of each row (its neighbour coefficients are De + max(0, -Fe),
+// De + max(0, Fe of the west row), Dn + max(0, -Fn), Dn + max(0, Fn of the
+// south row): a face's flux is shared by the rows on either side of it), the
+// initial residual r0 and two buffers of the Jacobi correction delta.
+enum MomArr { M_AP = 0, M_FE, M_FN, M_R0, M_D0, M_D1, M_COUNT };
 // P_Z: preconditioned residual z = M^-1 r (multigrid); P_MGX: the finest
 // level's pre-smoothed iterate. P_X stays last: FP64 aliases it to p'.
 enum PresArr { P_CE = 0, P_CN, P_AP, P_R, P_PK0, P_PK1, P_Q, P_Z, P_MGX, P_X, P_COUNT };
@@ -81,7 +86,8 @@ struct alignas(64) SlotDesc {
     int enableSER, serMinIter, serUseMax, enableLS, lsMaxTries, enableRamp;
     int mixed;
     int dE32;                    // dE, dN hold floats (mixed-precision implicit momentum, which forms
-                                 // them with a single-precision reciprocal): their arrays' first halves
+                                 // them with a single-precision reciprocal), and so do the Brinkman
+                                 // faces aU, aV: their arrays' first halves
     int dpConv, dpWindow;        // pressure-drop window exit and its length
     int slopeGate, slopeWindow;  // slope gate of the window exit and its length
     double dpPercent, slopeMaxDelta, xCoreIn, xCoreOut;

@@ -861,8 +861,17 @@ int batchAdmit(cp_batch* b, const cp_params* prm, const double* ct, int* slotOut
     std::vector<double> aU, aV, gam;
     buildGeometry(*prm, ct, d.pitch, alphaMaxOf(*prm), aU, aV, gam, nullptr, nullptr);
     const size_t plane = size_t(prm->M + 1) * d.pitch;
-    HIP_TRY(hipMemcpyAsync(d.aU, aU.data(), plane * sizeof(double), hipMemcpyHostToDevice, b->stream));
-    HIP_TRY(hipMemcpyAsync(d.aV, aV.data(), plane * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    std::vector<float> aU32, aV32;
+    if (d.dE32) {
+        // mixed-precision implicit slots read the Brinkman faces as floats
+        aU32.assign(aU.begin(), aU.end());
+        aV32.assign(aV.begin(), aV.end());
+        HIP_TRY(hipMemcpyAsync(d.aU, aU32.data(), plane * sizeof(float), hipMemcpyHostToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(d.aV, aV32.data(), plane * sizeof(float), hipMemcpyHostToDevice, b->stream));
+    } else {
+        HIP_TRY(hipMemcpyAsync(d.aU, aU.data(), plane * sizeof(double), hipMemcpyHostToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(d.aV, aV.data(), plane * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    }
     HIP_TRY(hipMemcpyAsync(d.gam, gam.data(), plane * sizeof(double), hipMemcpyHostToDevice, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
     SlotHost& sh = b->slots[s];

@@ -333,15 +333,17 @@ struct MomConst {
 };
 
 // Assembles one U (IsU) or V row at node k: writes the relaxed diagonal, the
-// four neighbour coefficients, dE/dN and the initial residual r0 = b - A u
-// of the Jacobi system (the sweeps iterate on the correction delta = x - u
-// from delta = 0, which is the reference's x0 = u), and returns r0 as
-// stored. With `sweep` it also makes the first Jacobi sweep: from delta = 0
-// the sweep's residual is r0 and its iterate 0 + r0 / aP, in the arithmetic
-// of k_jacobi (T, the stored aP), written to the second delta buffer.
-// Without it (no sweeps) the first buffer, which the momentum finish then
-// reads, is zeroed. u, v, p are read from the workgroup's LDS tile (index
-// ks, row pitch P); k indexes the node's global arrays.
+// east and north face fluxes (the row's four neighbour coefficients follow
+// from them and from the west and south rows' fluxes, MomArr), dE/dN and the
+// initial residual r0 = b - A u of the Jacobi system (the sweeps iterate on
+// the correction delta = x - u from delta = 0, which is the reference's
+// x0 = u), and returns r0 as stored. With `sweep` the row also counts as the
+// first Jacobi sweep: from delta = 0 the sweep's residual is r0 and its
+// iterate delta_1 = 0 + r0 / aP (T arithmetic, the stored values), which is
+// not stored: the next sweep and the momentum finish form it from r0 and aP.
+// Without it (no sweeps) the first delta buffer, which the momentum finish
+// then reads, is zeroed. u, v, p are read from the workgroup's LDS tile
+// (index ks, row pitch P); k indexes the node's global arrays.
 template <typename T, bool IsU>
 __device__ __forceinline__ T momRow(const MomConst<T>& d, const double* u, const double* v, const double* pp,
                                     size_t ks, int P, size_t k, int i, int j, double maxVel, double cfl,
@@ -390,14 +392,11 @@ __device__ __forceinline__ T momRow(const MomConst<T>& d, const double* u, const
     const double r0 = Sp + aE * f[ks + 1] + aW * f[ks - 1] + aN * f[ks + P] + aS * f[ks - P] - aP * cur;
     const gp<T>* A = IsU ? d.mu : d.mv;
     A[M_AP][k] = T(aP);
-    A[M_AE][k] = T(aE);
-    A[M_AW][k] = T(aW);
-    A[M_AN][k] = T(aN);
-    A[M_AS][k] = T(aS);
-    const T r0t = T(r0), aPt = T(aP);
+    A[M_FE][k] = T(Fe);
+    A[M_FN][k] = T(Fn);
+    const T r0t = T(r0);
     A[M_R0][k] = r0t;
-    if (sweep) A[M_D1][k] = T(0) + r0t / aPt;
-    else A[M_D0][k] = T(0);
+    if (!sweep) A[M_D0][k] = T(0);
     return r0t;
 }
 
@@ -411,8 +410,8 @@ __device__ __forceinline__ T momRow(const MomConst<T>& d, const double* u, const
 // (cfl hChar)) for the reference's dt = min(timeStep, cfl hChar / speed) and
 // aP = aP0 / uvAlpha by the reciprocal of uvAlpha, and dE = hy / aP0, which
 // only steers the correction and is 0 in effect at the fixed point, takes a
-// single-precision reciprocal and is stored as a float. The stored
-// coefficients are single precision.
+// single-precision reciprocal and is stored as a float. The stored diagonal
+// and face fluxes are single precision.
 struct MomScalars {
     double invTimeStep, invCflH, invUvAlpha, relaxK, rhoVol;
     float hy, hx;
@@ -460,16 +459,12 @@ __device__ __forceinline__ float momRowMixed(const MomConst<float>& d, const Mom
     else d.dN32[k] = m.hx * rcpAP0;
     const double r0 = Sp + aE * f[ks + 1] + aW * f[ks - 1] + aN * f[ks + P] + aS * f[ks - P] - aP * cur;
     const gp<float>* A = IsU ? d.mu : d.mv;
-    const float aPt = float(aP);
-    A[M_AP][k] = aPt;
-    A[M_AE][k] = float(aE);
-    A[M_AW][k] = float(aW);
-    A[M_AN][k] = float(aN);
-    A[M_AS][k] = float(aS);
+    A[M_AP][k] = float(aP);
+    A[M_FE][k] = float(Fe);
+    A[M_FN][k] = float(Fn);
     const float r0t = float(r0);
     A[M_R0][k] = r0t;
-    if (sweep) A[M_D1][k] = 0.0f + r0t / aPt;
-    else A[M_D0][k] = 0.0f;
+    if (!sweep) A[M_D0][k] = 0.0f;
     return r0t;
 }
 
@@ -528,12 +523,24 @@ __global__ __launch_bounds__(kBlock, momWaves<R>()) void k_mom_assemble(KArgs a)
             lv[it] = gv[k];
             lp[it] = gpp[k];
         }
+        // the Brinkman faces: floats in mixed-precision slots (SlotDesc::dE32,
+        // set for every implicit slot of a single-precision batch)
         double aUr[R], aVr[R];
+        if constexpr (sizeof(T) == 4) {
+            const auto fU = G(reinterpret_cast<const float*>(d.aU)), fV = G(reinterpret_cast<const float*>(d.aV));
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const size_t k = clampIdx(tl.i + r, j, M, P);
-            aUr[r] = G(d.aU)[k];
-            aVr[r] = G(d.aV)[k];
+            for (int r = 0; r < R; ++r) {
+                const size_t k = clampIdx(tl.i + r, j, M, P);
+                aUr[r] = fU[k];
+                aVr[r] = fV[k];
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const size_t k = clampIdx(tl.i + r, j, M, P);
+                aUr[r] = G(d.aU)[k];
+                aVr[r] = G(d.aV)[k];
+            }
         }
         __syncthreads();  // the previous tile's LDS reads are done
 #pragma unroll
@@ -763,46 +770,101 @@ __device__ __forceinline__ T at(const T* f, int i, int j, int M, int jmax, int P
     return (i >= 0 && i <= M && j >= 0 && j <= jmax) ? f[size_t(i) * P + j] : T(0);
 }
 
+// Jacobi correction of one system at node (i, j) for the virtual first
+// iterate delta_1 = r0 / aP (the assembly's sweep, not stored): 0 outside the
+// system's unknowns i in [1, iMax], j in [1, jMax], whose r0 and aP the
+// caller loaded from clamped addresses.
+template <typename T>
+__device__ __forceinline__ T delta1(T r0, T aP, int i, int j, int iMax, int jMax) {
+    return (i >= 1 && i <= iMax && j >= 1 && j <= jMax) ? T(0) + r0 / aP : T(0);
+}
+
+// The arrays of one momentum system a sweep reads.
+template <typename T>
+struct JacSys {
+    gp<const T> aP, fe, fn, r0;
+};
+
 // One Jacobi sweep of one system for a wave's R rows x 64 columns, all loads
 // issued up front: delta for the rows and the rows above and below (lanes
 // 0-31 also at column j0-1, lanes 32-63 at column j0+64, for the lanes at
-// the wave's edges), and the coefficients. delta is zero outside each
-// system's unknowns and is taken as zero outside the node grid, so the
-// product needs no masks.
-template <typename T, int R>
-__device__ __forceinline__ void jacobiWave(const Tile& tl, int M, int N, int P, int iMax, int jMax,
-                                           gp<const T> const (&cf)[M_D0], gp<const T> src, gp<T> dst,
-                                           double& v) {
+// the wave's edges), aP and r0 of the rows, the east face fluxes of the rows
+// (and of column j0-1 for lane 0's west face) and the north face fluxes of
+// the rows and of the row below. The neighbour coefficients are formed from
+// the fluxes as the assembly formed them (MomArr). delta is zero outside
+// each system's unknowns and is taken as zero outside the node grid, so the
+// product needs no masks; the coefficients of faces towards non-unknowns
+// (whose fluxes the assembly leaves at 0) multiply those zeros. FromR0: the
+// sweep after the assembly's, whose iterate delta_1 = r0 / aP is formed here
+// for the rows, the rows above and below and the edge columns.
+template <typename T, int R, bool FromR0>
+__device__ __forceinline__ void jacobiWave(const Tile& tl, int M, int N, int P, int iMax, int jMax, T De, T Dn,
+                                           const JacSys<T>& sy, gp<const T> src, gp<T> dst, double& v) {
     const int lane = int(threadIdx.x & 63);
     const int j = tl.j, ib = tl.i;
     const int je = lane < 32 ? tl.j0 - 1 : tl.j0 + kTileW;
-    T dl[R + 2], de[R], c[R][M_D0];
+    T dl[R + 2], de[R], ap[R], r0[R], fe[R], fw0[R], fn[R + 1];
+    if constexpr (FromR0) {
+        T rh[R + 2], ah[R + 2], re[R], ae[R];
 #pragma unroll
-    for (int h = 0; h < R + 2; ++h) dl[h] = src[clampIdx(ib - 1 + h, j, M, P)];
+        for (int h = 0; h < R + 2; ++h) {
+            const size_t k = clampIdx(ib - 1 + h, j, M, P);
+            rh[h] = sy.r0[k];
+            ah[h] = sy.aP[k];
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const size_t k = clampIdx(ib + r, je, M, P);
+            re[r] = sy.r0[k];
+            ae[r] = sy.aP[k];
+        }
+#pragma unroll
+        for (int h = 0; h < R + 2; ++h) dl[h] = delta1(rh[h], ah[h], ib - 1 + h, j, iMax, jMax);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            de[r] = delta1(re[r], ae[r], ib + r, je, iMax, jMax);
+            ap[r] = ah[r + 1];
+            r0[r] = rh[r + 1];
+        }
+    } else {
+#pragma unroll
+        for (int h = 0; h < R + 2; ++h) dl[h] = src[clampIdx(ib - 1 + h, j, M, P)];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            de[r] = src[clampIdx(ib + r, je, M, P)];
+            const size_t k = clampIdx(ib + r, j, M, P);
+            ap[r] = sy.aP[k];
+            r0[r] = sy.r0[k];
+        }
+        const bool jIn = j >= 0 && j <= N, jeIn = je >= 0 && je <= N;
+#pragma unroll
+        for (int h = 0; h < R + 2; ++h) dl[h] = (jIn && ib - 1 + h >= 0 && ib - 1 + h <= M) ? dl[h] : T(0);
+#pragma unroll
+        for (int r = 0; r < R; ++r) de[r] = (jeIn && ib + r <= M) ? de[r] : T(0);
+    }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        de[r] = src[clampIdx(ib + r, je, M, P)];
-        const size_t k = clampIdx(ib + r, j, M, P);
-#pragma unroll
-        for (int q = 0; q < M_D0; ++q) c[r][q] = cf[q][k];
+        fe[r] = sy.fe[clampIdx(ib + r, j, M, P)];
+        fw0[r] = sy.fe[clampIdx(ib + r, tl.j0 - 1, M, P)];
     }
-    const bool jIn = j >= 0 && j <= N, jeIn = je >= 0 && je <= N;
 #pragma unroll
-    for (int h = 0; h < R + 2; ++h) dl[h] = (jIn && ib - 1 + h >= 0 && ib - 1 + h <= M) ? dl[h] : T(0);
-#pragma unroll
-    for (int r = 0; r < R; ++r) de[r] = (jeIn && ib + r <= M) ? de[r] : T(0);
+    for (int h = 0; h < R + 1; ++h) fn[h] = sy.fn[clampIdx(ib - 1 + h, j, M, P)];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int i = ib + r;
         const T dc = dl[r + 1];
-        T dW = fromWest(dc), dE = fromEast(dc);
+        T dW = fromWest(dc), dE = fromEast(dc), fw = fromWest(fe[r]);
         dW = lane == 0 ? de[r] : dW;
         dE = lane == 63 ? de[r] : dE;
+        fw = lane == 0 ? fw0[r] : fw;
         if (i >= 1 && i <= iMax && j >= 1 && j <= jMax) {
-            const T ap = c[r][M_AP];
-            const T Ax = ap * dc - c[r][M_AE] * dE - c[r][M_AW] * dW - c[r][M_AN] * dl[r + 2] - c[r][M_AS] * dl[r];
-            const T res = c[r][M_R0] - Ax;
-            dst[size_t(i) * P + j] = dc + res / ap;
+            const T aE = De + fmax(T(0), -fe[r]);
+            const T aW = De + fmax(T(0), fw);
+            const T aN = Dn + fmax(T(0), -fn[r + 1]);
+            const T aS = Dn + fmax(T(0), fn[r]);
+            const T Ax = ap[r] * dc - aE * dE - aW * dW - aN * dl[r + 2] - aS * dl[r];
+            const T res = r0[r] - Ax;
+            dst[size_t(i) * P + j] = dc + res / ap[r];
             v += double(res) * double(res);
         }
     }
@@ -810,7 +872,8 @@ __device__ __forceinline__ void jacobiWave(const Tile& tl, int M, int N, int P, 
 
 // Jacobi sweep: U and V in the same launch, each system skipped once its
 // stop test has fired. U unknowns: i in [1, M-1], j in [1, N-2]; V: i in
-// [1, M-2], j in [1, N-1].
+// [1, M-2], j in [1, N-1]. After sweep s the iterate is in delta buffer
+// s mod 2, except delta_1 (the assembly's), which is formed from r0 / aP.
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_jacobi(KArgs a) {
     const int nT = *a.nTiles;
@@ -822,20 +885,28 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(KArgs a) {
         const int status = st.status, doneU = st.jdone[0], doneV = st.jdone[1], jkU = st.jk[0], jkV = st.jk[1];
         if ((status != ST_RUNNING) | ((doneU != 0) & (doneV != 0))) continue;
         const int M = d.M, N = d.N, P = d.pitch;
+        const T De = T(d.De), Dn = T(d.Dn);
         const int pu = jkU & 1, pv = jkV & 1;
-        gp<const T> cu[M_D0], cv[M_D0];
-#pragma unroll
-        for (int q = 0; q < M_D0; ++q) {
-            cu[q] = G(static_cast<const T*>(d.mu[q]));
-            cv[q] = G(static_cast<const T*>(d.mv[q]));
-        }
+        JacSys<T> su, sv;
+        su.aP = G(static_cast<const T*>(d.mu[M_AP]));
+        su.fe = G(static_cast<const T*>(d.mu[M_FE]));
+        su.fn = G(static_cast<const T*>(d.mu[M_FN]));
+        su.r0 = G(static_cast<const T*>(d.mu[M_R0]));
+        sv.aP = G(static_cast<const T*>(d.mv[M_AP]));
+        sv.fe = G(static_cast<const T*>(d.mv[M_FE]));
+        sv.fn = G(static_cast<const T*>(d.mv[M_FN]));
+        sv.r0 = G(static_cast<const T*>(d.mv[M_R0]));
+        const auto srcU = G(static_cast<const T*>(d.mu[M_D0 + pu])), srcV = G(static_cast<const T*>(d.mv[M_D0 + pv]));
+        const auto dstU = G(static_cast<T*>(d.mu[M_D0 + (pu ^ 1)])), dstV = G(static_cast<T*>(d.mv[M_D0 + (pv ^ 1)]));
         double v[2] = {0.0, 0.0};
-        if (!doneU)
-            jacobiWave<T, R>(tl, M, N, P, M - 1, N - 2, cu, G(static_cast<const T*>(d.mu[M_D0 + pu])),
-                             G(static_cast<T*>(d.mu[M_D0 + (pu ^ 1)])), v[0]);
-        if (!doneV)
-            jacobiWave<T, R>(tl, M, N, P, M - 2, N - 1, cv, G(static_cast<const T*>(d.mv[M_D0 + pv])),
-                             G(static_cast<T*>(d.mv[M_D0 + (pv ^ 1)])), v[1]);
+        if (!doneU) {
+            if (jkU == 1) jacobiWave<T, R, true>(tl, M, N, P, M - 1, N - 2, De, Dn, su, srcU, dstU, v[0]);
+            else jacobiWave<T, R, false>(tl, M, N, P, M - 1, N - 2, De, Dn, su, srcU, dstU, v[0]);
+        }
+        if (!doneV) {
+            if (jkV == 1) jacobiWave<T, R, true>(tl, M, N, P, M - 2, N - 1, De, Dn, sv, srcV, dstV, v[1]);
+            else jacobiWave<T, R, false>(tl, M, N, P, M - 2, N - 1, De, Dn, sv, srcV, dstV, v[1]);
+        }
         blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials, nred);
     }
 }
@@ -875,11 +946,39 @@ __global__ __launch_bounds__(kBlock) void k_mom_finish(KArgs a) {
             um[r] = u[kw];
             vk[r] = vv[kc];
             vm[r] = vv[kw];
-            duk[r] = dU[kc];
-            dum[r] = dU[kw];
-            dvk[r] = dV[kc];
-            dvm[r] = dV[kw];
             ge[r] = gam[clampIdx(i, 1, M, P)];  // read at j = 0 only: one address per row
+        }
+        // delta of each system: after one sweep (the assembly's) the iterate
+        // r0 / aP, else the delta buffer of the last sweep. Every load is
+        // issued with the others, from bases chosen per slot: with a stored
+        // delta the second base is the delta buffer again, whose second load
+        // hits the line the first one fetched.
+        const bool oneU = itU == 1, oneV = itV == 1;
+        const auto xU = oneU ? G(static_cast<const T*>(d.mu[M_R0])) : dU;
+        const auto yU = oneU ? G(static_cast<const T*>(d.mu[M_AP])) : dU;
+        const auto xV = oneV ? G(static_cast<const T*>(d.mv[M_R0])) : dV;
+        const auto yV = oneV ? G(static_cast<const T*>(d.mv[M_AP])) : dV;
+        T xuk[R], yuk[R], xum[R], yum[R], xvk[R], yvk[R], xvm[R], yvm[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = tl.i + r;
+            const size_t kc = clampIdx(i, j, M, P), kw = clampIdx(i, j - 1, M, P);
+            xuk[r] = xU[kc];
+            yuk[r] = yU[kc];
+            xum[r] = xU[kw];
+            yum[r] = yU[kw];
+            xvk[r] = xV[kc];
+            yvk[r] = yV[kc];
+            xvm[r] = xV[kw];
+            yvm[r] = yV[kw];
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = tl.i + r;
+            duk[r] = oneU ? delta1(xuk[r], yuk[r], i, j, M - 1, N - 2) : xuk[r];
+            dum[r] = oneU ? delta1(xum[r], yum[r], i, j - 1, M - 1, N - 2) : xum[r];
+            dvk[r] = oneV ? delta1(xvk[r], yvk[r], i, j, M - 2, N - 1) : xvk[r];
+            dvm[r] = oneV ? delta1(xvm[r], yvm[r], i, j - 1, M - 2, N - 1) : xvm[r];
         }
         double v[3] = {0.0, 0.0, 0.0};
 #pragma unroll

@@ -494,7 +494,7 @@ int cp_thermal_solve(const cp_thermal_params* p, const double* gamma, const doub
    u, v, p (reference shapes, row-major, host): the initial fields when
    non-NULL (else zero), overwritten with the final ones. A grid whose solve
    does not fit CP_NOALLOC_SCRATCH_BYTES gets CP_E_CAPACITY (an FP64 solve
-   needs about 300 bytes per node of the (M+1) x (N+1) grid). Runs on the
+   needs about 270 bytes per node of the (M+1) x (N+1) grid). Runs on the
    null stream and returns when the solve has ended (error <= epsilon,
    maxIterations, or a non-finite field: health in *last). Calls that
    overlap on one device share the scratch and race; the caller serialises
