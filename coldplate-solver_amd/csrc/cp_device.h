@@ -160,7 +160,7 @@ struct KArgs {
 
 // Finalisation phases (k_finalize), one per reducing work kernel.
 enum Phase {
-    PH_JAC = 0, PH_MOM_FIN, PH_PRES_ASM, PH_PCG_Q, PH_PCG_U, PH_REF_ACC, PH_REF_RES, PH_CORR, PH_MG_Z,
+    PH_JAC = 0, PH_MOM_FIN, PH_PRES_ASM, PH_PCG_U, PH_REF_ACC, PH_REF_RES, PH_CORR, PH_MG_Z,
     PH_MOM_EXPL,  // explicit momentum: residuals
     PH_SOR,       // SOR: stop test after a red + black sweep pair
     PH_PCGU_MGZ,  // PH_PCG_U's stop test, then PH_MG_Z, after the V-cycle that follows the update
@@ -169,6 +169,10 @@ enum Phase {
 // Partial entry of the finest up-sweep's r.z: apart from the PCG update's
 // entries 0 and 1, which PH_PCGU_MGZ reads after the V-cycle.
 constexpr int kMgZPartial = 2;
+// Partial entry of k_pcg_q's p.q: every workgroup of the k_pcg_update that
+// follows sums the slot's entries while the update's own partials (entries
+// 0 and 1) are being written.
+constexpr int kPcgQPartial = 3;
 
 // Kernels of one SIMPLE step, in launch order.
 enum Kern {

@@ -1238,13 +1238,18 @@ __global__ __launch_bounds__(kBlock) void k_pcg_q(KArgs a) {
             if (kk == 0) pcgQWave<T, R, false, true>(d, tl, kk, beta, v[0]);
             else pcgQWave<T, R, false, false>(d, tl, kk, beta, v[0]);
         }
-        blockReduceTo<1, 0u>(v, a.partials + size_t(t) * kNumPartials, nred);
+        blockReduceTo<1, 0u>(v, a.partials + size_t(t) * kNumPartials + kPcgQPartial, nred);
     }
 }
 
 // ---------------------------------------------------------------------------
-// PCG iteration, part 2: x += alpha p, r -= alpha q, partial r.r and r.z;
-// stop on ||r||_2 < pTol or the iteration cap (pressure_solver.cpp:159-182)
+// PCG iteration, part 2: alpha = r.z / p.q, x += alpha p, r -= alpha q,
+// partial r.r and r.z; stop on ||r||_2 < pTol or the iteration cap
+// (pressure_solver.cpp:157-182). Every workgroup sums its slot's p.q partials
+// itself, in the finalisation's order and block shape (slotReduce), so each
+// gets the same alpha bit for bit and no finalisation launch sits between
+// the two kernels. On |p.q| < 1e-20 (pressure_solver.cpp:158) every
+// workgroup skips the slot and the one holding its first tile ends its CG.
 // ---------------------------------------------------------------------------
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
@@ -1257,7 +1262,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
         const int status = st.status, pdone = st.pdone, phase = st.stepPhase, kk = st.pk;
         if ((status != ST_RUNNING) | (pdone != 0) | (phase != 0) | (kk < 0)) continue;
         const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
-        const T alpha = T(st.alpha);
+        const double rz = st.rz;
         const auto pk = G(static_cast<const T*>(d.pr[P_PK0 + ((kk & 1) ^ 1)]));
         const auto x = G(static_cast<T*>(d.pr[P_X]));
         const auto r = G(static_cast<T*>(d.pr[P_R]));
@@ -1283,6 +1288,19 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
 #pragma unroll
             for (int rr = 0; rr < R; ++rr) as[rr] = ld2(aP, clampPair32(tl.i + rr, j, M, P));
         }
+        // p.q of the slot, while the loads above are in flight
+        double pq[1];
+        slotReduce<1, 0u>(a.partials + kPcgQPartial, d.ptile0, d.pntiles, pq);
+        if (fabs(pq[0]) < 1e-20) {
+            if (t == d.ptile0 && threadIdx.x == 0) {
+                st.pdone = 1;
+                st.piters = kk;
+            }
+            continue;
+        }
+        const double alphaD = rz / pq[0];
+        if (t == d.ptile0 && threadIdx.x == 0) st.alpha = alphaD;
+        const T alpha = T(alphaD);
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
             const int i = tl.i + rr;
@@ -2476,7 +2494,6 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
         case PH_MG_Z:
             if (!d.precond) return;
             [[fallthrough]];
-        case PH_PCG_Q:
         case PH_PCG_U:
         case PH_PCGU_MGZ:
             if (st.pdone || st.stepPhase) return;
@@ -2560,19 +2577,6 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
                 } else {
                     st.pdone = (d.maxPresIter <= 0) ? 1 : 0;
                     st.stepPhase = 0;  // refinement not finished
-                }
-            }
-            break;
-        }
-        case PH_PCG_Q: {
-            double tot[1];
-            slotReduce<1, 0u>(a.partials, d.ptile0, d.pntiles, tot);
-            if (threadIdx.x == 0) {
-                if (fabs(tot[0]) < 1e-20) {  // pressure_solver.cpp:158
-                    st.pdone = 1;
-                    st.piters = st.pk;
-                } else {
-                    st.alpha = st.rz / tot[0];
                 }
             }
             break;
@@ -2697,8 +2701,7 @@ static int phaseOf(Kern k) {
         case K_JAC: return PH_JAC;
         case K_MOM_FIN: return PH_MOM_FIN;
         case K_PRES_ASM: return PH_PRES_ASM;
-        case K_PCG_Q: return PH_PCG_Q;
-        case K_PCG_U: return PH_PCG_U;
+        case K_PCG_U: return PH_PCG_U;  // k_pcg_q's p.q is summed by k_pcg_update itself
         case K_REF_ACC: return PH_REF_ACC;
         case K_REF_RES: return PH_REF_RES;
         case K_CORR:
