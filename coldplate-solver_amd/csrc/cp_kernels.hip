@@ -126,6 +126,22 @@ __device__ __forceinline__ void blockReduceTo(double (&v)[NQ], double* dst) {
     blockReduceTo<NQ, MaxMask>(v, dst, calls);
 }
 
+// Sum of one partial entry over a slot's tiles for a work kernel that needs
+// the slot's reduction itself: thread t adds tiles t, t + 256, ... in order
+// (one load per thread for up to 256 tiles), then the block sum. Every
+// workgroup of a launch gets the same value bit for bit. The result is valid
+// in every thread.
+__device__ __forceinline__ double slotSumLean(const double* partials, int tile0, int ntiles) {
+    __shared__ double res[1];
+    double v = 0.0;
+    for (int k = int(threadIdx.x); k < ntiles; k += kBlock) v += G(partials)[size_t(tile0 + k) * kNumPartials];
+    double vv[1] = {v};
+    __syncthreads();
+    blockReduceTo<1, 0u>(vv, res);
+    __syncthreads();
+    return res[0];
+}
+
 // Sums (or maxes) the partials of one slot's tiles in tile order (ntiles >=
 // 1); the result is valid in every thread of the block. Each thread issues its loads in
 // groups of 8 before combining them, so the latency is paid once per group.
@@ -1246,9 +1262,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_q(KArgs a) {
 // PCG iteration, part 2: alpha = r.z / p.q, x += alpha p, r -= alpha q,
 // partial r.r and r.z; stop on ||r||_2 < pTol or the iteration cap
 // (pressure_solver.cpp:157-182). Every workgroup sums its slot's p.q partials
-// itself, in the finalisation's order and block shape (slotReduce), so each
-// gets the same alpha bit for bit and no finalisation launch sits between
-// the two kernels. On |p.q| < 1e-20 (pressure_solver.cpp:158) every
+// itself in one fixed order (slotSumLean), so each gets the same alpha bit
+// for bit and no finalisation launch sits between the two kernels. On |p.q| < 1e-20 (pressure_solver.cpp:158) every
 // workgroup skips the slot and the one holding its first tile ends its CG.
 // ---------------------------------------------------------------------------
 template <typename T, int R>
@@ -1290,7 +1305,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
         }
         // p.q of the slot, while the loads above are in flight
         double pq[1];
-        slotReduce<1, 0u>(a.partials + kPcgQPartial, d.ptile0, d.pntiles, pq);
+        pq[0] = slotSumLean(a.partials + kPcgQPartial, d.ptile0, d.pntiles);
         if (fabs(pq[0]) < 1e-20) {
             if (t == d.ptile0 && threadIdx.x == 0) {
                 st.pdone = 1;
