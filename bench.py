@@ -614,11 +614,16 @@ def main() -> int:
     if stream5:
         log(f"served-stream subsample: {stream5}")
     traffic = {}
-    if TRAFFIC.exists():
+    if TRAFFIC.exists() and tuned and (tuned["tile_rows"], tuned["mg_rows"]) == (4, 4):
         # HBM bytes per launch, from the committed rocprofv3 FETCH_SIZE /
-        # WRITE_SIZE runs of this workload at this pool shape
+        # WRITE_SIZE runs of this workload at this batch size and tiling, in
+        # the tile order this run used for the kernel's family
         t = json.loads(TRAFFIC.read_text())
-        traffic = {k: int(v["traffic_bytes_per_launch"]) for k, v in t.get("kernels", {}).items()}
+        for k, v in t.get("kernels", {}).items():
+            bit = (tuned.get("tile_order", 0) >> t["tile_order_bit"][k]) & 1
+            e = v["by_launch_order"].get(str(bit))
+            if e and t["workload"].startswith(f"network_1024x512, one {a.batch}-slot batch"):
+                traffic[k] = int(e["traffic_bytes_per_launch"])
     ws_pcg = PCG_Q_BYTES * slots * (GEOMS[0][0].M - 1) * (GEOMS[0][0].N - 2)
     ws_mom = MOM_ASM_BYTES * slots * (GEOMS[0][0].M + 1) * (GEOMS[0][0].N + 1)
     g0 = GEOMS[0][0]
