@@ -1263,8 +1263,11 @@ __global__ __launch_bounds__(kBlock) void k_pcg_q(KArgs a) {
 // partial r.r and r.z; stop on ||r||_2 < pTol or the iteration cap
 // (pressure_solver.cpp:157-182). Every workgroup sums its slot's p.q partials
 // itself in one fixed order (slotSumLean), so each gets the same alpha bit
-// for bit and no finalisation launch sits between the two kernels. On |p.q| < 1e-20 (pressure_solver.cpp:158) every
-// workgroup skips the slot and the one holding its first tile ends its CG.
+// for bit and no finalisation launch sits between the two kernels. On
+// |p.q| < 1e-20 (pressure_solver.cpp:158) every workgroup skips the slot and
+// the one holding its first tile ends its CG (pdone, piters). Another
+// workgroup of the same launch may read pdone before or after that write:
+// it skips the slot either way, from the flag or from its own p.q.
 // ---------------------------------------------------------------------------
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
