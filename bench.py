@@ -36,8 +36,10 @@ Reported beside the value (one JSON line):
     recomputes its outputs from unchanged inputs, so a burst leaves every
     solve as it was; traffic = HBM bytes per launch from the committed
     rocprofv3 byte-counter profile of this workload;
-  * roofline_dominant: the same for k_mom_assemble, the kernel with the
-    largest share of GPU time;
+  * roofline_dominant: the same for the finest multigrid level's up-sweep
+    k_mg_up, the kernel with the largest share of GPU time (four launches
+    per outer iteration); roofline_mom_assemble: the same for the momentum
+    assembly, the largest single launch;
   * plain_fp64_single: the plain FP64 single-geometry path (the drop-in:
     reference algorithm and inner budgets, Jacobi-preconditioned CG,
     polled) on every seed with the same stop test;
@@ -75,6 +77,8 @@ PEAK_HBM_GBS = 8000.0
 DP_TOL = 1e-4
 # algorithmic bytes per work unit of the timed kernels (mixed precision)
 PCG_Q_BYTES = 28    # per p' unknown: read z, p_old, aP, cE, cN; write p_new, q (4 B each)
+MG_UP_BYTES = 25    # per p' unknown, finest multigrid level: read x2, r, aP, cE, cN (4 B each) and the
+#                     coarse z (1 B per unknown: one float per 2 x 2 aggregate); write z
 MOM_ASM_BYTES = 72  # per grid node: read u, v, p (8 B each) and the Brinkman faces aU, aV (4 B each); write
 #                     dE, dN (4 B each) and per system aP, the face fluxes Fe, Fn and r0 (4 B each)
 GAMMA_MIN = 1.0 - 1e-6  # reference plane row rule (binary geometry)
@@ -174,7 +178,7 @@ def serve(bs: list, reqs: list, a, probe: bool):
     """Serve `reqs` over the batches `bs` (cp_serve_pool, one HIP stream
     each); the timing bursts run from the first batch, rotating over all."""
     for k, b in enumerate(bs):
-        b.probe((cb.K_PCG_Q, cb.K_MOM_ASM) if (probe and k == 0) else ())
+        b.probe((cb.K_PCG_Q, cb.K_MOM_ASM, cb.K_MG_UP) if (probe and k == 0) else ())
     t0 = time.perf_counter()
     res = bs[0].serve(reqs, a.check) if len(bs) == 1 else cb.serve_pool(bs, reqs, a.check)
     elapsed = time.perf_counter() - t0
@@ -629,7 +633,10 @@ def main() -> int:
     g0 = GEOMS[0][0]
     roof = roofline_of(runs, cb.K_PCG_Q, PCG_Q_BYTES, "k_pcg_q<float>", ws_pcg, traffic.get("k_pcg_q"),
                        a.batch * (g0.M - 1) * (g0.N - 2))
-    roof_dom = roofline_of(runs, cb.K_MOM_ASM, MOM_ASM_BYTES, "k_mom_assemble<float>", ws_mom,
+    roof_dom = roofline_of(runs, cb.K_MG_UP, MG_UP_BYTES, "k_mg_up<float> (finest level)",
+                           MG_UP_BYTES * slots * (g0.M - 1) * (g0.N - 2), traffic.get("k_mg_up"),
+                           a.batch * (g0.M - 1) * (g0.N - 2))
+    roof_mom = roofline_of(runs, cb.K_MOM_ASM, MOM_ASM_BYTES, "k_mom_assemble<float>", ws_mom,
                            traffic.get("k_mom_assemble"), a.batch * (g0.M + 1) * (g0.N + 1))
     drop = None if a.no_dropin else dropin_baseline(a, refs)
     thermal = thermal_leg(a) if a.thermal_solves > 0 else None
@@ -682,6 +689,7 @@ def main() -> int:
         },
         "roofline": roof,
         "roofline_dominant": roof_dom,
+        "roofline_mom_assemble": roof_mom,
         "thermal_cht": thermal,
         "cpu_baseline": cpu,
     }

@@ -28,6 +28,7 @@ SLOT_FREE, SLOT_RUNNING, SLOT_CONVERGED, SLOT_MAXITER, SLOT_FAILED = range(5)
 
 K_MOM_ASM = 0
 K_PCG_Q = 4
+K_MG_UP = 15
 
 
 class Params(C.Structure):
@@ -482,7 +483,7 @@ class Batch:
         check(load().cp_batch_set_mg_tiling(self.h, rows), self.h)
 
     def probe(self, kernels=()) -> None:
-        """Time the kernels (K_PCG_Q, K_MOM_ASM) in bursts during serve."""
+        """Time the kernels (K_PCG_Q, K_MOM_ASM, K_MG_UP) in bursts during serve."""
         mask = 0
         for k in kernels:
             mask |= 1 << k

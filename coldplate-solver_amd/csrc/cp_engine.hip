@@ -1684,15 +1684,18 @@ int cp_batch_set_graph(cp_batch* b, int use_graph) {
 // Kernels the timing bursts can time: each recomputes its outputs from
 // inputs no launch of it changes, between two outer iterations (k_pcg_q: the
 // search direction and q = A p; k_mom_assemble: the momentum coefficients,
-// r0 and the first sweep, which the next outer iteration recomputes before
-// any reader), so a burst leaves every solve as it was.
-static bool probeable(int k) { return k == int(K_PCG_Q) || k == int(K_MOM_ASM); }
+// r0 and the first sweep; the finest level's k_mg_up: the V-cycle result z;
+// each of which the next outer iteration recomputes before any reader), so a
+// burst leaves every solve as it was.
+static bool probeable(int k) { return k == int(K_PCG_Q) || k == int(K_MOM_ASM) || k == int(K_MG_UP); }
+static_assert(CP_K_MOM_ASM == K_MOM_ASM && CP_K_PCG_Q == K_PCG_Q && CP_K_MG_UP == K_MG_UP, "probe kernel ids");
 
 int cp_batch_probe(cp_batch* b, int kernel_mask) {
     if (!b) return setError(nullptr, CP_E_ARG, "null batch");
-    if (kernel_mask & ~((1 << CP_K_PCG_Q) | (1 << CP_K_MOM_ASM)))
+    if (kernel_mask & ~((1 << CP_K_PCG_Q) | (1 << CP_K_MOM_ASM) | (1 << CP_K_MG_UP)))
         return setError(&b->err, CP_E_ARG,
-                        "timing bursts are defined for CP_K_PCG_Q and CP_K_MOM_ASM only (mask 0x%x)", kernel_mask);
+                        "timing bursts are defined for CP_K_PCG_Q, CP_K_MOM_ASM and CP_K_MG_UP only (mask 0x%x)",
+                        kernel_mask);
     b->probeMask = kernel_mask;
     return CP_OK;
 }
@@ -1714,7 +1717,7 @@ int cp_batch_probe_burst(cp_batch* b, int kernel, int launches, double* ms_per_l
     if (!b || !ms_per_launch) return setError(nullptr, CP_E_ARG, "null argument");
     std::string& err = b->err;
     if (!probeable(kernel))
-        return setError(&err, CP_E_ARG, "timing bursts are defined for CP_K_PCG_Q and CP_K_MOM_ASM only");
+        return setError(&err, CP_E_ARG, "timing bursts are defined for CP_K_PCG_Q, CP_K_MOM_ASM and CP_K_MG_UP only");
     if (launches < 1 || launches > 4096) return setError(&err, CP_E_ARG, "launches must be in [1, 4096]");
     double ms = 0.0, units = 0.0;
     int n = 0;
@@ -1727,7 +1730,8 @@ int cp_batch_probe_burst(cp_batch* b, int kernel, int launches, double* ms_per_l
 namespace {
 // Work units of kernel k over a batch's running slots: p' unknowns of the CG
 // slots for k_pcg_q, nodes of the (M+1) x (N+1) grid of the implicit-momentum
-// slots for k_mom_assemble.
+// slots for k_mom_assemble, p' unknowns of the multigrid slots whose finest
+// level is a grid level (below the tail) for k_mg_up.
 double burstUnits(const cp_batch* b, Kern k) {
     double u = 0.0;
     for (int s = 0; s < b->maxSlots; ++s) {
@@ -1735,6 +1739,8 @@ double burstUnits(const cp_batch* b, Kern k) {
         const SlotDesc& d = b->hDesc[s];
         if (k == K_PCG_Q && d.presType == 1) u += double(d.M - 1) * double(d.N - 2);
         if (k == K_MOM_ASM && d.momType == 1) u += double(d.M + 1) * double(d.N + 1);
+        if (k == K_MG_UP && d.presType == 1 && d.precond == CP_PRECOND_AMG && d.mgTail > 0)
+            u += double(d.M - 1) * double(d.N - 2);
     }
     return u;
 }
@@ -1773,7 +1779,14 @@ int burst(cp_batch* const* bs, int nb, Kern k, int rounds, double* ms, double* u
             KArgs a = kargsFor(b, k);
             a.force = 1;
             const bool f32 = b->precision == CP_PREC_MIXED;
-            const int rows = pOnly(k) ? pRows(f32, b->tileRows) : b->tileRows;
+            int rows = pOnly(k) ? pRows(f32, b->tileRows) : b->tileRows;
+            if (k == K_MG_UP) {  // the finest level's tile list, as launchMg sets it
+                a.tileSlot = b->dMgTileSlot;
+                a.nTiles = b->dMgNTiles;
+                a.level = 0;
+                a.order = (b->tileOrder >> 2) & 1;
+                rows = b->mgRowsEff();
+            }
             const hipError_t e = launchWorkOnly(k, f32, rows, gridFor(b, 1 << 30), b0->stream, a);
             if (e != hipSuccess) return setError(&err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
         }

@@ -2040,7 +2040,11 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(KArgs a) {
         const MgTile tl = decodeMgTile<R>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         const SlotState& st = a.state[tl.s];
-        if (!mgActive(d, st) || a.level >= d.mgTail) continue;
+        // a.force (timing bursts of the finest level): also slots whose CG
+        // has stopped; the recomputed z is overwritten by the next V-cycle
+        // before any kernel reads it
+        const bool forced = (a.force != 0) & (st.status == ST_RUNNING) & (d.precond != 0);
+        if (!(mgActive(d, st) | forced) || a.level >= d.mgTail) continue;
         const MgView<T> L(d.lv[a.level]);
         const MgView<T> C(d.lv[a.level + 1]);
         const T om = T(d.mgOmega), sc = T(d.mgScale);

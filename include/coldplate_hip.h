@@ -344,6 +344,7 @@ int cp_batch_autotune(cp_batch* b, int steps, int* rows, int* mg_rows, int* wg_p
    of 1 << CP_K_*). */
 #define CP_K_MOM_ASM      0   /* momentum assembly with the first Jacobi sweep */
 #define CP_K_PCG_Q        4   /* PCG: search direction and q = A p */
+#define CP_K_MG_UP       15   /* multigrid, finest level: prolongation, post-smoothing, z */
 
 /* Measurement hooks (no counterpart in the reference, which has no device
    timing). cp_batch_probe(b, mask) makes cp_serve / cp_serve_pool (with b
@@ -351,14 +352,15 @@ int cp_batch_autotune(cp_batch* b, int steps, int* rows, int* mg_rows, int* wg_p
    of 16 rounds of back-to-back launches between two HIP events on b's
    stream, with the GPU to itself; each round launches the kernel once per
    batch serving the queue, over that batch's running slots, so with several
-   batches the rounds rotate over all of their arrays. Both kernels
-   recompute their outputs from inputs no launch of theirs changes, between
-   two outer iterations, so a burst leaves every solve as it was. mask 0
-   disables. cp_batch_probe_read returns the bursts of the last serve,
-   oldest first: kernel, launches, elapsed ms, and work units over all its
-   launches (p' unknowns for CP_K_PCG_Q, (M+1) x (N+1) grid nodes for
-   CP_K_MOM_ASM); any output but ms may be NULL; the return value is their
-   count. cp_batch_probe_burst runs one burst of `launches` launches on b
+   batches the rounds rotate over all of their arrays. Each kernel
+   recomputes its outputs from inputs no launch of it changes, between two
+   outer iterations, and the next outer iteration recomputes them before
+   any reader, so a burst leaves every solve as it was. mask 0 disables.
+   cp_batch_probe_read returns the bursts of the last serve, oldest first:
+   kernel, launches, elapsed ms, and work units over all its launches (p'
+   unknowns for CP_K_PCG_Q, and for CP_K_MG_UP those of the multigrid slots;
+   (M+1) x (N+1) grid nodes for CP_K_MOM_ASM); any output but ms may be
+   NULL; the return value is their count. cp_batch_probe_burst runs one burst of `launches` launches on b
    alone now, synchronises, and gives the mean ms per launch. */
 int cp_batch_probe(cp_batch* b, int kernel_mask);
 int cp_batch_probe_read(cp_batch* b, int* kernel, int* launches, double* ms, double* units, int cap);

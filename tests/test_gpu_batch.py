@@ -177,12 +177,14 @@ def test_tile_orders_are_bitwise_identical(precision):
                 np.testing.assert_array_equal(x, y)
 
 
-@pytest.mark.parametrize("kernel", [cb.K_PCG_Q, cb.K_MOM_ASM], ids=["pcg_q", "mom_assemble"])
+@pytest.mark.parametrize("kernel", [cb.K_PCG_Q, cb.K_MOM_ASM, cb.K_MG_UP], ids=["pcg_q", "mom_assemble", "mg_up"])
 def test_timing_burst_leaves_results_unchanged(kernel):
-    """Timing bursts of k_pcg_q or k_mom_assemble between replays (as
-    bench.py runs them) do not change any slot's results."""
+    """Timing bursts of k_pcg_q, k_mom_assemble or the finest level's k_mg_up
+    between replays (as bench.py runs them) do not change any slot's
+    results."""
     p, ct = wl.network(64, 128, 17, cb.CP_PREC_MIXED)
     q = wl.fast_mode(p)
+    q.pressurePrecond = cb.CP_PRECOND_AMG
     say(f"[batch] timing burst of kernel {kernel}")
     runs = []
     for burst in (False, True):
