@@ -618,11 +618,11 @@ def main() -> int:
     if stream5:
         log(f"served-stream subsample: {stream5}")
     traffic = {}
-    if TRAFFIC.exists() and tuned and (tuned["tile_rows"], tuned["mg_rows"]) == (4, 4):
+    t = json.loads(TRAFFIC.read_text()) if TRAFFIC.exists() else {}
+    if t and tuned and [tuned["tile_rows"], tuned["mg_rows"]] == t.get("tiling"):
         # HBM bytes per launch, from the committed rocprofv3 FETCH_SIZE /
         # WRITE_SIZE runs of this workload at this batch size and tiling, in
         # the tile order this run used for the kernel's family
-        t = json.loads(TRAFFIC.read_text())
         for k, v in t.get("kernels", {}).items():
             bit = (tuned.get("tile_order", 0) >> t["tile_order_bit"][k]) & 1
             e = v["by_launch_order"].get(str(bit))

@@ -1568,10 +1568,10 @@ int cp_serve_pool(cp_batch* const* batches, int nbatches, const cp_request* reqs
 
 int cp_batch_set_mg_tiling(cp_batch* b, int rows) {
     if (!b) return setError(nullptr, CP_E_ARG, "null batch");
-    const int maxRows = b->precision == CP_PREC_MIXED ? 4 : 2;
-    if (!(rows == 0 || rows == 2 || rows == 4) || rows > maxRows)
+    const int maxRows = b->precision == CP_PREC_MIXED ? 8 : 2;
+    if (!(rows == 0 || rows == 2 || rows == 4 || rows == 8) || rows > maxRows)
         return setError(&b->err, CP_E_ARG, "multigrid tile rows %d out of range (0, 2%s)", rows,
-                        maxRows == 4 ? ", 4" : "");
+                        maxRows == 8 ? ", 4, 8" : "");
     b->mgTileRows = rows;
     for (int s = 0; s < b->maxSlots; ++s)
         if (b->slots[s].used) setTiling(b->hDesc[s], b->tileRows, b->mgRowsEff());
@@ -1591,7 +1591,7 @@ int cp_batch_autotune(cp_batch* b, int steps, int* rows_out, int* mg_rows_out, i
     if (!any) return setError(&err, CP_E_STATE, "autotune needs running slots");
     const bool f32 = b->precision == CP_PREC_MIXED;
     const int rowsC[] = {1, 2, 4, 8};
-    const int mgC[] = {2, 4};
+    const int mgC[] = {2, 4, 8};
     const int wgC[] = {4, 8, 16};
     hipEvent_t ev[2] = {nullptr, nullptr};
     // one candidate: retile, capture, time `steps` replays
@@ -1626,7 +1626,7 @@ int cp_batch_autotune(cp_batch* b, int steps, int* rows_out, int* mg_rows_out, i
     for (int r : rowsC) {
         for (int m : mgC) {
             for (int w : wgC) {
-                if (rc || m > (f32 ? 4 : 2)) continue;
+                if (rc || m > (f32 ? 8 : 2)) continue;
                 double per = 0.0;
                 if ((rc = timed(r, m, w, order0, per))) break;
                 if (per < best) {

@@ -2667,11 +2667,11 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
 // ---------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------
-// Multigrid level tiles: R even (whole aggregates per wave), 4 only in
+// Multigrid level tiles: R even (whole aggregates per wave), 4 and 8 only in
 // single precision (register footprint); the host uses no other.
 template <typename T, int R>
 constexpr bool mgRowsOk() {
-    return R == 2 || (R == 4 && sizeof(T) == 4);
+    return R == 2 || ((R == 4 || R == 8) && sizeof(T) == 4);
 }
 
 template <typename T, int R>

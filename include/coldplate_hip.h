@@ -311,9 +311,9 @@ int cp_batch_set_tiling(cp_batch* b, int tile_rows, int wg_per_cu);
 int cp_batch_set_graph(cp_batch* b, int use_graph);
 
 /* Rows per wave R of the finest multigrid level's 60 x 4R tiles (no
-   counterpart in the reference; coarser levels use R = 2): 2 or 4 (4 in
-   mixed precision only); 0 derives it from the batch tile height. Every
-   slot is retiled. */
+   counterpart in the reference; coarser levels use R = 2): 2, 4 or 8 (4 and
+   8 in mixed precision only); 0 derives it from the batch tile height.
+   Every slot is retiled. */
 int cp_batch_set_mg_tiling(cp_batch* b, int rows);
 
 /* Tile order of the batch's launches (no counterpart in the reference), a
@@ -327,8 +327,8 @@ int cp_batch_set_mg_tiling(cp_batch* b, int rows);
 int cp_batch_set_tile_order(cp_batch* b, int order);
 
 /* Autotuner (no counterpart in the reference): for every candidate tiling
-   (rows per wave R in {1, 2, 4, 8}, finest-multigrid-level rows in {2, 4}
-   (2 in FP64), workgroups per CU in {4, 8, 16}) replays `steps`
+   (rows per wave R in {1, 2, 4, 8}, finest-multigrid-level rows in
+   {2, 4, 8} (2 in FP64), workgroups per CU in {4, 8, 16}) replays `steps`
    outer iterations of the admitted slots, timed with HIP events on the batch
    stream, and keeps the fastest tiling for the batch; then it times the eight
    tile orders (cp_batch_set_tile_order) with that tiling and keeps the

@@ -217,7 +217,7 @@ def test_autotune_keeps_results():
         s = b.admit(q, ct)
         if tune:
             t = b.autotune(1)
-            assert t["tile_rows"] in (1, 2, 4, 8) and t["mg_rows"] in (2, 4) and t["wg_per_cu"] in (4, 8, 16)
+            assert t["tile_rows"] in (1, 2, 4, 8) and t["mg_rows"] in (2, 4, 8) and t["wg_per_cu"] in (4, 8, 16)
             assert 0 <= t["tile_order"] <= 7
             assert t["ms_per_step"] > 0
         for _ in range(400):

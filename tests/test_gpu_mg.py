@@ -173,3 +173,29 @@ def test_mg_alone_equals_mixed_bitwise(precision):
     for x, y in zip(ref, got):
         np.testing.assert_array_equal(x, y)
     mixed.close()
+
+
+def test_mg_finest_tile_rows_converge_alike():
+    """The finest level's tiles of 2, 4 and 8 rows per wave (the autotuner's
+    choices in mixed precision) reach the same converged pressure drop: a
+    tile shape changes only the grouping of the V-cycle's r.z partial sums."""
+    p, ct = wl.network(48, 128, 6, cb.CP_PREC_MIXED)
+    q = mg(wl.fast_mode(p, 4, 4, 2))
+    q.epsilon = 1e-9
+    q.pTol = 1e-14
+    q.maxIterations = 20000
+    dps = []
+    for rows in (2, 4, 8):
+        say(f"[mg] finest tile rows {rows}")
+        b = cb.Batch(cb.CP_PREC_MIXED, 64 << 20, 1)
+        b.set_mg_tiling(rows)
+        s = b.admit(q, ct)
+        for _ in range(400):
+            b.run(50)
+            if b.poll()[s]["status"] != cb.SLOT_RUNNING:
+                break
+        assert b.poll()[s]["status"] == cb.SLOT_CONVERGED
+        dps.append(b.full_dp(s))
+        b.close()
+    for d in dps[1:]:
+        assert abs(d / dps[0] - 1) <= 1e-8, dps
