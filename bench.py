@@ -37,9 +37,9 @@ Reported beside the value (one JSON line):
     solve as it was; traffic = HBM bytes per launch from the committed
     rocprofv3 byte-counter profile of this workload;
   * roofline_dominant: the same for the finest multigrid level's up-sweep
-    k_mg_up, the kernel with the largest share of GPU time (four launches
-    per outer iteration); roofline_mom_assemble: the same for the momentum
-    assembly, the largest single launch;
+    k_mg_up, which with k_pcg_q holds the largest shares of GPU time (four
+    launches of each per outer iteration); roofline_mom_assemble: the same
+    for the momentum assembly, the largest single launch;
   * plain_fp64_single: the plain FP64 single-geometry path (the drop-in:
     reference algorithm and inner budgets, Jacobi-preconditioned CG,
     polled) on every seed with the same stop test;
