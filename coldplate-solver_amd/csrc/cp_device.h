@@ -25,9 +25,7 @@ constexpr int kBlock = kTileW * kTileH;
 // level plus a two-column halo on each side (the two smoothing sweeps and the
 // residual reach two nodes out).
 constexpr int kMgTileW = kTileW - 4;
-// p' unknowns' tiles: each lane holds two adjacent columns (paired loads).
-constexpr int kPTileW = 2 * kTileW;
-constexpr int kNumPartials = 4;  // per-tile reduction partials (doubles)
+constexpr int kNumPartials = 5;  // per-tile reduction partials (doubles)
 constexpr int kNumXCD = 8;       // accelerator complex dies of an MI355X
 constexpr int kMaxLevels = 12;   // aggregation-multigrid levels (incl. the finest)
 constexpr int kCoarseMax = 64;   // unknowns of the coarsest level (dense inverse)
@@ -75,8 +73,8 @@ struct alignas(64) SlotDesc {
     int M, N, pitch;
     int tile0, ntiles, tilesX;
     // the p' unknowns' own tile list (kernels that touch only p' unknowns):
-    // 128 x 4Rp tiles from node (1, 0), covering rows 1 .. M-1, columns
-    // 0 .. N-2, with Rp = pRows(mixed, tileRows)
+    // 64 Cp x 4Rp tiles from node (1, 0), covering rows 1 .. M-1, columns
+    // 0 .. N-2, with Rp = pRows(mixed, tileRows) and Cp = pCols(mixed)
     int ptile0, pntiles, ptilesX;
     int tileRows;                // rows per wave R: a tile is 64 x 4R nodes
     int nU, nV;                  // momentum DOF counts (Jacobi stop test)
@@ -156,7 +154,37 @@ struct KArgs {
     int level;  // multigrid kernels: hierarchy level of the tile list
     int color;  // SOR sweep: 0 = cells with i + j even, 1 = odd
     int order;  // tile order of the launch: 0 = one contiguous range per XCD, 1 = launch order
+    int fold;   // folded steps of the fixed-budget mixed-precision step (kFold* bits)
+    int pvar;   // k_pcg_q only: the slots the launch runs and their form (kPcgQ*)
 };
+
+// KArgs::pvar of a single-precision k_pcg_q launch. kPcgQAll: every slot,
+// with its preconditioner's z and its first or later iteration chosen per
+// slot. Otherwise one form only, which holds fewer registers (the four
+// forms together take 162 VGPRs per lane at 4 x 4 values per lane, the
+// multigrid form of a later iteration 122): the slots whose preconditioner
+// is multigrid (kPcgQMg) or Jacobi, at the CG's first iteration
+// (kPcgQFirst) or a later one. The host knows both: a batch records the
+// preconditioners of its slots and launches one form per kind, and every
+// running slot's CG is at the iteration the launch sequence is at.
+constexpr int kPcgQAll = 0;
+constexpr int kPcgQMg = 1;
+constexpr int kPcgQJac = 2;
+constexpr int kPcgQFirst = 4;
+
+// KArgs::fold bits, set for the fixed-budget step of a mixed-precision batch.
+// kFoldMom: u* and v* of an implicit-momentum slot are not stored; the
+// pressure assembly and the correction form them from u, v and the last
+// Jacobi iterate's delta by the finish's rules (k_mom_finish is not
+// launched), and the assembly's finalisation takes over the finish's
+// residuals. kFoldPc: every slot makes one refinement cycle, so p' = 0 + x
+// exactly; the correction of a CG slot reads the single-precision iterate x
+// (k_refine_accum is not launched).
+constexpr int kFoldMom = 1;
+constexpr int kFoldPc = 2;
+// Partial entries of the folded pressure assembly: the finish's max |u* - u|,
+// max |v* - v| and non-finite flag follow the assembly's own two.
+constexpr int kFoldFinPartial = 2;
 
 // Finalisation phases (k_finalize), one per reducing work kernel.
 enum Phase {
@@ -224,6 +252,14 @@ inline int pRows(bool f32, int rows) {
     (void)rows;
     return f32 ? 4 : 1;
 }
+// Adjacent columns per lane of the p' list's tiles (one vector access per
+// array and row): 4 in single precision, 16-byte accesses, so a wave's row
+// is 1 KB of contiguous memory (measured on 8 x 1024x512 slots rotating
+// over 3 sets: the PCG stencil 26.2 against 28.8 us with 2 columns per
+// lane, 56 against 51 % of 8 TB/s); 2 in double precision. A tile is 64 Cp
+// columns wide.
+constexpr int pColsOf(bool f32) { return f32 ? 4 : 2; }
+inline int pTileW(bool f32) { return kTileW * pColsOf(f32); }
 hipError_t launchPlane(int nreq, hipStream_t s, const SlotDesc* desc, const PlaneReq* req, PlaneOut* out);
 
 // Shared by the library's translation units (defined in cp_engine.hip):

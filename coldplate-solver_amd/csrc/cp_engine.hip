@@ -238,6 +238,8 @@ struct cp_batch {
     // backend families some slot admitted so far uses (sticky, like the
     // multigrid shape: the captured step graph keeps its launches)
     int anyExplicit = 0, anyImplicit = 0, anySOR = 0, anyCG = 0;
+    // CG slots by preconditioner (k_pcg_q launches one form per kind)
+    int anyMgCG = 0, anyJacCG = 0;
     double* dPartials = nullptr;
     PlaneReq* dPlaneReq = nullptr;
     PlaneOut* dPlaneOut = nullptr;
@@ -276,6 +278,9 @@ struct cp_batch {
     }
     // replay the captured step graph (1) or launch its kernels directly (0)
     int useGraph = 1;
+    // KArgs::fold of the launches being enqueued (set by the fixed-budget
+    // step only)
+    int fold = 0;
     // host mirrors of a batch laid out in the static device scratch (no
     // pinned allocations): hState / hPlaneReq / hPlaneOut point here
     SlotState staticState;
@@ -319,6 +324,8 @@ KArgs kargs(cp_batch* b) {
     a.level = 0;
     a.color = 0;
     a.order = b->tileOrder & 1;
+    a.fold = b->fold;
+    a.pvar = kPcgQAll;
     return a;
 }
 
@@ -340,6 +347,28 @@ int launch(cp_batch* b, Kern k, int grid) {
     const int rows = pOnly(k) ? pRows(f32, b->tileRows) : b->tileRows;
     const hipError_t e = launchKernel(k, f32, rows, grid, b->maxSlots, b->stream, kargsFor(b, k));
     if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+    return CP_OK;
+}
+
+// The k_pcg_q launches of one CG iteration: in single precision one launch
+// of one form (KArgs::pvar) per preconditioner kind the batch holds, at the
+// first iteration of a CG (every running slot's pk is 0) or a later one;
+// in double precision one launch for every slot. With `force` (timing
+// bursts) the later-iteration forms, on the given stream.
+int launchPcgQ(cp_batch* b, int grid, bool first, bool force = false, hipStream_t stream = nullptr) {
+    const bool f32 = b->precision == CP_PREC_MIXED;
+    hipStream_t s = stream ? stream : b->stream;
+    KArgs a = kargsFor(b, K_PCG_Q);
+    a.force = force ? 1 : 0;
+    const int rows = pRows(f32, b->tileRows);
+    const int vars[2] = {b->anyMgCG ? kPcgQMg : -1, b->anyJacCG ? kPcgQJac : -1};
+    for (int v : vars) {
+        if (f32 && v < 0) continue;
+        a.pvar = f32 ? (v | (first ? kPcgQFirst : 0)) : kPcgQAll;
+        const hipError_t e = launchWorkOnly(K_PCG_Q, f32, rows, grid, s, a);
+        if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+        if (!f32) break;
+    }
     return CP_OK;
 }
 
@@ -445,6 +474,7 @@ int rebuildTiles(cp_batch* b) {
         const SlotDesc& d = b->hDesc[s];
         (d.momType == 0 ? b->anyExplicit : b->anyImplicit) = 1;
         (d.presType == 0 ? b->anySOR : b->anyCG) = 1;
+        if (d.presType != 0) (d.precond ? b->anyMgCG : b->anyJacCG) = 1;
     }
     int counts[kMaxLevels];
     std::vector<std::vector<int>> lists(kMaxLevels);  // alive until the stream is synchronised
@@ -573,7 +603,7 @@ void setTiling(SlotDesc& d, int tileRows, int mgRows0) {
     d.ntiles = d.tilesX * tilesY;
     // p' unknowns: rows 1 .. M-1 from row 1, columns 1 .. N-2 from column 0
     const int pRowsPerTile = kTileH * pRows(d.mixed != 0, d.tileRows);
-    d.ptilesX = (d.N - 1 + kPTileW - 1) / kPTileW;
+    d.ptilesX = (d.N - 1 + pTileW(d.mixed != 0) - 1) / pTileW(d.mixed != 0);
     d.pntiles = d.ptilesX * ((d.M - 1 + pRowsPerTile - 1) / pRowsPerTile);
     for (int l = 0; l < d.nLevels; ++l) {
         MgLevel& L = d.lv[l];
@@ -994,7 +1024,7 @@ int polledPressure(cp_batch* b) {
         while (launched < cap) {
             const int n = std::min(chunk, cap - launched);
             for (int k = 0; k < n; ++k) {
-                if ((rc = launch(b, K_PCG_Q, grid))) return rc;
+                if ((rc = launchPcgQ(b, grid, launched + k == 0))) return rc;
                 if ((rc = launch(b, K_PCG_U, grid))) return rc;
                 if ((rc = mgVcycle(b))) return rc;
             }
@@ -1042,7 +1072,27 @@ int polledStep(cp_batch* b, bool outer) {
 }
 
 // ---- graph mode (fixed inner budgets) -------------------------------------
+// KArgs::fold of the fixed-budget step: a mixed-precision batch folds the
+// momentum finish into the pressure assembly and the correction, and with
+// one refinement cycle (p' = 0 + x) the accumulation into the correction.
+// u*, v* and p' then hold no step's values; only the batch API runs this
+// step, and it reads u, v and p alone (the context API, which reads every
+// field, runs polled steps).
+int stepFold(const cp_batch* b, int R) {
+    if (b->precision != CP_PREC_MIXED) return 0;
+    return (b->anyImplicit ? kFoldMom : 0) | (R == 1 ? kFoldPc : 0);
+}
+
+int enqueueFixedStepImpl(cp_batch* b, int J, int P, int R, int grid);
+
 int enqueueFixedStep(cp_batch* b, int J, int P, int R, int grid) {
+    b->fold = stepFold(b, R);
+    const int rc = enqueueFixedStepImpl(b, J, P, R, grid);
+    b->fold = 0;
+    return rc;
+}
+
+int enqueueFixedStepImpl(cp_batch* b, int J, int P, int R, int grid) {
     int rc;
     if (b->anyExplicit) {
         if ((rc = launch(b, K_MOM_EXPL, grid)) || (rc = launch(b, K_MOM_EXPL_BC, grid))) return rc;
@@ -1062,7 +1112,8 @@ int enqueueFixedStep(cp_batch* b, int J, int P, int R, int grid) {
             }
         }
         if (J == 0 && (rc = launch(b, K_MOM_ASM, grid))) return rc;
-        if ((rc = launch(b, K_MOM_FIN, grid))) return rc;
+        // folded: the pressure assembly forms u*, v* and finalises the finish
+        if (!(b->fold & kFoldMom) && (rc = launch(b, K_MOM_FIN, grid))) return rc;
     }
     if ((rc = launch(b, K_PRES_ASM, grid))) return rc;
     if (b->anySOR)
@@ -1076,7 +1127,7 @@ int enqueueFixedStep(cp_batch* b, int J, int P, int R, int grid) {
     for (int c = 0; c < R; ++c) {
         bool updateUnfinalised = false;
         for (int k = 0; k < P; ++k) {
-            if ((rc = launch(b, K_PCG_Q, grid))) return rc;
+            if ((rc = launchPcgQ(b, grid, k == 0))) return rc;
             // the V-cycle after the last iteration would only feed a search
             // direction that is never used; before it, the update's stop
             // test is finalised together with the V-cycle's r.z
@@ -1098,7 +1149,13 @@ int enqueueFixedStep(cp_batch* b, int J, int P, int R, int grid) {
             }
         }
         if (b->precision != CP_PREC_MIXED) break;
-        if (!updateUnfinalised) {
+        if (b->fold & kFoldPc) {
+            // p' = 0 + x is read by the correction from x itself: the
+            // accumulation's bookkeeping only
+            const hipError_t e =
+                launchFinalize(updateUnfinalised ? PH_PCGU_REFACC : PH_REF_ACC, b->maxSlots, b->stream, kargs(b));
+            if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+        } else if (!updateUnfinalised) {
             if ((rc = launch(b, K_REF_ACC, grid))) return rc;
         } else {
             KArgs ka = kargsFor(b, K_REF_ACC);
@@ -1129,8 +1186,9 @@ int graphRun(cp_batch* b, int steps) {
         }
         return CP_OK;
     }
-    const int families = b->anyExplicit | b->anyImplicit << 1 | b->anySOR << 2 | b->anyCG << 3;
-    const int shapeL = ((b->mgLevels * (kMaxLevels + 1) + b->mgTail) * 2 + b->dpAny) * 16 + families;
+    const int families =
+        b->anyExplicit | b->anyImplicit << 1 | b->anySOR << 2 | b->anyCG << 3 | b->anyMgCG << 4 | b->anyJacCG << 5;
+    const int shapeL = ((b->mgLevels * (kMaxLevels + 1) + b->mgTail) * 2 + b->dpAny) * 64 + families;
     if (!b->graphExec || J != b->graphJ || P != b->graphP || R != b->graphR || shapeL != b->graphL) {
         if (b->graphExec) {
             // a replay of the old graph may still be in flight (a tiling
@@ -1786,6 +1844,14 @@ int burst(cp_batch* const* bs, int nb, Kern k, int rounds, double* ms, double* u
                 a.level = 0;
                 a.order = (b->tileOrder >> 2) & 1;
                 rows = b->mgRowsEff();
+            }
+            if (k == K_PCG_Q) {
+                const int rc = launchPcgQ(b, gridFor(b, 1 << 30), false, true, b0->stream);
+                if (rc) {
+                    if (b != b0) err = b->err;
+                    return rc;
+                }
+                continue;
             }
             const hipError_t e = launchWorkOnly(k, f32, rows, gridFor(b, 1 << 30), b0->stream, a);
             if (e != hipSuccess) return setError(&err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));

@@ -45,25 +45,51 @@ template <typename T>
 __device__ __forceinline__ void st32(gp<T> base, uint32_t k, T v) {
     *reinterpret_cast<gp<T>>(reinterpret_cast<gp<char>>(base) + k * uint32_t(sizeof(T))) = v;
 }
-// Two adjacent elements k, k+1 (k even: the pair is aligned) in one access.
-template <typename T>
-using Pair = T __attribute__((ext_vector_type(2)));
-template <typename T>
-__device__ __forceinline__ Pair<T> ld2(gp<const T> base, uint32_t k) {
-    return *reinterpret_cast<gp<const Pair<T>>>(reinterpret_cast<gp<const char>>(base) + k * uint32_t(sizeof(T)));
+// C adjacent elements k .. k+C-1 (k a multiple of C: the vector is aligned)
+// in one access, and the store of the vector's members whose mask is set
+// (all at once when all are).
+template <typename T, int C>
+using Vec = T __attribute__((ext_vector_type(C)));
+template <int C, typename T>
+__device__ __forceinline__ Vec<T, C> ldv(gp<const T> base, uint32_t k) {
+    return *reinterpret_cast<gp<const Vec<T, C>>>(reinterpret_cast<gp<const char>>(base) + k * uint32_t(sizeof(T)));
 }
-template <typename T>
-__device__ __forceinline__ void st2(gp<T> base, uint32_t k, Pair<T> v) {
-    *reinterpret_cast<gp<Pair<T>>>(reinterpret_cast<gp<char>>(base) + k * uint32_t(sizeof(T))) = v;
+template <int C, typename T>
+__device__ __forceinline__ void stv(gp<T> base, uint32_t k, Vec<T, C> v) {
+    *reinterpret_cast<gp<Vec<T, C>>>(reinterpret_cast<gp<char>>(base) + k * uint32_t(sizeof(T))) = v;
 }
-// Stores the pair's members whose mask is set (both at once when both are).
-template <typename T>
-__device__ __forceinline__ void stMasked(gp<T> base, uint32_t k, Pair<T> v, bool m0, bool m1) {
-    if (m0 && m1) {
-        st2(base, k, v);
+template <int C, typename T>
+__device__ __forceinline__ void stMaskedV(gp<T> base, uint32_t k, Vec<T, C> v, const bool (&m)[C]) {
+    bool all = true;
+#pragma unroll
+    for (int c = 0; c < C; ++c) all &= m[c];
+    if (all) {
+        stv<C>(base, k, v);
     } else {
-        if (m0) st32(base, k, v.x);
-        if (m1) st32(base, k + 1, v.y);
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+            if (m[c]) st32(base, k + c, v[c]);
+    }
+}
+// The same for two arrays at the same index under one branch: two separate
+// masked stores keep both vectors live across both branches (28 more VGPRs
+// in the PCG stencil at 4 x 4 values per lane).
+template <int C, typename T>
+__device__ __forceinline__ void stMaskedV2(gp<T> a, Vec<T, C> va, gp<T> b, Vec<T, C> vb, uint32_t k,
+                                           const bool (&m)[C]) {
+    bool all = true;
+#pragma unroll
+    for (int c = 0; c < C; ++c) all &= m[c];
+    if (all) {
+        stv<C>(a, k, va);
+        stv<C>(b, k, vb);
+    } else {
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+            if (m[c]) {
+                st32(a, k + c, va[c]);
+                st32(b, k + c, vb[c]);
+            }
     }
 }
 
@@ -95,7 +121,8 @@ __device__ __forceinline__ double waveReduce(double v) {
 // only after its reads.
 template <int NQ, unsigned MaxMask>
 __device__ __forceinline__ void blockReduceTo(double (&v)[NQ], double* dst, int& calls) {
-    __shared__ double sh[2][kBlock / 64][4];
+    static_assert(NQ <= kNumPartials, "entries of one reduction");
+    __shared__ double sh[2][kBlock / 64][kNumPartials];
     const int b = calls & 1;
     ++calls;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -147,7 +174,7 @@ __device__ __forceinline__ double slotSumLean(const double* partials, int tile0,
 // groups of 8 before combining them, so the latency is paid once per group.
 template <int NQ, unsigned MaxMask>
 __device__ __forceinline__ void slotReduce(const double* partials, int tile0, int ntiles, double (&out)[NQ]) {
-    __shared__ double res[4];
+    __shared__ double res[kNumPartials];
     double v[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) v[q] = 0.0;
@@ -195,7 +222,7 @@ struct Tile {
 // Tile t of the p' list (a slot's ptile0 .. ptile0 + pntiles - 1): rows from
 // 1, so the tiles hold no row outside the unknowns' range 1 .. M-1; 128
 // columns, lane l holding columns j0 + 2l and j0 + 2l + 1 (tl.j is the first).
-template <int R>
+template <int R, int C>
 __device__ __forceinline__ Tile decodePTile(const KArgs& a, int t) {
     Tile tl;
     tl.s = __builtin_amdgcn_readfirstlane(a.tileSlot[t]);
@@ -204,9 +231,9 @@ __device__ __forceinline__ Tile decodePTile(const KArgs& a, int t) {
     const int ty = lt / d.ptilesX;
     const int tx = lt - ty * d.ptilesX;
     tl.i0 = 1 + ty * (kTileH * R);
-    tl.j0 = tx * kPTileW;
+    tl.j0 = tx * (kTileW * C);
     tl.i = tl.i0 + int(threadIdx.x >> 6) * R;
-    tl.j = tl.j0 + 2 * int(threadIdx.x & 63);
+    tl.j = tl.j0 + C * int(threadIdx.x & 63);
     return tl;
 }
 
@@ -275,9 +302,11 @@ __device__ __forceinline__ size_t clampIdx(int i, int j, int M, int P) {
 __device__ __forceinline__ uint32_t clampIdx32(int i, int j, int M, int P) {
     return uint32_t(min(max(i, 0), M)) * uint32_t(P) + uint32_t(min(max(j, 0), P - 1));
 }
-// the same for an even column j of a pair (P is even: the pair stays aligned)
-__device__ __forceinline__ uint32_t clampPair32(int i, int j, int M, int P) {
-    return uint32_t(min(max(i, 0), M)) * uint32_t(P) + uint32_t(min(max(j, 0), P - 2));
+// the same for the first column j of a C-column vector (j and P multiples of
+// C: the vector stays aligned)
+template <int C>
+__device__ __forceinline__ uint32_t clampVec32(int i, int j, int M, int P) {
+    return uint32_t(min(max(i, 0), M)) * uint32_t(P) + uint32_t(min(max(j, 0), P - C));
 }
 
 // lane l receives lane l-1's value (west) / lane l+1's value (east); lanes 0
@@ -795,6 +824,25 @@ __device__ __forceinline__ T delta1(T r0, T aP, int i, int j, int iMax, int jMax
     return (i >= 1 && i <= iMax && j >= 1 && j <= jMax) ? T(0) + r0 / aP : T(0);
 }
 
+// Where the last Jacobi iterate's delta of one momentum system (c = 0: U,
+// 1: V) of a single-precision slot is read after the fixed-budget sweeps, as
+// k_mom_finish reads it: after one sweep (the assembly's) it is the implicit
+// r0 / aP, else the delta buffer of the sweep count's parity. A system still
+// marked running made every sweep of its budget. Load x and y at a node;
+// delta = one ? delta1(x, y, ...) : x. With a stored delta both bases are the
+// delta buffer, so the second load hits the line of the first.
+struct DeltaSrc {
+    gp<const float> x, y;
+    bool one;
+};
+__device__ __forceinline__ DeltaSrc deltaSrc(const SlotDesc& d, const SlotState& st, int c) {
+    void* const* arr = c == 0 ? d.mu : d.mv;
+    const int it = st.jdone[c] ? st.jiters[c] : d.maxMomIter;
+    const bool one = it == 1;
+    const auto dl = G(static_cast<const float*>(arr[M_D0 + (it & 1)]));
+    return {one ? G(static_cast<const float*>(arr[M_R0])) : dl, one ? G(static_cast<const float*>(arr[M_AP])) : dl, one};
+}
+
 // The arrays of one momentum system a sweep reads.
 template <typename T>
 struct JacSys {
@@ -1067,36 +1115,106 @@ __global__ __launch_bounds__(kBlock) void k_pres_assemble(KArgs a) {
         const bool zeroPc = (d.presType == 0) | (d.mixed == 0);
         // all loads first (clamped addresses)
         double uE[R], uW[R], vN[R], vS[R], dEe[R], dEw[R], dNn[R], dNs[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int i = tl.i + r;
-            const size_t kc = clampIdx(i, j, M, P), kw = clampIdx(i, j - 1, M, P), ks = clampIdx(i - 1, j, M, P);
-            uE[r] = us[kc];
-            uW[r] = us[kw];
-            vN[r] = vs[kc];
-            vS[r] = vs[ks];
-        }
-        if (d.dE32) {
-            const auto fE = G(reinterpret_cast<const float*>(d.dE)), fN = G(reinterpret_cast<const float*>(d.dN));
+        // the finish's max |u* - u|, max |v* - v| and non-finite flag when
+        // this launch forms u*, v* itself
+        double fin[3] = {0.0, 0.0, 0.0};
+        bool folded = false;
+        if constexpr (sizeof(T) == 4) folded = ((a.fold & kFoldMom) != 0) & (d.momType == 1);
+        if (folded) {
+            // u* at the cell's east and west faces and v* at its north and
+            // south faces by k_mom_finish's rules: clamp(u + delta) inside,
+            // the inlet profile at j = 0, the outlet column's copy of its west
+            // neighbour, 0 on the walls
+            const double Uin = st.inletVelocity;
+            const double maxVel = 3.0 * fmax(fabs(Uin), 0.1);
+            const DeltaSrc sU = deltaSrc(d, st, 0), sV = deltaSrc(d, st, 1);
+            const auto u = G(d.u), vv = G(d.v), gam = G(d.gam);
+            // u at the row's node and its west neighbour; v from the row
+            // below the wave's first row up (row h is row tl.i - 1 + h: the
+            // south face of row r is the north face of row r - 1)
+            double uk[R], um[R], vh[R + 1], ge[R];
+            float xuk[R], yuk[R], xum[R], yum[R], xvh[R + 1], yvh[R + 1];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int i = tl.i + r;
-                const size_t kc = clampIdx(i, j, M, P), kw = clampIdx(i, j - 1, M, P), ks = clampIdx(i - 1, j, M, P);
-                dEe[r] = fE[kc];
-                dEw[r] = fE[kw];
-                dNn[r] = fN[kc];
-                dNs[r] = fN[ks];
+                const size_t kc = clampIdx(i, j, M, P), kw = clampIdx(i, j - 1, M, P);
+                uk[r] = u[kc];
+                um[r] = u[kw];
+                ge[r] = gam[clampIdx(i, 1, M, P)];  // read at j = 0 only: one address per row
+                xuk[r] = sU.x[kc];
+                yuk[r] = sU.y[kc];
+                xum[r] = sU.x[kw];
+                yum[r] = sU.y[kw];
+            }
+#pragma unroll
+            for (int h = 0; h <= R; ++h) {
+                const size_t k = clampIdx(tl.i - 1 + h, j, M, P);
+                vh[h] = vv[k];
+                xvh[h] = sV.x[k];
+                yvh[h] = sV.y[k];
+            }
+            double vsh[R + 1];  // v* by k_mom_finish's interior rule, rows tl.i - 1 ..
+#pragma unroll
+            for (int h = 0; h <= R; ++h) {
+                const float dv = sV.one ? delta1(xvh[h], yvh[h], tl.i - 1 + h, j, M - 2, N - 1) : xvh[h];
+                vsh[h] = clampv(vh[h] + double(dv), maxVel);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = tl.i + r;
+                const float duk = sU.one ? delta1(xuk[r], yuk[r], i, j, M - 1, N - 2) : xuk[r];
+                const float dum = sU.one ? delta1(xum[r], yum[r], i, j - 1, M - 1, N - 2) : xum[r];
+                const double west = clampv(um[r] + double(dum), maxVel);  // u*(i, j-1), and u*(i, N-1)
+                const double here = clampv(uk[r] + double(duk), maxVel);
+                uE[r] = (j == N - 1) ? west : here;
+                uW[r] = (j == 1) ? ((ge[r] > 0.5) ? Uin : 0.0) : west;
+                vN[r] = (i == M - 1) ? 0.0 : vsh[r + 1];
+                vS[r] = (i == 1) ? 0.0 : vsh[r];
+                if (!(i >= 1 && i <= M - 1 && j >= 1 && j <= N - 1)) continue;
+                // each u* and v* the finish wrote is one of these faces, or 0,
+                // or a copy of one of them
+                if (j <= N - 2) fin[0] = nanmax(fin[0], fabs(uE[r] - uk[r]));
+                if (i <= M - 2) fin[1] = nanmax(fin[1], fabs(vN[r] - vh[r + 1]));
+                if (!(uE[r] == uE[r]) || !(uW[r] == uW[r]) || !(vN[r] == vN[r]) || !(vS[r] == vS[r])) fin[2] = 1.0;
             }
         } else {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int i = tl.i + r;
                 const size_t kc = clampIdx(i, j, M, P), kw = clampIdx(i, j - 1, M, P), ks = clampIdx(i - 1, j, M, P);
-                dEe[r] = dE[kc];
-                dEw[r] = dE[kw];
-                dNn[r] = dN[kc];
-                dNs[r] = dN[ks];
+                uE[r] = us[kc];
+                uW[r] = us[kw];
+                vN[r] = vs[kc];
+                vS[r] = vs[ks];
             }
+        }
+        // dE at the row and its west neighbour; dN from the row below the
+        // wave's first row up (the south face of row r is row r - 1's north)
+        double dNh[R + 1];
+        if (d.dE32) {
+            const auto fE = G(reinterpret_cast<const float*>(d.dE)), fN = G(reinterpret_cast<const float*>(d.dN));
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = tl.i + r;
+                dEe[r] = fE[clampIdx(i, j, M, P)];
+                dEw[r] = fE[clampIdx(i, j - 1, M, P)];
+            }
+#pragma unroll
+            for (int h = 0; h <= R; ++h) dNh[h] = fN[clampIdx(tl.i - 1 + h, j, M, P)];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = tl.i + r;
+                dEe[r] = dE[clampIdx(i, j, M, P)];
+                dEw[r] = dE[clampIdx(i, j - 1, M, P)];
+            }
+#pragma unroll
+            for (int h = 0; h <= R; ++h) dNh[h] = dN[clampIdx(tl.i - 1 + h, j, M, P)];
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            dNn[r] = dNh[r + 1];
+            dNs[r] = dNh[r];
         }
         double v[2] = {0.0, 0.0};
 #pragma unroll
@@ -1127,7 +1245,14 @@ __global__ __launch_bounds__(kBlock) void k_pres_assemble(KArgs a) {
                 v[1] += double(rr) * double(z);
             }
         }
-        blockReduceTo<2, 1u>(v, a.partials + size_t(t) * kNumPartials, nred);
+        if (a.fold & kFoldMom) {
+            // every tile of a folded launch writes five entries (zeros for a
+            // slot whose finish ran), so the finalisation reads one shape
+            double v5[5] = {v[0], v[1], fin[0], fin[1], fin[2]};
+            blockReduceTo<5, 29u>(v5, a.partials + size_t(t) * kNumPartials, nred);
+        } else {
+            blockReduceTo<2, 1u>(v, a.partials + size_t(t) * kNumPartials, nred);
+        }
     }
 }
 
@@ -1159,12 +1284,12 @@ __device__ __forceinline__ T pLoad(gp<const T> r, gp<const T> aP, gp<const T> Z,
     return First ? z : z + beta * ld32(pold, k);
 }
 
-// q = A p_k for one wave's R rows x 128 columns, two adjacent columns per
-// lane. The wave loads p_k for its rows and for the rows above and below
-// them (lanes 0-31 also at column j0-1, lanes 32-63 at column j0+128), then
-// marches down the rows taking the neighbours across lanes from the
-// adjacent lanes.
-template <typename T, int R, bool HasZ, bool First>
+// q = A p_k for one wave's R rows x 64 C columns, C adjacent columns per
+// lane (one vector access per array and row). The wave loads p_k for its
+// rows and for the rows above and below them (lanes 0-31 also at column
+// j0-1, lanes 32-63 at column j0+64C), then marches down the rows taking
+// the neighbours across lanes from the adjacent lanes.
+template <typename T, int R, int C, bool HasZ, bool First>
 __device__ __forceinline__ void pcgQWave(const SlotDesc& d, const Tile& tl, int kk, T beta, double& v) {
     const int M = d.M, N = d.N, P = d.pitch;
     const auto r = G(static_cast<const T*>(d.pr[P_R]));
@@ -1177,67 +1302,74 @@ __device__ __forceinline__ void pcgQWave(const SlotDesc& d, const Tile& tl, int 
     const auto Q = G(static_cast<T*>(d.pr[P_Q]));
     const int lane = int(threadIdx.x & 63);
     const int j = tl.j, ib = tl.i;
-    const int je = lane < 32 ? tl.j0 - 1 : tl.j0 + kPTileW;
-    Pair<T> p[R + 2], ap[R], ce[R], cn[R + 1];
+    const int je = lane < 32 ? tl.j0 - 1 : tl.j0 + kTileW * C;
+    Vec<T, C> p[R + 2], ap[R], ce[R], cn[R + 1];
     T pe[R + 2], cee[R];
 #pragma unroll
     for (int h = 0; h < R + 2; ++h) {
         const int i = ib - 1 + h;
-        const uint32_t k2 = clampPair32(i, j, M, P);
-        Pair<T> z;
+        const uint32_t kv = clampVec32<C>(i, j, M, P);
+        Vec<T, C> z;
         if (HasZ) {
-            z = ld2(Z, k2);
+            z = ldv<C>(Z, kv);
         } else {
-            const Pair<T> rv = ld2(r, k2), av = ld2(aP, k2);
-            z.x = rv.x / av.x;
-            z.y = rv.y / av.y;
+            const Vec<T, C> rv = ldv<C>(r, kv), av = ldv<C>(aP, kv);
+#pragma unroll
+            for (int c = 0; c < C; ++c) z[c] = rv[c] / av[c];
         }
         if (!First) {
-            const Pair<T> po = ld2(pold, k2);
-            z.x = z.x + beta * po.x;
-            z.y = z.y + beta * po.y;
+            const Vec<T, C> po = ldv<C>(pold, kv);
+#pragma unroll
+            for (int c = 0; c < C; ++c) z[c] = z[c] + beta * po[c];
         }
         const T px = pLoad<T, HasZ, First>(r, aP, Z, pold, clampIdx32(i, je, M, P), beta);
-        p[h].x = isPDof(i, j, M, N) ? z.x : T(0);
-        p[h].y = isPDof(i, j + 1, M, N) ? z.y : T(0);
+#pragma unroll
+        for (int c = 0; c < C; ++c) p[h][c] = isPDof(i, j + c, M, N) ? z[c] : T(0);
         pe[h] = isPDof(i, je, M, N) ? px : T(0);
     }
 #pragma unroll
     for (int rr = 0; rr < R; ++rr) {
-        const uint32_t k2 = clampPair32(ib + rr, j, M, P);
-        ap[rr] = ld2(aP, k2);
-        ce[rr] = ld2(cE, k2);
-        cee[rr] = ld32(cE, clampIdx32(ib + rr, je, M, P));
+        const uint32_t kv = clampVec32<C>(ib + rr, j, M, P);
+        ap[rr] = ldv<C>(aP, kv);
+        ce[rr] = ldv<C>(cE, kv);
+        cee[rr] = ld32(cE, clampIdx32(ib + rr, tl.j0 - 1, M, P));  // lane 0's west face: one address
     }
 #pragma unroll
-    for (int h = 0; h < R + 1; ++h) cn[h] = ld2(cN, clampPair32(ib - 1 + h, j, M, P));
+    for (int h = 0; h < R + 1; ++h) cn[h] = ldv<C>(cN, clampVec32<C>(ib - 1 + h, j, M, P));
 #pragma unroll
     for (int rr = 0; rr < R; ++rr) {
         const int i = ib + rr;
-        const Pair<T> pC = p[rr + 1];
-        T pW = fromWest(pC.y), pE = fromEast(pC.x), cEw = fromWest(ce[rr].y);
+        const Vec<T, C> pC = p[rr + 1];
+        T pW = fromWest(pC[C - 1]), pE = fromEast(pC[0]), cEw = fromWest(ce[rr][C - 1]);
         pW = lane == 0 ? pe[rr + 1] : pW;
         cEw = lane == 0 ? cee[rr] : cEw;
         pE = lane == 63 ? pe[rr + 1] : pE;
         // CSR column order S, W, P, E, N (pressure_solver.cpp:108-118)
-        Pair<T> q;
-        q.x = -cn[rr].x * p[rr].x - cEw * pW + ap[rr].x * pC.x - ce[rr].x * pC.y - cn[rr + 1].x * p[rr + 2].x;
-        q.y = -cn[rr].y * p[rr].y - ce[rr].x * pC.x + ap[rr].y * pC.y - ce[rr].y * pE - cn[rr + 1].y * p[rr + 2].y;
-        const bool d0 = isPDof(i, j, M, N), d1 = isPDof(i, j + 1, M, N);
+        Vec<T, C> q;
+        bool dm[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const T west = c == 0 ? pW : pC[c > 0 ? c - 1 : 0];
+            const T cw = c == 0 ? cEw : ce[rr][c > 0 ? c - 1 : 0];
+            const T east = c == C - 1 ? pE : pC[c < C - 1 ? c + 1 : 0];
+            q[c] = -cn[rr][c] * p[rr][c] - cw * west + ap[rr][c] * pC[c] - ce[rr][c] * east - cn[rr + 1][c] * p[rr + 2][c];
+            dm[c] = isPDof(i, j + c, M, N);
+        }
         const uint32_t k = uint32_t(max(i, 0)) * uint32_t(P) + uint32_t(j);
-        stMasked(pnew, k, pC, d0, d1);
-        stMasked(Q, k, q, d0, d1);
-        if (d0) v += double(pC.x) * double(q.x);
-        if (d1) v += double(pC.y) * double(q.y);
+        stMaskedV2<C>(pnew, pC, Q, q, k, dm);
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+            if (dm[c]) v += double(pC[c]) * double(q[c]);
     }
 }
 
-template <typename T, int R>
+template <typename T, int R, int V>
 __global__ __launch_bounds__(kBlock) void k_pcg_q(KArgs a) {
     const int nT = *a.nTiles;
     int nred = 0;  // block reductions made so far (alternating buffers)
+    constexpr int C = pColsOf(sizeof(T) == 4);
     XCD_TILE_LOOP(t, nT) {
-        const Tile tl = decodePTile<R>(a, t);
+        const Tile tl = decodePTile<R, C>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
         // read the gate fields together and test them without short-circuit
@@ -1247,12 +1379,23 @@ __global__ __launch_bounds__(kBlock) void k_pcg_q(KArgs a) {
         const T beta = T(st.beta);
         double v[1] = {0.0};
         // the preconditioner and the first iteration are uniform per slot
-        if (d.precond) {
-            if (kk == 0) pcgQWave<T, R, true, true>(d, tl, kk, beta, v[0]);
-            else pcgQWave<T, R, true, false>(d, tl, kk, beta, v[0]);
+        if constexpr (V == kPcgQAll) {
+            if (d.precond) {
+                if (kk == 0) pcgQWave<T, R, C, true, true>(d, tl, kk, beta, v[0]);
+                else pcgQWave<T, R, C, true, false>(d, tl, kk, beta, v[0]);
+            } else {
+                if (kk == 0) pcgQWave<T, R, C, false, true>(d, tl, kk, beta, v[0]);
+                else pcgQWave<T, R, C, false, false>(d, tl, kk, beta, v[0]);
+            }
         } else {
-            if (kk == 0) pcgQWave<T, R, false, true>(d, tl, kk, beta, v[0]);
-            else pcgQWave<T, R, false, false>(d, tl, kk, beta, v[0]);
+            // one form (KArgs::pvar): a slot of the other preconditioner is
+            // another launch's; a slot at another iteration, which the launch
+            // sequence never holds, gets a non-finite p.q, so that its CG
+            // fails loudly instead of taking a wrong search direction
+            constexpr bool mg = (V & kPcgQMg) != 0, first = (V & kPcgQFirst) != 0;
+            if ((d.precond != 0) != mg) continue;
+            if ((kk == 0) == first) pcgQWave<T, R, C, mg, first>(d, tl, kk, beta, v[0]);
+            else v[0] = __builtin_nan("");
         }
         blockReduceTo<1, 0u>(v, a.partials + size_t(t) * kNumPartials + kPcgQPartial, nred);
     }
@@ -1273,8 +1416,9 @@ template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
     const int nT = *a.nTiles;
     int nred = 0;  // block reductions made so far (alternating buffers)
+    constexpr int C = pColsOf(sizeof(T) == 4);
     XCD_TILE_LOOP(t, nT) {
-        const Tile tl = decodePTile<R>(a, t);
+        const Tile tl = decodePTile<R, C>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
         const int status = st.status, pdone = st.pdone, phase = st.stepPhase, kk = st.pk;
@@ -1291,20 +1435,20 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
         // preconditioner, whose beta follows the V-cycle: a multigrid slot
         // neither loads aP nor divides
         const bool jacobiZ = d.precond == 0;
-        // all loads first (column pairs, clamped addresses); stores only on
-        // unknowns
-        Pair<T> xs[R], ps[R], rs[R], qs[R], as[R];
+        // all loads first (column vectors, clamped addresses); stores only
+        // on unknowns
+        Vec<T, C> xs[R], ps[R], rs[R], qs[R], as[R];
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
-            const uint32_t k = clampPair32(tl.i + rr, j, M, P);
-            xs[rr] = ld2<T>(x, k);
-            ps[rr] = ld2(pk, k);
-            rs[rr] = ld2<T>(r, k);
-            qs[rr] = ld2(Q, k);
+            const uint32_t k = clampVec32<C>(tl.i + rr, j, M, P);
+            xs[rr] = ldv<C, T>(x, k);
+            ps[rr] = ldv<C>(pk, k);
+            rs[rr] = ldv<C, T>(r, k);
+            qs[rr] = ldv<C>(Q, k);
         }
         if (jacobiZ) {
 #pragma unroll
-            for (int rr = 0; rr < R; ++rr) as[rr] = ld2(aP, clampPair32(tl.i + rr, j, M, P));
+            for (int rr = 0; rr < R; ++rr) as[rr] = ldv<C>(aP, clampVec32<C>(tl.i + rr, j, M, P));
         }
         // p.q of the slot, while the loads above are in flight
         double pq[1];
@@ -1322,20 +1466,23 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
             const int i = tl.i + rr;
-            const bool d0 = isPDof(i, j, M, N), d1 = isPDof(i, j + 1, M, N);
+            bool dm[C];
+            Vec<T, C> xn, rn;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                dm[c] = isPDof(i, j + c, M, N);
+                xn[c] = xs[rr][c] + alpha * ps[rr][c];
+                rn[c] = rs[rr][c] - alpha * qs[rr][c];
+            }
             const uint32_t k = uint32_t(max(i, 0)) * uint32_t(P) + uint32_t(j);
-            Pair<T> xn, rn;
-            xn.x = xs[rr].x + alpha * ps[rr].x;
-            xn.y = xs[rr].y + alpha * ps[rr].y;
-            rn.x = rs[rr].x - alpha * qs[rr].x;
-            rn.y = rs[rr].y - alpha * qs[rr].y;
-            stMasked(x, k, xn, d0, d1);
-            stMasked(r, k, rn, d0, d1);
-            if (d0) v[0] += double(rn.x) * double(rn.x);
-            if (d1) v[0] += double(rn.y) * double(rn.y);
+            stMaskedV2<C>(x, xn, r, rn, k, dm);
+#pragma unroll
+            for (int c = 0; c < C; ++c)
+                if (dm[c]) v[0] += double(rn[c]) * double(rn[c]);
             if (jacobiZ) {
-                if (d0) v[1] += double(rn.x) * double(rn.x / as[rr].x);
-                if (d1) v[1] += double(rn.y) * double(rn.y / as[rr].y);
+#pragma unroll
+                for (int c = 0; c < C; ++c)
+                    if (dm[c]) v[1] += double(rn[c]) * double(rn[c] / as[rr][c]);
             }
         }
         blockReduceTo<2, 0u>(v, a.partials + size_t(t) * kNumPartials, nred);
@@ -1349,36 +1496,43 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_refine_accum(KArgs a) {
     const int nT = *a.nTiles;
+    constexpr int C = pColsOf(sizeof(T) == 4);
     XCD_TILE_LOOP(t, nT) {
-        const Tile tl = decodePTile<R>(a, t);
+        const Tile tl = decodePTile<R, C>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
         if (st.status != ST_RUNNING || st.stepPhase) continue;
         const int M = d.M, N = d.N, P = d.pitch, j = tl.j;
         const auto x = G(static_cast<const T*>(d.pr[P_X]));
         const auto pc = G(d.pc);
-        // all loads first (column pairs, clamped addresses); the first cycle
-        // accumulates onto p' = 0 (the pressure assembly does not clear it)
+        // all loads first (column vectors, clamped addresses); the first
+        // cycle accumulates onto p' = 0 (the pressure assembly does not clear
+        // it)
         const bool first = st.pcycle == 0;
-        Pair<T> xs[R];
-        Pair<double> ps[R];
+        Vec<T, C> xs[R];
+        Vec<double, C> ps[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const uint32_t k = clampPair32(tl.i + r, j, M, P);
-            xs[r] = ld2(x, k);
-            ps[r].x = ps[r].y = 0.0;
+            const uint32_t k = clampVec32<C>(tl.i + r, j, M, P);
+            xs[r] = ldv<C>(x, k);
+#pragma unroll
+            for (int c = 0; c < C; ++c) ps[r][c] = 0.0;
         }
         if (!first) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) ps[r] = ld2<double>(pc, clampPair32(tl.i + r, j, M, P));
+            for (int r = 0; r < R; ++r) ps[r] = ldv<C, double>(pc, clampVec32<C>(tl.i + r, j, M, P));
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int i = tl.i + r;
-            Pair<double> pn;
-            pn.x = ps[r].x + double(xs[r].x);
-            pn.y = ps[r].y + double(xs[r].y);
-            stMasked(pc, uint32_t(i) * uint32_t(P) + uint32_t(j), pn, isPDof(i, j, M, N), isPDof(i, j + 1, M, N));
+            Vec<double, C> pn;
+            bool dm[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                pn[c] = ps[r][c] + double(xs[r][c]);
+                dm[c] = isPDof(i, j + c, M, N);
+            }
+            stMaskedV<C>(pc, uint32_t(i) * uint32_t(P) + uint32_t(j), pn, dm);
         }
     }
 }
@@ -1387,8 +1541,9 @@ template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_refine_resid(KArgs a) {
     const int nT = *a.nTiles;
     int nred = 0;  // block reductions made so far (alternating buffers)
+    constexpr int C = pColsOf(sizeof(T) == 4);
     XCD_TILE_LOOP(t, nT) {
-        const Tile tl = decodePTile<R>(a, t);
+        const Tile tl = decodePTile<R, C>(a, t);
         const SlotDesc& d = a.desc[tl.s];
         SlotState& st = a.state[tl.s];
         if (st.status != ST_RUNNING || st.stepPhase) continue;
@@ -1399,8 +1554,8 @@ __global__ __launch_bounds__(kBlock) void k_refine_resid(KArgs a) {
         const auto pc = G(d.pc);
         double v[2] = {0.0, 0.0};
 #pragma unroll
-        for (int n = 0; n < 2 * R; ++n) {
-            const int i = tl.i + (n >> 1), j = tl.j + (n & 1);
+        for (int n = 0; n < C * R; ++n) {
+            const int i = tl.i + n / C, j = tl.j + n % C;
             if (!isPDof(i, j, M, N)) continue;
             const size_t k = size_t(i) * P + j;
             auto pAt64 = [&](int ii, int jj, size_t kn) -> double { return isPDof(ii, jj, M, N) ? pc[kn] : 0.0; };
@@ -2188,41 +2343,93 @@ __global__ __launch_bounds__(kBlock) void k_correct(KArgs a) {
         const double Uin = st.inletVelocity;
         const double maxVel = 3.0 * fmax(fabs(Uin), 0.1);
         const double pAlpha = d.pAlpha;
-        const auto gp_ = G(d.p), pc = G(d.pc), dE = G(d.dE), dN = G(d.dN);
+        const auto gp_ = G(d.p), dE = G(d.dE), dN = G(d.dN);
         const auto fE = G(reinterpret_cast<const float*>(d.dE)), fN = G(reinterpret_cast<const float*>(d.dN));
-        const auto us = G(d.us), vs = G(d.vs), gam = G(d.gam), u = G(d.u), vv = G(d.v);
-        // the u row at the outlet column copies its west neighbour, the v row
-        // at j = N too: load the node each row reads, from clamped addresses
-        const int ju = (j == N - 1) ? j - 1 : j, jv = (j == N) ? j - 1 : j;
-        double pr[R], pcc[R], dEu[R], pcu0[R], pcu1[R], usu[R], ge[R], dNv[R], pcv0[R], pcv1[R], vsv[R];
+        const auto gam = G(d.gam), u = G(d.u), vv = G(d.v);
+        // p' as the single-precision CG iterate (one refinement cycle), and
+        // u*, v* formed here from u, v and the last sweep's delta, in the
+        // folded fixed-budget step (KArgs::fold)
+        const bool pcFromX = ((a.fold & kFoldPc) != 0) & (d.presType != 0);
+        const bool usFolded = ((a.fold & kFoldMom) != 0) & (d.momType == 1);
+        // The outlet column of u (j = N-1) copies its west neighbour, and the
+        // column j = N of v too: the lane of that neighbour writes both nodes,
+        // so each lane reads p', u*, v*, dE and dN at its own node (p' also
+        // east and north) and no lane reads a node another lane writes.
+        double pr[R], pc0[R], pcE[R], pcN[R], usu[R], vsv[R], dEu[R], dNv[R], ge[R];
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
             const int i = tl.i + rr;
-            const size_t kc = clampIdx(i, j, M, P), ku = clampIdx(i, ju, M, P), kv = clampIdx(i, jv, M, P);
+            const size_t kc = clampIdx(i, j, M, P);
             pr[rr] = gp_[kc];
-            pcc[rr] = pc[kc];
-            dEu[rr] = 0.0;
-            pcu0[rr] = pc[ku];
-            pcu1[rr] = pc[clampIdx(i, ju + 1, M, P)];
-            usu[rr] = us[ku];
             ge[rr] = gam[clampIdx(i, 1, M, P)];  // read at j = 0 only: one address per row
-            dNv[rr] = 0.0;
-            pcv0[rr] = pc[kv];
-            pcv1[rr] = pc[clampIdx(i + 1, jv, M, P)];
-            vsv[rr] = vs[kv];
+        }
+        if (pcFromX) {
+            const auto x = G(static_cast<const float*>(d.pr[P_X]));
+#pragma unroll
+            for (int rr = 0; rr < R; ++rr) {
+                const int i = tl.i + rr;
+                pc0[rr] = x[clampIdx(i, j, M, P)];
+                pcE[rr] = x[clampIdx(i, j + 1, M, P)];
+                pcN[rr] = x[clampIdx(i + 1, j, M, P)];
+            }
+        } else {
+            const auto pc = G(d.pc);
+#pragma unroll
+            for (int rr = 0; rr < R; ++rr) {
+                const int i = tl.i + rr;
+                pc0[rr] = pc[clampIdx(i, j, M, P)];
+                pcE[rr] = pc[clampIdx(i, j + 1, M, P)];
+                pcN[rr] = pc[clampIdx(i + 1, j, M, P)];
+            }
+        }
+        if (usFolded) {
+            // k_mom_finish's interior value clamp(u + delta); the correction
+            // reads u* only inside (1 <= i <= M-1, 1 <= j <= N-2) and v* only
+            // inside (1 <= i <= M-2, 1 <= j <= N-1)
+            const DeltaSrc sU = deltaSrc(d, st, 0), sV = deltaSrc(d, st, 1);
+            double uk[R], vk[R];
+            float xu[R], yu[R], xv[R], yv[R];
+#pragma unroll
+            for (int rr = 0; rr < R; ++rr) {
+                const size_t kc = clampIdx(tl.i + rr, j, M, P);
+                uk[rr] = u[kc];
+                vk[rr] = vv[kc];
+                xu[rr] = sU.x[kc];
+                yu[rr] = sU.y[kc];
+                xv[rr] = sV.x[kc];
+                yv[rr] = sV.y[kc];
+            }
+#pragma unroll
+            for (int rr = 0; rr < R; ++rr) {
+                const int i = tl.i + rr;
+                const float du = sU.one ? delta1(xu[rr], yu[rr], i, j, M - 1, N - 2) : xu[rr];
+                const float dv = sV.one ? delta1(xv[rr], yv[rr], i, j, M - 2, N - 1) : xv[rr];
+                usu[rr] = clampv(uk[rr] + double(du), maxVel);
+                vsv[rr] = clampv(vk[rr] + double(dv), maxVel);
+            }
+        } else {
+            const auto us = G(d.us), vs = G(d.vs);
+#pragma unroll
+            for (int rr = 0; rr < R; ++rr) {
+                const size_t kc = clampIdx(tl.i + rr, j, M, P);
+                usu[rr] = us[kc];
+                vsv[rr] = vs[kc];
+            }
         }
         // dE, dN as floats or doubles (SlotDesc::dE32), a uniform choice
         if (d.dE32) {
 #pragma unroll
             for (int rr = 0; rr < R; ++rr) {
-                dEu[rr] = fE[clampIdx(tl.i + rr, ju, M, P)];
-                dNv[rr] = fN[clampIdx(tl.i + rr, jv, M, P)];
+                const size_t kc = clampIdx(tl.i + rr, j, M, P);
+                dEu[rr] = fE[kc];
+                dNv[rr] = fN[kc];
             }
         } else {
 #pragma unroll
             for (int rr = 0; rr < R; ++rr) {
-                dEu[rr] = dE[clampIdx(tl.i + rr, ju, M, P)];
-                dNv[rr] = dN[clampIdx(tl.i + rr, jv, M, P)];
+                const size_t kc = clampIdx(tl.i + rr, j, M, P);
+                dEu[rr] = dE[kc];
+                dNv[rr] = dN[kc];
             }
         }
         double bad = 0.0;
@@ -2233,7 +2440,7 @@ __global__ __launch_bounds__(kBlock) void k_correct(KArgs a) {
             const size_t k = size_t(i) * P + j;
             // pressure update + ghosts; each ghost has exactly one writer
             if (i >= 1 && i <= M - 1 && j >= 1 && j <= N - 1) {
-                const double pn = pr[rr] + pAlpha * pcc[rr];
+                const double pn = pr[rr] + pAlpha * pc0[rr];
                 gp_[k] = pn;
                 if (i == 1) gp_[k - P] = pn;      // p(0,j) = p(1,j)
                 if (i == M - 1) gp_[k + P] = pn;  // p(M,j) = p(M-1,j)
@@ -2241,24 +2448,29 @@ __global__ __launch_bounds__(kBlock) void k_correct(KArgs a) {
                 if (j == N - 1) gp_[k + 1] = 0.0; // p(i,N) = 0
                 if (!(pn == pn)) bad = 1.0;
             }
+            // u: walls, inlet, inside; the outlet column from column N-2
             if (j <= N - 1) {
-                double val;
-                if (i == 0 || i == M) val = 0.0;
-                else if (j == 0) val = (ge[rr] > 0.5) ? Uin : 0.0;
-                else {  // interior, or the outlet column from its west neighbour
-                    const double du = clampv(dEu[rr] * (pcu0[rr] - pcu1[rr]), maxVel);
-                    val = clampv(usu[rr] + du, maxVel);
+                if (i == 0 || i == M) {
+                    u[k] = 0.0;
+                } else if (j == 0) {
+                    u[k] = (ge[rr] > 0.5) ? Uin : 0.0;
+                } else if (j <= N - 2) {
+                    const double du = clampv(dEu[rr] * (pc0[rr] - pcE[rr]), maxVel);
+                    const double val = clampv(usu[rr] + du, maxVel);
+                    u[k] = val;
+                    if (j == N - 2) u[k + 1] = val;  // u(i,N-1) = u(i,N-2)
                 }
-                u[k] = val;
             }
+            // v: walls, inflow column, inside; the outlet column from N-1
             if (i <= M - 1) {
-                double val;
-                if (i == 0 || i == M - 1 || j == 0) val = 0.0;
-                else {  // interior, or j = N from its west neighbour
-                    const double dv = clampv(dNv[rr] * (pcv0[rr] - pcv1[rr]), maxVel);
-                    val = clampv(vsv[rr] + dv, maxVel);
+                if (i == 0 || i == M - 1 || j == 0) {
+                    vv[k] = 0.0;
+                } else if (j <= N - 1) {
+                    const double dv = clampv(dNv[rr] * (pc0[rr] - pcN[rr]), maxVel);
+                    const double val = clampv(vsv[rr] + dv, maxVel);
+                    vv[k] = val;
+                    if (j == N - 1) vv[k + 1] = val;  // v(i,N) = v(i,N-1)
                 }
-                vv[k] = val;
             }
         }
         double vb[1] = {bad};
@@ -2475,6 +2687,23 @@ __device__ __forceinline__ void mgZFinalize(const KArgs& a, const SlotDesc& d, S
     }
 }
 
+// The momentum finish's scalars (thread 0): the residuals and the
+// non-finite flag of its partials; with `budgetEnds` (implicit momentum) a
+// system still marked running made the budget's last sweep unfinalised.
+__device__ __forceinline__ void momFinishScalars(const SlotDesc& d, SlotState& st, double residU, double residV,
+                                                 double flag, bool budgetEnds) {
+    st.residU = residU;
+    st.residV = residV;
+    if (flag != 0.0) st.health = 1;
+    for (int c = 0; c < 2; ++c)
+        if (budgetEnds && !st.jdone[c]) {
+            st.jdone[c] = 1;
+            st.jiters[c] = d.maxMomIter;
+        }
+    st.itU = st.jiters[0];
+    st.itV = st.jiters[1];
+}
+
 __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int outer) {
     const int s = blockIdx.x;
     // The slot's descriptor and state are read into LDS in one round trip
@@ -2554,18 +2783,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
             if (d.momType != (phase == PH_MOM_EXPL ? 0 : 1)) return;
             double tot[3];
             slotReduce<3, 7u>(a.partials, d.tile0, d.ntiles, tot);
-            if (threadIdx.x == 0) {
-                st.residU = tot[0];
-                st.residV = tot[1];
-                if (tot[2] != 0.0) st.health = 1;
-                for (int c = 0; c < 2; ++c)
-                    if (phase == PH_MOM_FIN && !st.jdone[c]) {  // the budget's last sweep, unfinalised
-                        st.jdone[c] = 1;
-                        st.jiters[c] = d.maxMomIter;
-                    }
-                st.itU = st.jiters[0];
-                st.itV = st.jiters[1];
-            }
+            if (threadIdx.x == 0) momFinishScalars(d, st, tot[0], tot[1], tot[2], phase == PH_MOM_FIN);
             break;
         }
         case PH_SOR: {
@@ -2581,9 +2799,22 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
             break;
         }
         case PH_PRES_ASM: {
-            double tot[2];
-            slotReduce<2, 1u>(a.partials, d.tile0, d.ntiles, tot);
+            // a folded assembly also holds the finish's partials of an
+            // implicit-momentum slot (kFoldFinPartial ..)
+            double tot[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            const bool folded = (a.fold & kFoldMom) != 0;
+            if (folded) {
+                slotReduce<5, 29u>(a.partials, d.tile0, d.ntiles, tot);
+            } else {
+                double t2[2];
+                slotReduce<2, 1u>(a.partials, d.tile0, d.ntiles, t2);
+                tot[0] = t2[0];
+                tot[1] = t2[1];
+            }
             if (threadIdx.x == 0) {
+                if (folded && d.momType == 1)
+                    momFinishScalars(d, st, tot[kFoldFinPartial], tot[kFoldFinPartial + 1], tot[kFoldFinPartial + 2],
+                                     true);
                 st.residMass = tot[0];
                 st.rz = tot[1];
                 st.beta = 0.0;
@@ -2682,7 +2913,23 @@ static void launchTR(Kern k, int grid, hipStream_t s, const KArgs& a) {
         case K_JAC: hipLaunchKernelGGL((k_jacobi<T, R>), g, b, 0, s, a); break;
         case K_MOM_FIN: hipLaunchKernelGGL((k_mom_finish<T, R>), g, b, 0, s, a); break;
         case K_PRES_ASM: hipLaunchKernelGGL((k_pres_assemble<T, R>), g, b, 0, s, a); break;
-        case K_PCG_Q: hipLaunchKernelGGL((k_pcg_q<T, R>), g, b, 0, s, a); break;
+        case K_PCG_Q:
+            if constexpr (sizeof(T) == 4) {
+                switch (a.pvar) {
+                    case kPcgQMg: hipLaunchKernelGGL((k_pcg_q<T, R, kPcgQMg>), g, b, 0, s, a); break;
+                    case kPcgQMg | kPcgQFirst:
+                        hipLaunchKernelGGL((k_pcg_q<T, R, kPcgQMg | kPcgQFirst>), g, b, 0, s, a);
+                        break;
+                    case kPcgQJac: hipLaunchKernelGGL((k_pcg_q<T, R, kPcgQJac>), g, b, 0, s, a); break;
+                    case kPcgQJac | kPcgQFirst:
+                        hipLaunchKernelGGL((k_pcg_q<T, R, kPcgQJac | kPcgQFirst>), g, b, 0, s, a);
+                        break;
+                    default: hipLaunchKernelGGL((k_pcg_q<T, R, kPcgQAll>), g, b, 0, s, a); break;
+                }
+            } else {
+                hipLaunchKernelGGL((k_pcg_q<T, R, kPcgQAll>), g, b, 0, s, a);
+            }
+            break;
         case K_PCG_U: hipLaunchKernelGGL((k_pcg_update<T, R>), g, b, 0, s, a); break;
         case K_REF_ACC: hipLaunchKernelGGL((k_refine_accum<T, R>), g, b, 0, s, a); break;
         case K_REF_RES: hipLaunchKernelGGL((k_refine_resid<T, R>), g, b, 0, s, a); break;

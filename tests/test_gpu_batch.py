@@ -255,17 +255,32 @@ def test_small_sweep_budgets_match_oracle(sweeps):
     b.close()
 
 
-@pytest.mark.parametrize("cycles,precond", [(1, cb.CP_PRECOND_AMG), (2, cb.CP_PRECOND_AMG),
-                                            (2, cb.CP_PRECOND_JACOBI)], ids=["amg-1", "amg-2", "jacobi-2"])
-def test_graph_batch_matches_polled_mixed(cycles, precond):
-    """Mixed precision, 3 sweeps / 4 CG iterations per refinement cycle: the
-    graph step (its finalisations merged where a later one can carry them)
-    equals the polled context, which finalises every phase on its own, bit
-    for bit and with the same inner counts."""
+@pytest.mark.parametrize(
+    "cycles,precond,sweeps,mom,pres",
+    [(1, cb.CP_PRECOND_AMG, 3, 1, 1), (2, cb.CP_PRECOND_AMG, 3, 1, 1), (2, cb.CP_PRECOND_JACOBI, 3, 1, 1),
+     (1, cb.CP_PRECOND_JACOBI, 3, 1, 1), (1, cb.CP_PRECOND_AMG, 1, 1, 1), (1, cb.CP_PRECOND_AMG, 2, 1, 1),
+     (1, cb.CP_PRECOND_AMG, 3, 0, 1), (1, cb.CP_PRECOND_AMG, 3, 1, 0)],
+    ids=["amg-1", "amg-2", "jacobi-2", "jacobi-1", "amg-1-sweeps1", "amg-1-sweeps2", "explicit-cg", "implicit-sor"])
+def test_graph_batch_matches_polled_mixed(cycles, precond, sweeps, mom, pres):
+    """Mixed precision, 4 CG iterations per refinement cycle: the graph step
+    equals the polled context bit for bit and with the same inner counts.
+    The graph step merges finalisations where a later one can carry them,
+    forms u* and v* of an implicit-momentum slot in the pressure assembly
+    and the correction instead of storing them, and with one refinement
+    cycle reads p' as the CG iterate itself; the context finalises every
+    phase on its own and stores u*, v* and p'. The cases cover both
+    preconditioners (each its own form of the CG stencil), one and two
+    refinement cycles, momentum budgets ending after the assembly's sweep,
+    on an even and on an odd sweep, and the explicit-momentum and SOR
+    backends, which the graph step leaves unfolded."""
     p, ct = wl.network(64, 128, 3, cb.CP_PREC_MIXED)
-    q = wl.fast_mode(p, 3, 4, cycles)
+    q = wl.fast_mode(p, sweeps, 4, cycles)
     q.pressurePrecond = precond
-    say(f"[batch] mixed, {cycles} refinement cycles, preconditioner {precond}: graph vs polled")
+    q.momentumSolverType, q.pressureSolverType = mom, pres
+    if pres == 0:
+        q.maxPressureIter = 20
+    say(f"[batch] mixed, {cycles} cycles, precond {precond}, {sweeps} sweeps, momentum {mom}, pressure {pres}: "
+        "graph vs polled")
     b = cb.Batch(cb.CP_PREC_MIXED, 64 << 20, 1)
     b.set_tiling(1)  # the context's tiling for a 64 x 128 grid
     s = b.admit(q, ct)
@@ -276,7 +291,8 @@ def test_graph_batch_matches_polled_mixed(cycles, precond):
         info = b.poll()[s]
         for k in ("itU", "itV", "itP", "iter"):
             assert info[k] == sg[k], (step, k, info[k], sg[k])
-        assert info["error"] == sg["error"], (step, info["error"], sg["error"])
+        for k in ("error", "residU", "residV", "residMass"):
+            assert info[k] == sg[k], (step, k, info[k], sg[k])
     u, v, pp = b.fields(s)
     np.testing.assert_array_equal(u, g.field(cb.F_U))
     np.testing.assert_array_equal(v, g.field(cb.F_V))
