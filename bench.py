@@ -130,9 +130,9 @@ def parse():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run 3 device-to-device copies of MIB MiB (byte-counter calibration)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tiling (no autotuning pass)")
-    ap.add_argument("--tiling", default="", metavar="R,MG[,ORDER]",
-                    help="fixed tiling instead of autotuning: rows per wave R, finest multigrid rows MG and the "
-                         "tile-order mask (cp_batch_set_tile_order, default 0)")
+    ap.add_argument("--tiling", default="", metavar="R,MG[,ORDER[,WG]]",
+                    help="fixed tiling instead of autotuning: rows per wave R, finest multigrid rows MG, the "
+                         "tile-order mask (cp_batch_set_tile_order, default 0) and workgroups per CU (default 16)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the step's kernels directly instead of replaying the captured graph")
     ap.add_argument("--profile-steps", type=int, default=0,
@@ -557,8 +557,8 @@ def main() -> int:
     tuned = None
     if a.tiling:
         f_ = [int(x) for x in a.tiling.split(",")]
-        tuned = {"tile_rows": f_[0], "mg_rows": f_[1], "wg_per_cu": 16, "tile_order": f_[2] if len(f_) > 2 else 0,
-                 "fixed": True}
+        tuned = {"tile_rows": f_[0], "mg_rows": f_[1], "wg_per_cu": f_[3] if len(f_) > 3 else 16,
+                 "tile_order": f_[2] if len(f_) > 2 else 0, "fixed": True}
     elif not a.no_tune:
         # autotune the tiling on the workload's own geometries (one batch
         # alone), then start every measured step from empty batches
