@@ -40,6 +40,9 @@ Reported beside the value (one JSON line):
     k_mg_up, which with k_pcg_q holds the largest shares of GPU time (four
     launches of each per outer iteration); roofline_mom_assemble: the same
     for the momentum assembly, the largest single launch;
+  * field_l2_err: relative L2 errors of the served form's final u, v and p
+    on seed 0 against the device FP64 fixed point of that geometry (whose
+    pressure drop is checked against the CPU FP64 oracle's);
   * plain_fp64_single: the plain FP64 single-geometry path (the drop-in:
     reference algorithm and inner budgets, Jacobi-preconditioned CG,
     polled) on every seed with the same stop test;
@@ -127,6 +130,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dropin", action="store_true")
     ap.add_argument("--no-probe", action="store_true", help="no timing bursts")
+    ap.add_argument("--no-field-check", action="store_false", dest="field_check",
+                    help="skip the L2 errors of the final fields against the FP64 fixed point")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run 3 device-to-device copies of MIB MiB (byte-counter calibration)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tiling (no autotuning pass)")
@@ -164,6 +169,56 @@ def served_params(p, a, eps, cap=20000):
     q.pTol = 1e-12
     q.maxIterations = a.profile_steps if a.profile_steps else cap
     return q
+
+
+def field_accuracy(a, tuned, refs: list[float]) -> dict:
+    """Relative L2 errors of the served form's final u, v and p on seed 0
+    against the FP64 fixed point of the same geometry. The served settings
+    are solved alone on a one-slot batch (a slot's result is bit-identical
+    alone and in the pool); the reference is the device FP64 path (whose
+    sub-steps equal the oracle's) run to a tight fixed point: the oracle's
+    strong settings (mass residual over the p' columns, epsilon 1e-9, CG
+    tolerance 1e-10, at most 300 CG iterations) with the multigrid
+    preconditioner, which changes the path to the fixed point, not the point.
+    Its pressure drop is checked against the CPU oracle's."""
+    p, ct = GEOMS[0]
+    q = served_params(p, a, a.eps)
+    pb = cb.Batch(cb.CP_PREC_MIXED, 1 << 20, 1)
+    need = pb.footprint(p.M, p.N) + (64 << 20)
+    pb.close()
+    b = cb.Batch(cb.CP_PREC_MIXED, need, 1)
+    if tuned:
+        b.set_tiling(tuned["tile_rows"], tuned["wg_per_cu"])
+        b.set_mg_tiling(tuned["mg_rows"])
+        b.set_tile_order(tuned["tile_order"])
+    s = b.admit(q, ct)
+    while b.poll()[s]["status"] == cb.SLOT_RUNNING:
+        b.run(a.check)
+    info = b.poll()[s]
+    u, v, pp = b.fields(s)
+    dp = b.full_dp(s, GAMMA_MIN)
+    b.close()
+    r = p.copy()
+    r.precision = cb.CP_PREC_FP64
+    r.massResidualScope = 1
+    r.epsilon = 1e-9
+    r.pTol = 1e-10
+    r.maxPressureIter = 300
+    r.pressurePrecond = cb.CP_PRECOND_AMG
+    r.maxIterations = 20000
+    t0 = time.perf_counter()
+    g = cb.Solver(r, ct)
+    st = g.run()
+    ref = [g.field(f) for f in (cb.F_U, cb.F_V, cb.F_P)]
+    dp64 = g.full_dp(GAMMA_MIN)
+    g.close()
+    l2 = {n: float(np.linalg.norm(x - y) / np.linalg.norm(y)) for n, x, y in zip(("u", "v", "p"), (u, v, pp), ref)}
+    return {"seed": 0, "u": l2["u"], "v": l2["v"], "p": l2["p"], "dp_rel_err": abs(dp / dp64 - 1),
+            "served_iterations": info["iter"], "served_status": info["status"],
+            "reference": "device FP64 fixed point (epsilon 1e-9, CG tolerance 1e-10, multigrid preconditioner), "
+                         f"{st['iter']} outer iterations in {time.perf_counter() - t0:.1f} s; its pressure drop "
+                         f"against the CPU FP64 oracle's: {abs(dp64 / refs[0] - 1):.1e}",
+            "norm": "||x - x_ref||_2 / ||x_ref||_2 over the field's whole staggered array"}
 
 
 def network_requests(a, n: int, arrivals=None) -> list:
@@ -614,6 +669,9 @@ def main() -> int:
         log(f"latency run: {latency}")
     for x in bs:
         x.close()
+    fields = field_accuracy(a, tuned, refs) if a.field_check else None
+    if fields:
+        log(f"field accuracy: {fields}")
     stream5 = stream_subsample(a, a.stream_requests) if a.stream_requests > 0 else None
     if stream5:
         log(f"served-stream subsample: {stream5}")
@@ -680,6 +738,7 @@ def main() -> int:
                            "basis": "admission to completion in the timed steps"},
         "dp_rel_err": {"max": float(max(errs)), "median": float(np.median(errs)), "reference": "FP64 oracle"},
         "outer_iterations": {"mean": round(float(np.mean(iters)), 1), "max": int(max(iters))},
+        "field_l2_err": fields,
         "stream_config5": stream5,
         "plain_fp64_single": None if not drop else {
             "solves_per_s": round(N_SEEDS / drop["seconds"], 4),

@@ -5,7 +5,8 @@ same algorithm, through the C ABI.
 Tolerances: FP64 fields within 1e-9 of their scale per step with identical
 inner iteration counts (only summation order and contraction differ); the
 mixed-precision form to its converged full-system pressure drop within 1e-6
-of the FP64 oracle's (the bench's bar is 1e-4).
+of the FP64 oracle's (the bench's bar is 1e-4) and to its converged u, v
+and p within 1e-5 in relative L2 norm.
 """
 from __future__ import annotations
 
@@ -108,10 +109,16 @@ def test_mg_batch_graph_fp64_matches_oracle():
     b.close()
 
 
+def rel_l2(a, b) -> float:
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
 @pytest.mark.parametrize("kind", ["network", "gamma"])
 def test_mg_mixed_converges_to_fp64_oracle(kind):
     """Mixed precision (FP32 V-cycles and CG, FP64 refinement) with four CG
-    iterations per cycle converges to the FP64 oracle's fixed point."""
+    iterations per cycle converges to the FP64 oracle's fixed point: its
+    pressure drop, and its final u, v and p in relative L2 norm over the
+    whole staggered arrays."""
     if kind == "network":
         p, ct = wl.network(48, 128, 6, cb.CP_PREC_MIXED)
     else:
@@ -143,6 +150,11 @@ def test_mg_mixed_converges_to_fp64_oracle(kind):
     assert st["converged"] == 1
     ref = o.full_dp(gmin)
     assert abs(dp / ref - 1) <= 1e-6, (dp, ref, info)
+    fields = b.fields(s)
+    errs = {n: rel_l2(x, o.field(f)) for n, x, f in zip("uvp", fields, (cb.F_U, cb.F_V, cb.F_P))}
+    say(f"[mg] mixed {kind}: relative L2 errors against the oracle {errs}, dp {abs(dp / ref - 1):.2e}")
+    for n, e in errs.items():
+        assert e <= 1e-5, (n, e, errs)
     b.close()
 
 
