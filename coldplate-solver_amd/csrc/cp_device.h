@@ -238,12 +238,15 @@ hipError_t launchWorkOnly(Kern k, bool f32, int rows, int grid, hipStream_t s, c
 // The finalisation kernel alone (one workgroup per slot) for `phase`.
 hipError_t launchFinalize(int phase, int nslots, hipStream_t s, const KArgs& a);
 // Rows per wave of the finest multigrid level's tiles for a batch tile
-// height (coarser levels use 2). Even, so that a wave's rows hold whole 2 x 2
-// aggregates; 4 and 8 only in single precision (register footprint). The
-// two sweeps reach two rows out, so a wave loads R + 4 rows for its R: at 8
-// rows the finest up-sweep over 8 x 1024x512 slots takes 40.8 instead of
-// 46.4 us per launch.
+// height. Even, so that a wave's rows hold whole 2 x 2 aggregates; 4 and 8
+// only in single precision (register footprint). The two sweeps reach two
+// rows out, so a wave loads R + 4 rows for its R: at 8 rows the finest
+// up-sweep over 8 x 1024x512 slots takes 40.8 instead of 46.4 us per launch.
 inline int mgRows(bool f32, int rows) { return (f32 && rows >= 4) ? 8 : 2; }
+// Rows per wave of the coarser grid-wide levels' tiles: 8 in single
+// precision (fewer, taller tiles: the 4 x 16-slot pool serves 1.8 % more
+// than with 2; one batch alone is as fast), 2 in double precision.
+inline int mgCoarseRows(bool f32) { return f32 ? 8 : 2; }
 // Rows per wave of the p' list's tiles: 4 in single precision (measured on
 // 8 x 1024x512: k_pcg_q 21.6 us with 4 against 22.9 us with 2, the halo rows
 // being a smaller share), 1 in double precision (register footprint of the

@@ -391,7 +391,7 @@ int launchMg(cp_batch* b, Kern k, int level, int finalPhase = -1) {
     a.nTiles = b->dMgNTiles + level;
     a.level = level;
     a.order = (b->tileOrder >> 2) & 1;
-    const int rows = level == 0 ? b->mgRowsEff() : 2;
+    const int rows = level == 0 ? b->mgRowsEff() : mgCoarseRows(f32);
     const int grid = perSlot ? b->maxSlots : mgGrid(b, level);
     hipError_t e;
     if (finalPhase < 0) {
@@ -607,7 +607,7 @@ void setTiling(SlotDesc& d, int tileRows, int mgRows0) {
     d.pntiles = d.ptilesX * ((d.M - 1 + pRowsPerTile - 1) / pRowsPerTile);
     for (int l = 0; l < d.nLevels; ++l) {
         MgLevel& L = d.lv[l];
-        const int rows = kTileH * (l == 0 ? mgRows0 : 2);
+        const int rows = kTileH * (l == 0 ? mgRows0 : mgCoarseRows(d.mixed != 0));
         L.tilesX = (L.W + kMgTileW - 1) / kMgTileW;
         L.ntiles = L.tilesX * ((L.H + rows - 1) / rows);
     }

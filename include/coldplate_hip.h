@@ -311,9 +311,9 @@ int cp_batch_set_tiling(cp_batch* b, int tile_rows, int wg_per_cu);
 int cp_batch_set_graph(cp_batch* b, int use_graph);
 
 /* Rows per wave R of the finest multigrid level's 60 x 4R tiles (no
-   counterpart in the reference; coarser levels use R = 2): 2, 4 or 8 (4 and
-   8 in mixed precision only); 0 derives it from the batch tile height.
-   Every slot is retiled. */
+   counterpart in the reference; the coarser levels use 8 in mixed precision
+   and 2 in FP64): 2, 4 or 8 (4 and 8 in mixed precision only); 0 derives it
+   from the batch tile height. Every slot is retiled. */
 int cp_batch_set_mg_tiling(cp_batch* b, int rows);
 
 /* Tile order of the batch's launches (no counterpart in the reference), a
