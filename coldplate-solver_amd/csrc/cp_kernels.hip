@@ -1163,7 +1163,9 @@ __global__ __launch_bounds__(kBlock) void k_pres_assemble(KArgs a) {
         const bool keepB = (d.presType == 0) | (d.refineCycles > 1);
         // p' starts at 0 for SOR, which sweeps on it, and in FP64, where the
         // CG iterate is p' itself; a mixed-precision CG slot's p' is set whole
-        // by the first refinement accumulation (p' = 0 + x)
+        // by the first refinement accumulation (p' = 0 + x), or, in the folded
+        // fixed-budget step with one refinement cycle, not at all: the
+        // correction reads x
         const bool zeroPc = (d.presType == 0) | (d.mixed == 0);
         // all loads first (clamped addresses)
         double uE[R], uW[R], vN[R], vS[R], dEe[R], dEw[R], dNn[R], dNs[R];
