@@ -359,11 +359,28 @@ __device__ __forceinline__ uint32_t clampVec32(int i, int j, int M, int P) {
 }
 
 // lane l receives lane l-1's value (west) / lane l+1's value (east); lanes 0
-// and 63 receive their own, which only halo lanes consume
+// and 63 receive their own, which only halo lanes consume. A DPP move with a
+// whole-wave shift (wave_shr:1 = 0x138, wave_shl:1 = 0x130) does this in the
+// vector ALU; a lane without a source keeps the old value, its own (bound_ctrl
+// off). 8-byte values move as two dwords.
+template <int Ctrl, typename T>
+__device__ __forceinline__ T dppWaveShift(T x) {
+    if constexpr (sizeof(T) == 4) {
+        const int v = __builtin_bit_cast(int, x);
+        return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(v, v, Ctrl, 0xf, 0xf, false));
+    } else {
+        static_assert(sizeof(T) == 8, "4- or 8-byte lanes");
+        const unsigned long long v = __builtin_bit_cast(unsigned long long, x);
+        const int lo = int(uint32_t(v)), hi = int(uint32_t(v >> 32));
+        const uint32_t rl = uint32_t(__builtin_amdgcn_update_dpp(lo, lo, Ctrl, 0xf, 0xf, false));
+        const uint32_t rh = uint32_t(__builtin_amdgcn_update_dpp(hi, hi, Ctrl, 0xf, 0xf, false));
+        return __builtin_bit_cast(T, (static_cast<unsigned long long>(rh) << 32) | rl);
+    }
+}
 template <typename T>
-__device__ __forceinline__ T fromWest(T x) { return __shfl_up(x, 1, 64); }
+__device__ __forceinline__ T fromWest(T x) { return dppWaveShift<0x138>(x); }
 template <typename T>
-__device__ __forceinline__ T fromEast(T x) { return __shfl_down(x, 1, 64); }
+__device__ __forceinline__ T fromEast(T x) { return dppWaveShift<0x130>(x); }
 
 // ---------------------------------------------------------------------------
 // momentum assembly (momentum_solver.cpp:185-298, delta form)
@@ -1769,9 +1786,15 @@ struct MgView {
           cN(G(static_cast<const T*>(L.cN))), b(G(static_cast<const T*>(L.b))), x(G(static_cast<const T*>(L.x))),
           xo(G(static_cast<T*>(L.x))), z(G(static_cast<T*>(L.z))), bo(G(static_cast<T*>(L.b))) {}
     __device__ __forceinline__ bool in(int I, int J) const { return I >= 0 && I < H && J >= 0 && J < W; }
-    __device__ __forceinline__ size_t at(int I, int J) const { return size_t(I + 1) * P + (J + 1); }
+    // element index of (I, J) inside the level (0 <= I < H, 0 <= J < W), in
+    // 32 bits so that ld32 / st32 address it from the array's scalar base
+    __device__ __forceinline__ uint32_t at(int I, int J) const {
+        return uint32_t(I + 1) * uint32_t(P) + uint32_t(J + 1);
+    }
     // index of (I, J) clamped into the level (a valid address for any I, J)
-    __device__ __forceinline__ size_t atc(int I, int J) const { return at(min(max(I, 0), H - 1), min(max(J, 0), W - 1)); }
+    __device__ __forceinline__ uint32_t atc(int I, int J) const {
+        return at(min(max(I, 0), H - 1), min(max(J, 0), W - 1));
+    }
 };
 
 // (A x) at a node from its own and its four neighbours' values, terms in
@@ -1815,10 +1838,10 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarsen(KArgs a) {
             const int I = tl.I0 + w * R + rr, J = tl.J0 + lane;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const size_t k = F.atc(2 * I + (q >> 1), 2 * J + (q & 1));
-                fa[rr][q] = F.aP[k];
-                fe[rr][q] = F.cE[k];
-                fn[rr][q] = F.cN[k];
+                const uint32_t k = F.atc(2 * I + (q >> 1), 2 * J + (q & 1));
+                fa[rr][q] = ld32(F.aP, k);
+                fe[rr][q] = ld32(F.cE, k);
+                fn[rr][q] = ld32(F.cN, k);
             }
         }
 #pragma unroll
@@ -1846,10 +1869,10 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarsen(KArgs a) {
             internal += eE(2);
             internal += eN(0);
             internal += eN(1);
-            const size_t k = C.at(I, J);
-            caP[k] = T(acc - 2.0 * internal);
-            ccE[k] = T(eE(1) + eE(3));
-            ccN[k] = T(eN(2) + eN(3));
+            const uint32_t k = C.at(I, J);
+            st32(caP, k, T(acc - 2.0 * internal));
+            st32(ccE, k, T(eE(1) + eE(3)));
+            st32(ccN, k, T(eN(2) + eN(3)));
         }
     }
 }
@@ -2044,11 +2067,11 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(KArgs a) {
         T b[NR], ap[NR], ce[NR], cn[NR];
 #pragma unroll
         for (int h = 0; h < NR; ++h) {
-            const size_t k = L.atc(ib - 2 + h, J);
-            b[h] = L.b[k];
-            ap[h] = L.aP[k];
-            ce[h] = L.cE[k];
-            cn[h] = L.cN[k];
+            const uint32_t k = L.atc(ib - 2 + h, J);
+            b[h] = ld32(L.b, k);
+            ap[h] = ld32(L.aP, k);
+            ce[h] = ld32(L.cE, k);
+            cn[h] = ld32(L.cN, k);
         }
         T x1[NR];
 #pragma unroll
@@ -2077,7 +2100,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(KArgs a) {
                             fromEast(x2[h]), cn[h], x2[h + 1]);
             const bool in = L.in(I, J);
             r[rr] = in ? b[h] - s : T(0);
-            if (in && outLane) L.xo[L.at(I, J)] = x2[h];
+            if (in && outLane) st32(L.xo, L.at(I, J), x2[h]);
         }
         // aggregate (I, J), (I, J+1), (I+1, J), (I+1, J+1) at even lanes
 #pragma unroll
@@ -2090,7 +2113,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(KArgs a) {
                 acc += e0;
                 acc += r[2 * m + 1];
                 acc += e1;
-                C.bo[C.at(Ic, Jc)] = acc;
+                st32(C.bo, C.at(Ic, Jc), acc);
             }
         }
     }
@@ -2275,13 +2298,13 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(KArgs a) {
 #pragma unroll
             for (int h = 0; h < NR; ++h) {
                 const int Icl = min(max(ib - 2 + h, 0), L.H - 1);
-                const size_t k = L.at(Icl, Jcl);
-                b[h] = L.b[k];
-                ap[h] = L.aP[k];
-                ce[h] = L.cE[k];
-                cn[h] = L.cN[k];
-                xv[h] = L.x[k];
-                zc[h] = C.z[C.at(Icl >> 1, Jcl >> 1)];
+                const uint32_t k = L.at(Icl, Jcl);
+                b[h] = ld32(L.b, k);
+                ap[h] = ld32(L.aP, k);
+                ce[h] = ld32(L.cE, k);
+                cn[h] = ld32(L.cN, k);
+                xv[h] = ld32(L.x, k);
+                zc[h] = ld32<T>(C.z, C.at(Icl >> 1, Jcl >> 1));
             }
 #pragma unroll
             for (int h = 0; h < NR; ++h) {
@@ -2310,7 +2333,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(KArgs a) {
                             fromEast(x4[h]), cn[h], x4[h + 1]);
             if (outLane && L.in(I, J)) {
                 const T z = x4[h] + om * ((b[h] - s) / ap[h]);
-                L.z[L.at(I, J)] = z;
+                st32(L.z, L.at(I, J), z);
                 v[0] += double(b[h]) * double(z);
             }
         }
