@@ -25,6 +25,10 @@ constexpr int kBlock = kTileW * kTileH;
 // level plus a two-column halo on each side (the two smoothing sweeps and the
 // residual reach two nodes out).
 constexpr int kMgTileW = kTileW - 4;
+// A multigrid tile is R rows x kMgTileCols columns: the block's kTileH waves
+// side by side, wave w on columns [J0 + w kMgTileW, J0 + (w+1) kMgTileW), so
+// that the block reads each row as one contiguous run.
+constexpr int kMgTileCols = kTileH * kMgTileW;
 constexpr int kNumPartials = 5;  // per-tile reduction partials (doubles)
 constexpr int kNumXCD = 8;       // accelerator complex dies of an MI355X
 constexpr int kMaxLevels = 12;   // aggregation-multigrid levels (incl. the finest)

@@ -607,8 +607,8 @@ void setTiling(SlotDesc& d, int tileRows, int mgRows0) {
     d.pntiles = d.ptilesX * ((d.M - 1 + pRowsPerTile - 1) / pRowsPerTile);
     for (int l = 0; l < d.nLevels; ++l) {
         MgLevel& L = d.lv[l];
-        const int rows = kTileH * (l == 0 ? mgRows0 : mgCoarseRows(d.mixed != 0));
-        L.tilesX = (L.W + kMgTileW - 1) / kMgTileW;
+        const int rows = l == 0 ? mgRows0 : mgCoarseRows(d.mixed != 0);
+        L.tilesX = (L.W + kMgTileCols - 1) / kMgTileCols;
         L.ntiles = L.tilesX * ((L.H + rows - 1) / rows);
     }
 }

@@ -1724,13 +1724,13 @@ __global__ __launch_bounds__(kBlock) void k_sor_sweep(KArgs a) {
 // 2x2 aggregates into the next level's right-hand side; the coarsest level
 // solved with a dense inverse; then, going up, the coarse correction (scaled)
 // added to the pre-smoothed iterate and two more damped-Jacobi sweeps. Each
-// level is one launch down and one up: a tile stages its 64 x 4R unknowns
-// plus a two-deep halo in LDS and recomputes the halo's sweeps, so the two
+// level is one launch down and one up: a wave holds its 60 x R unknowns plus
+// a two-deep halo in registers and recomputes the halo's sweeps, so the two
 // sweeps and the residual need no grid-wide synchronisation.
 // ---------------------------------------------------------------------------
 struct MgTile {
     int s;       // slot
-    int I0, J0;  // first unknown of the tile
+    int I0, J0;  // first unknown of this wave's part of the tile
 };
 
 template <int R>
@@ -1741,8 +1741,8 @@ __device__ __forceinline__ MgTile decodeMgTile(const KArgs& a, int t) {
     const int lt = t - L.tile0;
     const int ty = lt / L.tilesX;
     const int tx = lt - ty * L.tilesX;
-    tl.I0 = ty * (kTileH * R);
-    tl.J0 = tx * kMgTileW;
+    tl.I0 = ty * R;
+    tl.J0 = tx * kMgTileCols + int(threadIdx.x >> 6) * kMgTileW;  // this wave's first column
     return tl;
 }
 
@@ -1829,13 +1829,13 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarsen(KArgs a) {
         const auto caP = G(static_cast<T*>(d.lv[a.level].aP));
         const auto ccE = G(static_cast<T*>(d.lv[a.level].cE));
         const auto ccN = G(static_cast<T*>(d.lv[a.level].cN));
-        const int lane = int(threadIdx.x & 63), w = int(threadIdx.x >> 6);
+        const int lane = int(threadIdx.x & 63);
         // the aggregate's fine aP, cE and cN, loaded together from clamped
         // addresses; only couplings between unknowns count
         T fa[R][4], fe[R][4], fn[R][4];
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
-            const int I = tl.I0 + w * R + rr, J = tl.J0 + lane;
+            const int I = tl.I0 + rr, J = tl.J0 + lane;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const uint32_t k = F.atc(2 * I + (q >> 1), 2 * J + (q & 1));
@@ -1846,7 +1846,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarsen(KArgs a) {
         }
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
-            const int I = tl.I0 + w * R + rr, J = tl.J0 + lane;
+            const int I = tl.I0 + rr, J = tl.J0 + lane;
             if (lane >= kMgTileW || I >= C.H || J >= C.W) continue;
             const int i0 = 2 * I, j0 = 2 * J;
             // q = 0: (i0, j0), 1: (i0, j0+1), 2: (i0+1, j0), 3: (i0+1, j0+1)
@@ -2063,7 +2063,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(KArgs a) {
         const MgView<T> C(d.lv[a.level + 1]);
         const T om = T(d.mgOmega);
         const int lane = int(threadIdx.x & 63);
-        const int J = tl.J0 - 2 + lane, ib = tl.I0 + int(threadIdx.x >> 6) * R;
+        const int J = tl.J0 - 2 + lane, ib = tl.I0;
         T b[NR], ap[NR], ce[NR], cn[NR];
 #pragma unroll
         for (int h = 0; h < NR; ++h) {
@@ -2290,7 +2290,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(KArgs a) {
         const MgView<T> C(d.lv[a.level + 1]);
         const T om = T(d.mgOmega), sc = T(d.mgScale);
         const int lane = int(threadIdx.x & 63);
-        const int J = tl.J0 - 2 + lane, ib = tl.I0 + int(threadIdx.x >> 6) * R;
+        const int J = tl.J0 - 2 + lane, ib = tl.I0;
         const int Jcl = min(max(J, 0), L.W - 1);
         T b[NR], ap[NR], ce[NR], cn[NR], x3[NR];
         {
