@@ -362,7 +362,9 @@ __device__ __forceinline__ uint32_t clampVec32(int i, int j, int M, int P) {
 // and 63 receive their own, which only halo lanes consume. A DPP move with a
 // whole-wave shift (wave_shr:1 = 0x138, wave_shl:1 = 0x130) does this in the
 // vector ALU; a lane without a source keeps the old value, its own (bound_ctrl
-// off). 8-byte values move as two dwords.
+// off). 8-byte values move as two dwords. Every caller runs with the whole
+// wave active (the branches around the stencils are uniform per slot): a DPP
+// move from an inactive lane would also leave the old value.
 template <int Ctrl, typename T>
 __device__ __forceinline__ T dppWaveShift(T x) {
     if constexpr (sizeof(T) == 4) {
