@@ -134,6 +134,11 @@ struct alignas(256) SlotState {
     int dpIdx, dpFilled, dpConverged, slopeHead, slopeCount;
     int sorDone;  // SOR slots: the pressure sweeps of this outer iteration have stopped
     double dpLast;  // core-plane total-pressure drop after the last iteration
+    // The CG iterate of the fixed-budget mixed step with one refinement cycle
+    // is formed late (kPcgQFold): the alphas of the last two updates, the
+    // updates applied and those already added into x in this solve.
+    double alphaPrev;
+    int nUpd, nFold;
 };
 
 // Plane-sample request/result (samplePlaneAtX).
@@ -175,6 +180,12 @@ constexpr int kPcgQAll = 0;
 constexpr int kPcgQMg = 1;
 constexpr int kPcgQJac = 2;
 constexpr int kPcgQFirst = 4;
+// With kPcgQFold (the fixed-budget step with one refinement cycle, at an even
+// iteration k >= 2) the launch also adds the two updates before it to the CG
+// iterate, x = (x + alpha_{k-2} p_{k-2}) + alpha_{k-1} p_{k-1}, which
+// k_pcg_update leaves out in that step; k_correct adds the last one or two.
+// p_{k-1} is the launch's own p_old and p_{k-2} the buffer it overwrites.
+constexpr int kPcgQFold = 8;
 
 // KArgs::fold bits, set for the fixed-budget step of a mixed-precision batch.
 // kFoldMom: u* and v* of an implicit-momentum slot are not stored; the
@@ -183,7 +194,10 @@ constexpr int kPcgQFirst = 4;
 // launched), and the assembly's finalisation takes over the finish's
 // residuals. kFoldPc: every slot makes one refinement cycle, so p' = 0 + x
 // exactly; the correction of a CG slot reads the single-precision iterate x
-// (k_refine_accum is not launched).
+// (k_refine_accum is not launched), which is formed late: k_pcg_update
+// neither reads nor writes x or p_k, k_pcg_q adds two updates at a time
+// (kPcgQFold) and the correction the last one or two, in the order of the
+// per-iteration updates, so p' is the same bit for bit.
 constexpr int kFoldMom = 1;
 constexpr int kFoldPc = 2;
 // Partial entries of the folded pressure assembly: the finish's max |u* - u|,

@@ -355,7 +355,8 @@ int launch(cp_batch* b, Kern k, int grid) {
 // first iteration of a CG (every running slot's pk is 0) or a later one;
 // in double precision one launch for every slot. With `force` (timing
 // bursts) the later-iteration forms, on the given stream.
-int launchPcgQ(cp_batch* b, int grid, bool first, bool force = false, hipStream_t stream = nullptr) {
+// fold: the form that also adds the two updates before it to x (kPcgQFold)
+int launchPcgQ(cp_batch* b, int grid, bool first, bool force = false, hipStream_t stream = nullptr, bool fold = false) {
     const bool f32 = b->precision == CP_PREC_MIXED;
     hipStream_t s = stream ? stream : b->stream;
     KArgs a = kargsFor(b, K_PCG_Q);
@@ -364,7 +365,7 @@ int launchPcgQ(cp_batch* b, int grid, bool first, bool force = false, hipStream_
     const int vars[2] = {b->anyMgCG ? kPcgQMg : -1, b->anyJacCG ? kPcgQJac : -1};
     for (int v : vars) {
         if (f32 && v < 0) continue;
-        a.pvar = f32 ? (v | (first ? kPcgQFirst : 0)) : kPcgQAll;
+        a.pvar = f32 ? (v | (first ? kPcgQFirst : 0) | (fold ? kPcgQFold : 0)) : kPcgQAll;
         const hipError_t e = launchWorkOnly(K_PCG_Q, f32, rows, grid, s, a);
         if (e != hipSuccess) return setError(&b->err, CP_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
         if (!f32) break;
@@ -1127,7 +1128,10 @@ int enqueueFixedStepImpl(cp_batch* b, int J, int P, int R, int grid) {
     for (int c = 0; c < R; ++c) {
         bool updateUnfinalised = false;
         for (int k = 0; k < P; ++k) {
-            if ((rc = launchPcgQ(b, grid, k == 0))) return rc;
+            // with one refinement cycle x is formed late: every even
+            // iteration from the third adds the two updates before it
+            const bool foldX = (b->fold & kFoldPc) != 0 && k >= 2 && (k & 1) == 0;
+            if ((rc = launchPcgQ(b, grid, k == 0, false, nullptr, foldX))) return rc;
             // the V-cycle after the last iteration would only feed a search
             // direction that is never used; before it, the update's stop
             // test is finalised together with the V-cycle's r.z
@@ -1150,7 +1154,8 @@ int enqueueFixedStepImpl(cp_batch* b, int J, int P, int R, int grid) {
         }
         if (b->precision != CP_PREC_MIXED) break;
         if (b->fold & kFoldPc) {
-            // p' = 0 + x is read by the correction from x itself: the
+            // p' = 0 + x is read by the correction from x itself (with the
+            // CG updates k_pcg_q has not added to it yet): the
             // accumulation's bookkeeping only
             const hipError_t e =
                 launchFinalize(updateUnfinalised ? PH_PCGU_REFACC : PH_REF_ACC, b->maxSlots, b->stream, kargs(b));

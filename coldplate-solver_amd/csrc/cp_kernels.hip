@@ -96,17 +96,21 @@ __device__ __forceinline__ Vec<T, C> ldvH(gp<const T> base, uint32_t k) {
     else return ldv<C>(base, k);
 }
 
-template <int C, typename T>
+template <int C, bool NT = false, typename T>
 __device__ __forceinline__ void stMaskedV(gp<T> base, uint32_t k, Vec<T, C> v, const bool (&m)[C]) {
     bool all = true;
 #pragma unroll
     for (int c = 0; c < C; ++c) all &= m[c];
     if (all) {
-        stv<C>(base, k, v);
+        if constexpr (NT) stvNT<C>(base, k, v);
+        else stv<C>(base, k, v);
     } else {
 #pragma unroll
         for (int c = 0; c < C; ++c)
-            if (m[c]) st32(base, k + c, v[c]);
+            if (m[c]) {
+                if constexpr (NT) st32NT(base, k + c, v[c]);
+                else st32(base, k + c, v[c]);
+            }
     }
 }
 // The same for two arrays at the same index under one branch: two separate
@@ -1364,8 +1368,9 @@ __device__ __forceinline__ T pLoad(gp<const T> r, gp<const T> aP, gp<const T> Z,
 // rows and for the rows above and below them (lanes 0-31 also at column
 // j0-1, lanes 32-63 at column j0+64C), then marches down the rows taking
 // the neighbours across lanes from the adjacent lanes.
-template <typename T, int R, int C, bool HasZ, bool First>
-__device__ __forceinline__ void pcgQWave(const SlotDesc& d, const Tile& tl, int kk, T beta, double& v) {
+template <typename T, int R, int C, bool HasZ, bool First, bool Fold = false>
+__device__ __forceinline__ void pcgQWave(const SlotDesc& d, const Tile& tl, int kk, T beta, double& v, T alphaA = T(0),
+                                         T alphaB = T(0)) {
     const int M = d.M, N = d.N, P = d.pitch;
     const auto r = G(static_cast<const T*>(d.pr[P_R]));
     const auto aP = G(static_cast<const T*>(d.pr[P_AP]));
@@ -1385,6 +1390,18 @@ __device__ __forceinline__ void pcgQWave(const SlotDesc& d, const Tile& tl, int 
     constexpr bool nt = sizeof(T) == 4;
     Vec<T, C> p[R + 2], ap[R], ce[R], cn[R + 1];
     T pe[R + 2], cee[R];
+    // Fold: the rows' p_{k-2} (in the buffer p_k replaces) and x, loaded
+    // with the rest; x takes the two updates as p_{k-1} arrives
+    [[maybe_unused]] Vec<T, C> fx[Fold ? R : 1], fp2[Fold ? R : 1];
+    const auto X = G(static_cast<T*>(d.pr[P_X]));
+    if constexpr (Fold) {
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+            const uint32_t kv = clampVec32<C>(ib + rr, j, M, P);
+            fx[rr] = ldvH<C, nt, T>(X, kv);
+            fp2[rr] = ldvH<C, nt, T>(pnew, kv);
+        }
+    }
 #pragma unroll
     for (int h = 0; h < R + 2; ++h) {
         const int i = ib - 1 + h;
@@ -1399,6 +1416,15 @@ __device__ __forceinline__ void pcgQWave(const SlotDesc& d, const Tile& tl, int 
         }
         if (!First) {
             const Vec<T, C> po = ldvH<C, nt>(pold, kv);
+            if constexpr (Fold)
+                if (h >= 1 && h <= R) {
+                    // the two per-iteration updates x += alpha p, in their order
+#pragma unroll
+                    for (int c = 0; c < C; ++c) {
+                        const T x1 = fx[h - 1][c] + alphaA * fp2[h - 1][c];
+                        fx[h - 1][c] = x1 + alphaB * po[c];
+                    }
+                }
 #pragma unroll
             for (int c = 0; c < C; ++c) z[c] = z[c] + beta * po[c];
         }
@@ -1436,6 +1462,7 @@ __device__ __forceinline__ void pcgQWave(const SlotDesc& d, const Tile& tl, int 
             dm[c] = isPDof(i, j + c, M, N);
         }
         const uint32_t k = uint32_t(max(i, 0)) * uint32_t(P) + uint32_t(j);
+        if constexpr (Fold) stMaskedV<C, nt>(X, k, fx[rr], dm);
         stMaskedV2<C, nt>(pnew, pC, Q, q, k, dm);
 #pragma unroll
         for (int c = 0; c < C; ++c)
@@ -1443,8 +1470,10 @@ __device__ __forceinline__ void pcgQWave(const SlotDesc& d, const Tile& tl, int 
     }
 }
 
+// The fold form holds x and p_{k-2} of its rows as well: three waves per SIMD
+// at most 168 VGPRs (without the hint it keeps 182-194, two waves).
 template <typename T, int R, int V>
-__global__ __launch_bounds__(kBlock) void k_pcg_q(KArgs a) {
+__global__ __launch_bounds__(kBlock, (V & kPcgQFold) ? 3 : 1) void k_pcg_q(KArgs a) {
     const int nT = *a.nTiles;
     int nred = 0;  // block reductions made so far (alternating buffers)
     constexpr int C = pColsOf(sizeof(T) == 4);
@@ -1473,9 +1502,20 @@ __global__ __launch_bounds__(kBlock) void k_pcg_q(KArgs a) {
             // sequence never holds, gets a non-finite p.q, so that its CG
             // fails loudly instead of taking a wrong search direction
             constexpr bool mg = (V & kPcgQMg) != 0, first = (V & kPcgQFirst) != 0;
+            constexpr bool fold = (V & kPcgQFold) != 0;
             if ((d.precond != 0) != mg) continue;
-            if ((kk == 0) == first) pcgQWave<T, R, C, mg, first>(d, tl, kk, beta, v[0]);
-            else v[0] = __builtin_nan("");
+            if constexpr (fold) {
+                if ((kk >= 2) & ((kk & 1) == 0)) {
+                    pcgQWave<T, R, C, mg, false, true>(d, tl, kk, beta, v[0], T(st.alphaPrev), T(st.alpha));
+                    if (t == d.ptile0 && threadIdx.x == 0) st.nFold = kk;
+                } else {
+                    v[0] = __builtin_nan("");
+                }
+            } else if ((kk == 0) == first) {
+                pcgQWave<T, R, C, mg, first>(d, tl, kk, beta, v[0]);
+            } else {
+                v[0] = __builtin_nan("");
+            }
         }
         blockReduceTo<1, 0u>(v, a.partials + size_t(t) * kNumPartials + kPcgQPartial, nred);
     }
@@ -1515,16 +1555,25 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
         // preconditioner, whose beta follows the V-cycle: a multigrid slot
         // neither loads aP nor divides
         const bool jacobiZ = d.precond == 0;
+        // the fixed-budget step with one refinement cycle forms x late
+        // (kPcgQFold): only r changes here
+        const bool lateX = (sizeof(T) == 4) & ((a.fold & kFoldPc) != 0);
         // all loads first (column vectors, clamped addresses); stores only
         // on unknowns
         // single precision: every array is touched once (non-temporal)
         constexpr bool nt = sizeof(T) == 4;
         Vec<T, C> xs[R], ps[R], rs[R], qs[R], as[R];
+        if (!lateX) {
+#pragma unroll
+            for (int rr = 0; rr < R; ++rr) {
+                const uint32_t k = clampVec32<C>(tl.i + rr, j, M, P);
+                xs[rr] = ldvH<C, nt, T>(x, k);
+                ps[rr] = ldvH<C, nt>(pk, k);
+            }
+        }
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
             const uint32_t k = clampVec32<C>(tl.i + rr, j, M, P);
-            xs[rr] = ldvH<C, nt, T>(x, k);
-            ps[rr] = ldvH<C, nt>(pk, k);
             rs[rr] = ldvH<C, nt, T>(r, k);
             qs[rr] = ldvH<C, nt>(Q, k);
         }
@@ -1543,21 +1592,31 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(KArgs a) {
             continue;
         }
         const double alphaD = rz / pq[0];
-        if (t == d.ptile0 && threadIdx.x == 0) st.alpha = alphaD;
+        if (t == d.ptile0 && threadIdx.x == 0) {
+            st.alphaPrev = st.alpha;
+            st.alpha = alphaD;
+            st.nUpd = kk + 1;
+        }
         const T alpha = T(alphaD);
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
             const int i = tl.i + rr;
             bool dm[C];
-            Vec<T, C> xn, rn;
+            Vec<T, C> rn;
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 dm[c] = isPDof(i, j + c, M, N);
-                xn[c] = xs[rr][c] + alpha * ps[rr][c];
                 rn[c] = rs[rr][c] - alpha * qs[rr][c];
             }
             const uint32_t k = uint32_t(max(i, 0)) * uint32_t(P) + uint32_t(j);
-            stMaskedV2<C, nt>(x, xn, r, rn, k, dm);
+            if (lateX) {
+                stMaskedV<C, nt>(r, k, rn, dm);
+            } else {
+                Vec<T, C> xn;
+#pragma unroll
+                for (int c = 0; c < C; ++c) xn[c] = xs[rr][c] + alpha * ps[rr][c];
+                stMaskedV2<C, nt>(x, xn, r, rn, k, dm);
+            }
 #pragma unroll
             for (int c = 0; c < C; ++c)
                 if (dm[c]) v[0] += double(rn[c]) * double(rn[c]);
@@ -2453,12 +2512,31 @@ __global__ __launch_bounds__(kBlock) void k_correct(KArgs a) {
         }
         if (pcFromX) {
             const auto x = G(static_cast<const float*>(d.pr[P_X]));
+            // the updates k_pcg_q has not added to x yet: p_{nFold} and, with
+            // two pending, p_{nFold+1}, in the order of the per-iteration
+            // updates x += alpha p
+            const int nF = st.nFold, pend = st.nUpd - nF;
+            const float aA = float(pend == 2 ? st.alphaPrev : st.alpha), aB = float(st.alpha);
+            const auto pA = G(static_cast<const float*>(d.pr[P_PK0 + ((nF & 1) ^ 1)]));
+            const auto pB = G(static_cast<const float*>(d.pr[P_PK0 + (((nF + 1) & 1) ^ 1)]));
 #pragma unroll
             for (int rr = 0; rr < R; ++rr) {
                 const int i = tl.i + rr;
-                pc0[rr] = x[clampIdx(i, j, M, P)];
-                pcE[rr] = x[clampIdx(i, j + 1, M, P)];
-                pcN[rr] = x[clampIdx(i + 1, j, M, P)];
+                const size_t k0 = clampIdx(i, j, M, P), kE = clampIdx(i, j + 1, M, P), kN = clampIdx(i + 1, j, M, P);
+                float x0 = x[k0], xE = x[kE], xN = x[kN];
+                if (pend >= 1) {
+                    x0 = x0 + aA * pA[k0];
+                    xE = xE + aA * pA[kE];
+                    xN = xN + aA * pA[kN];
+                }
+                if (pend == 2) {
+                    x0 = x0 + aB * pB[k0];
+                    xE = xE + aB * pB[kE];
+                    xN = xN + aB * pB[kN];
+                }
+                pc0[rr] = x0;
+                pcE[rr] = xE;
+                pcN[rr] = xN;
             }
         } else {
             const auto pc = G(d.pc);
@@ -2912,6 +2990,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
                 st.piters = 0;
                 st.pitTotal = 0;
                 st.pcycle = 0;
+                st.nUpd = 0;
+                st.nFold = 0;
                 if (d.presType == 0) {
                     // SOR slot: the CG, multigrid and refinement kernels skip it
                     st.pdone = 1;
@@ -2961,6 +3041,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(KArgs a, int phase, int out
                     st.pk = 0;
                     st.piters = 0;
                     st.pdone = 0;
+                    st.nUpd = 0;
+                    st.nFold = 0;
                 }
             }
             break;
@@ -3013,6 +3095,12 @@ static void launchTR(Kern k, int grid, hipStream_t s, const KArgs& a) {
                     case kPcgQJac: hipLaunchKernelGGL((k_pcg_q<T, R, kPcgQJac>), g, b, 0, s, a); break;
                     case kPcgQJac | kPcgQFirst:
                         hipLaunchKernelGGL((k_pcg_q<T, R, kPcgQJac | kPcgQFirst>), g, b, 0, s, a);
+                        break;
+                    case kPcgQMg | kPcgQFold:
+                        hipLaunchKernelGGL((k_pcg_q<T, R, kPcgQMg | kPcgQFold>), g, b, 0, s, a);
+                        break;
+                    case kPcgQJac | kPcgQFold:
+                        hipLaunchKernelGGL((k_pcg_q<T, R, kPcgQJac | kPcgQFold>), g, b, 0, s, a);
                         break;
                     default: hipLaunchKernelGGL((k_pcg_q<T, R, kPcgQAll>), g, b, 0, s, a); break;
                 }
