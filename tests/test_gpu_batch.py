@@ -256,35 +256,47 @@ def test_small_sweep_budgets_match_oracle(sweeps):
 
 
 @pytest.mark.parametrize(
-    "cycles,precond,sweeps,mom,pres",
-    [(1, cb.CP_PRECOND_AMG, 3, 1, 1), (2, cb.CP_PRECOND_AMG, 3, 1, 1), (2, cb.CP_PRECOND_JACOBI, 3, 1, 1),
-     (1, cb.CP_PRECOND_JACOBI, 3, 1, 1), (1, cb.CP_PRECOND_AMG, 1, 1, 1), (1, cb.CP_PRECOND_AMG, 2, 1, 1),
-     (1, cb.CP_PRECOND_AMG, 3, 0, 1), (1, cb.CP_PRECOND_AMG, 3, 1, 0)],
-    ids=["amg-1", "amg-2", "jacobi-2", "jacobi-1", "amg-1-sweeps1", "amg-1-sweeps2", "explicit-cg", "implicit-sor"])
-def test_graph_batch_matches_polled_mixed(cycles, precond, sweeps, mom, pres):
-    """Mixed precision, 4 CG iterations per refinement cycle: the graph step
-    equals the polled context bit for bit and with the same inner counts.
-    The graph step merges finalisations where a later one can carry them,
-    forms u* and v* of an implicit-momentum slot in the pressure assembly
-    and the correction instead of storing them, and with one refinement
-    cycle reads p' as the CG iterate itself; the context finalises every
-    phase on its own and stores u*, v* and p'. The cases cover both
-    preconditioners (each its own form of the CG stencil), one and two
-    refinement cycles, momentum budgets ending after the assembly's sweep,
-    on an even and on an odd sweep, and the explicit-momentum and SOR
-    backends, which the graph step leaves unfolded."""
+    "cycles,precond,sweeps,mom,pres,pcg,ptol",
+    [(1, cb.CP_PRECOND_AMG, 3, 1, 1, 4, None), (2, cb.CP_PRECOND_AMG, 3, 1, 1, 4, None),
+     (2, cb.CP_PRECOND_JACOBI, 3, 1, 1, 4, None), (1, cb.CP_PRECOND_JACOBI, 3, 1, 1, 4, None),
+     (1, cb.CP_PRECOND_AMG, 1, 1, 1, 4, None), (1, cb.CP_PRECOND_AMG, 2, 1, 1, 4, None),
+     (1, cb.CP_PRECOND_AMG, 3, 0, 1, 4, None), (1, cb.CP_PRECOND_AMG, 3, 1, 0, 4, None),
+     (1, cb.CP_PRECOND_AMG, 3, 1, 1, 2, None), (1, cb.CP_PRECOND_JACOBI, 3, 1, 1, 3, None),
+     (1, cb.CP_PRECOND_AMG, 3, 1, 1, 5, 0.02), (1, cb.CP_PRECOND_AMG, 3, 1, 1, 5, 0.012)],
+    ids=["amg-1", "amg-2", "jacobi-2", "jacobi-1", "amg-1-sweeps1", "amg-1-sweeps2", "explicit-cg", "implicit-sor",
+         "amg-1-cg2", "jacobi-1-cg3", "amg-1-cg5-stops", "amg-1-cg5-stop1"])
+def test_graph_batch_matches_polled_mixed(cycles, precond, sweeps, mom, pres, pcg, ptol):
+    """Mixed precision, `pcg` CG iterations per refinement cycle: the graph
+    step equals the polled context bit for bit and with the same inner
+    counts. The graph step merges finalisations where a later one can carry
+    them, forms u* and v* of an implicit-momentum slot in the pressure
+    assembly and the correction instead of storing them, and with one
+    refinement cycle reads p' as the CG iterate itself, which it forms late
+    (k_pcg_q adds two updates at every even iteration from the third, the
+    correction the one or two still pending); the context finalises every
+    phase on its own, updates x every iteration and stores u*, v* and p'.
+    The cases cover both preconditioners (each its own form of the CG
+    stencil), one and two refinement cycles, momentum budgets ending after
+    the assembly's sweep, on an even and on an odd sweep, the explicit-
+    momentum and SOR backends, which the graph step leaves unfolded, CG
+    budgets of 2, 3, 4 and 5 iterations, and a loose CG tolerance under
+    which solves stop inside the budget (after 1 and after 3 counted
+    iterations: two updates pending, none or two of them added by k_pcg_q)."""
     p, ct = wl.network(64, 128, 3, cb.CP_PREC_MIXED)
-    q = wl.fast_mode(p, sweeps, 4, cycles)
+    q = wl.fast_mode(p, sweeps, pcg, cycles)
     q.pressurePrecond = precond
     q.momentumSolverType, q.pressureSolverType = mom, pres
+    if ptol is not None:
+        q.pTol = ptol
     if pres == 0:
         q.maxPressureIter = 20
-    say(f"[batch] mixed, {cycles} cycles, precond {precond}, {sweeps} sweeps, momentum {mom}, pressure {pres}: "
-        "graph vs polled")
+    say(f"[batch] mixed, {cycles} cycles, precond {precond}, {sweeps} sweeps, momentum {mom}, pressure {pres}, "
+        f"{pcg} CG iterations, pTol {q.pTol}: graph vs polled")
     b = cb.Batch(cb.CP_PREC_MIXED, 64 << 20, 1)
     b.set_tiling(1)  # the context's tiling for a 64 x 128 grid
     s = b.admit(q, ct)
     g = cb.Solver(q, ct)
+    its = []
     for step in range(15):
         b.run(1)
         sg = g.step()
@@ -293,6 +305,10 @@ def test_graph_batch_matches_polled_mixed(cycles, precond, sweeps, mom, pres):
             assert info[k] == sg[k], (step, k, info[k], sg[k])
         for k in ("error", "residU", "residV", "residMass"):
             assert info[k] == sg[k], (step, k, info[k], sg[k])
+        its.append(info["itP"])
+    if ptol is not None:
+        # the CG stopped inside its budget in some step (the case this covers)
+        assert any(0 < x < pcg for x in its), its
     u, v, pp = b.fields(s)
     np.testing.assert_array_equal(u, g.field(cb.F_U))
     np.testing.assert_array_equal(v, g.field(cb.F_V))
