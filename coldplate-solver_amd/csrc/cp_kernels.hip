@@ -1367,7 +1367,9 @@ __device__ __forceinline__ T pLoad(gp<const T> r, gp<const T> aP, gp<const T> Z,
 // lane (one vector access per array and row). The wave loads p_k for its
 // rows and for the rows above and below them (lanes 0-31 also at column
 // j0-1, lanes 32-63 at column j0+64C), then marches down the rows taking
-// the neighbours across lanes from the adjacent lanes.
+// the neighbours across lanes from the adjacent lanes. Fold (kPcgQFold): the
+// rows' CG iterate also takes the two updates before this iteration,
+// x = (x + alphaA p_{k-2}) + alphaB p_{k-1}.
 template <typename T, int R, int C, bool HasZ, bool First, bool Fold = false>
 __device__ __forceinline__ void pcgQWave(const SlotDesc& d, const Tile& tl, int kk, T beta, double& v, T alphaA = T(0),
                                          T alphaB = T(0)) {
@@ -1393,7 +1395,7 @@ __device__ __forceinline__ void pcgQWave(const SlotDesc& d, const Tile& tl, int 
     // Fold: the rows' p_{k-2} (in the buffer p_k replaces) and x, loaded
     // with the rest; x takes the two updates as p_{k-1} arrives
     [[maybe_unused]] Vec<T, C> fx[Fold ? R : 1], fp2[Fold ? R : 1];
-    const auto X = G(static_cast<T*>(d.pr[P_X]));
+    [[maybe_unused]] const auto X = G(static_cast<T*>(d.pr[P_X]));
     if constexpr (Fold) {
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
