@@ -105,7 +105,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=6)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=None, help="slots per batch (default 16; 8 for --workload stream)")
+    ap.add_argument("--batch", type=int, default=None, help="slots per batch (default 20; 8 for --workload stream)")
     ap.add_argument("--streams", type=int, default=None,
                     help="batches served at once (cp_serve_pool), each on its own HIP stream (default 4)")
     ap.add_argument("--requests", type=int, default=None,
@@ -594,7 +594,7 @@ def main() -> int:
         return 0
     if a.calibrate_copy:
         copy_calibration(a.calibrate_copy)
-    a.batch = a.batch or 16
+    a.batch = a.batch or 20
     a.streams = a.streams or 4
     if a.streams < 1:
         raise SystemExit("--streams must be at least 1")
