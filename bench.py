@@ -124,8 +124,9 @@ def parse():
     ap.add_argument("--rate", type=float, default=2.0, help="served-stream subsample: Poisson arrival rate (1/s)")
     ap.add_argument("--stream-requests", type=int, default=48,
                     help="requests of the served-stream subsample (0: skip it)")
-    ap.add_argument("--latency-requests", type=int, default=64,
-                    help="requests of the Poisson latency run of the headline workload (0: skip it)")
+    ap.add_argument("--latency-requests", type=int, default=None,
+                    help="requests of the Poisson latency run of the headline workload (default: twice the slots, "
+                         "so that more requests than slots pass through; 0: skip it)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dropin", action="store_true")
@@ -600,6 +601,8 @@ def main() -> int:
         raise SystemExit("--streams must be at least 1")
     slots = a.batch * a.streams
     a.requests = a.requests or 2 * slots
+    if a.latency_requests is None:
+        a.latency_requests = 2 * slots
     refs = references()
     GEOMS[:] = [wl.network_1024x512(s) for s in range(N_SEEDS)]
     probe_b = cb.Batch(cb.CP_PREC_MIXED, 1 << 20, 1)
